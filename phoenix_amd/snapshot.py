@@ -1,0 +1,540 @@
+"""Seeded synthetic BHAC snapshots: a Fishbone-Moncrief torus on an MKS-BHAC AMR grid.
+
+The files written here are byte-compatible with what the reference's BHAC reader
+expects (``src/host/io/bhac_io.hpp:331-499`` for the dump, ``:143-174`` for the
+grid file):
+
+* leaf blocks in forest order, each ``nwini x cells`` float64 (variable-major,
+  cell = i + nx0*(j + nx1*k)) followed by an ignored staggered block of
+  ``(nx0+1)(nx1+1)(nx2+1) x nws`` float64;
+* the forest: one pre-order tree of int32 ``is_leaf`` flags per base block,
+  base blocks in Morton order of (i, j, k) when ``sfc`` (``bhac_io.hpp:392-416``),
+  children ordered (2i + c%2, 2j + (c/2)%2, 2k + c/4) (``bhac_io.hpp:39-66``);
+* the tail header: nx[3] int32, neqpar[] float64, then
+  {nleafs, levmax, ndim, ndir, nwini, nws, neqparini, it} int32 + t float64.
+
+Conserved slots follow ``src/core/model/bhac.hpp:26-36`` and ``:257-346``:
+D, S_1..S_3 (covariant), tau, B^1..B^3, D*s, Lorentz factor, xi = Gamma^2 rho h.
+Primitives are encoded with the metric at the sqrt(gamma)-weighted cell centroid
+(the point ``convert2prim`` decodes at), so decoding returns them to round-off.
+
+Physics of the torus (Fishbone & Moncrief 1976, ApJ 207, 962; the constant
+specific angular momentum l = u_phi u^t fixed by the pressure maximum):
+rho normalised to 1 at (r_max, pi/2), u = K rho^gamma with K = 0.5 (the FM
+enthalpy value K = hm1_max (gamma-1)/gamma ~ 0.005 leaves the torus too cold to
+emit at 230 GHz; K = 0.5 gives ~3 Jy at m_unit 1.2e18 for Sgr A*'s mass and
+distance), a seeded uniform +-2% factor on u, one poloidal loop
+A_phi = max(rho - 0.2, 0) scaled so that min(beta) = 100 over the cells, floors
+rho >= 1e-4 r^-1.5, u >= 1e-6 r^-2.5.
+"""
+from __future__ import annotations
+
+import dataclasses
+import hashlib
+import math
+import os
+import struct
+from typing import Sequence
+
+import numpy as np
+
+NWINI = 11
+NWS = 3
+NPRIM = 8
+
+
+@dataclasses.dataclass(frozen=True)
+class SnapshotSpec:
+    """Everything that determines a synthetic snapshot (and hence its bytes)."""
+
+    spin: float = 0.9375
+    gamma: float = 13.0 / 9.0
+    r_in: float = 6.0
+    r_max: float = 12.0
+    beta_min: float = 100.0
+    aphi_cut: float = 0.2
+    seed: int = 1
+    nxlone: tuple = (48, 24, 24)
+    block: tuple = (12, 12, 12)
+    refine: tuple = (0.1,)          # density threshold to go from level L to L+1
+    r_lo: float = 1.2
+    r_hi: float = 1000.0
+    hslope: float = 0.3
+    nspin: int = 3
+    sfc: bool = True
+    pert: float = 0.02
+    u_K: float = 0.5                # u = u_K rho^gamma (0 selects the FM enthalpy value, far too cold to emit)
+
+    def grid_values(self):
+        return (self.nxlone, (math.log(self.r_lo), 0.0, 0.0),
+                (math.log(self.r_hi), 0.5, 1.0), self.hslope)
+
+
+# Named workloads.  "c1": 48x24x24 in 12^3 blocks + 1 level (rho > 0.1);
+# "c2": 96x48x48 in 12^3 blocks + 2 levels (rho > 0.1, rho > 0.5);
+# "c5": 128x64x64 in 16^3 blocks + 2 levels (rho > 0.1, rho > 0.5), spin 0.9375 / 0.5.
+PRESETS = {
+    "c1": SnapshotSpec(),
+    "c1_a05": SnapshotSpec(spin=0.5, seed=2),
+    "c2": SnapshotSpec(nxlone=(96, 48, 48), refine=(0.1, 0.5)),
+    "c2_a05": SnapshotSpec(spin=0.5, seed=2, nxlone=(96, 48, 48), refine=(0.1, 0.5)),
+    "c5": SnapshotSpec(nxlone=(128, 64, 64), block=(16, 16, 16), refine=(0.1, 0.5)),
+    "c5_a05": SnapshotSpec(spin=0.5, seed=2, nxlone=(128, 64, 64), block=(16, 16, 16),
+                           refine=(0.1, 0.5)),
+    # a tiny one for unit tests: 24x12x12 in 6^3 blocks + 1 level
+    "tiny": SnapshotSpec(nxlone=(24, 12, 12), block=(6, 6, 6), refine=(0.1,)),
+}
+
+
+# --------------------------------------------------------------------------- metric
+def theta_of_x2(x2, h):
+    return x2 + 0.5 * h * np.sin(2.0 * x2)
+
+
+def metric_dd(a, h, x1, x2):
+    """Covariant Kerr-Schild metric in MKS-BHAC coordinates (t, ln r, x2, phi).
+
+    Returns a dict of the non-zero components.  Same geometry as
+    ``src/core/geometry/metric.hpp:26-54``.
+    """
+    r = np.exp(x1)
+    th = theta_of_x2(x2, h)
+    s, c = np.sin(th), np.cos(th)
+    s2 = s * s
+    sig = r * r + a * a * c * c
+    dthdx2 = 1.0 + h * np.cos(2.0 * x2)
+    z = 2.0 * r / sig
+    g = {}
+    g["00"] = -(1.0 - z)
+    g["01"] = z * r
+    g["03"] = -z * a * s2
+    g["11"] = (1.0 + z) * r * r
+    g["13"] = -a * s2 * (1.0 + z) * r
+    g["22"] = sig * dthdx2 * dthdx2
+    g["33"] = s2 * (sig + a * a * s2 * (1.0 + z))
+    return g
+
+
+def metric_uu(a, h, x1, x2):
+    """Contravariant KS metric in MKS-BHAC coordinates (``metric.hpp:64-91``)."""
+    r = np.exp(x1)
+    th = theta_of_x2(x2, h)
+    s, c = np.sin(th), np.cos(th)
+    sig = r * r + a * a * c * c
+    dthdx2 = 1.0 + h * np.cos(2.0 * x2)
+    g = {}
+    g["00"] = -(1.0 + 2.0 * r / sig)
+    g["01"] = 2.0 / sig
+    g["11"] = (r * (r - 2.0) + a * a) / (sig * r * r)
+    g["13"] = a / (sig * r)
+    g["22"] = 1.0 / (sig * dthdx2 * dthdx2)
+    g["33"] = 1.0 / (sig * s * s)
+    return g
+
+
+def spatial(gdd):
+    """3x3 spatial block gamma_ij (indices 1..3) as an array [..., 3, 3]."""
+    shape = np.broadcast(gdd["11"], gdd["22"]).shape
+    m = np.zeros(shape + (3, 3))
+    m[..., 0, 0] = gdd["11"]
+    m[..., 1, 1] = gdd["22"]
+    m[..., 2, 2] = gdd["33"]
+    m[..., 0, 2] = m[..., 2, 0] = gdd["13"]
+    return m
+
+
+def sqrt_detgamma(a, h, x1, x2):
+    g = metric_dd(a, h, x1, x2)
+    det = g["11"] * (g["22"] * g["33"]) - g["13"] * (g["22"] * g["13"])
+    return np.sqrt(det)
+
+
+def centroid(a, h, X, dxc):
+    """sqrt(gamma)-weighted 27-point Simpson centroid of cells centred at X [N,3]
+    (the point ``convert2prim`` evaluates the metric at, ``bhac.hpp:202-235``)."""
+    w1 = np.array([1.0, 4.0, 1.0])
+    off = np.array([-1.0, 0.0, 1.0])
+    norm = np.zeros(X.shape[0])
+    acc = np.zeros_like(X)
+    for k in range(3):
+        for j in range(3):
+            for i in range(3):
+                w = w1[i] * w1[j] * w1[k]
+                p = X + 0.5 * dxc * np.array([off[i], off[j], off[k]])
+                dg = sqrt_detgamma(a, h, p[:, 0], p[:, 1]) * w
+                norm += dg
+                acc += dg[:, None] * p
+    return acc / norm[:, None]
+
+
+# --------------------------------------------------------------------------- torus
+def _fm_l(a, r):
+    """FM angular momentum l = u_phi u^t of the torus whose pressure peaks at r."""
+    sr = math.sqrt(r)
+    num = ((a * a - 2.0 * a * sr + r * r)
+           * ((-2.0 * a * r * (a * a - 2.0 * a * sr + r * r)) / math.sqrt(2.0 * a * sr + (r - 3.0) * r)
+              + ((a + (r - 2.0) * sr) * (r ** 3 + a * a * (2.0 + r))) / math.sqrt(1.0 + 2.0 * a / r ** 1.5 - 3.0 / r)))
+    den = r ** 3 * math.sqrt(2.0 * a * sr + (r - 3.0) * r) * (a * a + (r - 2.0) * r)
+    return num / den
+
+
+@np.errstate(invalid="ignore", divide="ignore")
+def _fm_potential(a, l, r, th):
+    s = np.sin(th)
+    c = np.cos(th)
+    dd = r * r - 2.0 * r + a * a
+    aa = (r * r + a * a) ** 2 - dd * a * a * s * s
+    ss = r * r + a * a * c * c
+    q = np.sqrt(1.0 + 4.0 * l * l * ss * ss * dd / (aa * aa * s * s))
+    return 0.5 * np.log((1.0 + q) / (ss * dd / aa)) - 0.5 * q - 2.0 * a * r * l / aa
+
+
+class Torus:
+    def __init__(self, spec: SnapshotSpec):
+        self.a = spec.spin
+        self.gam = spec.gamma
+        self.r_in = spec.r_in
+        self.l = _fm_l(self.a, spec.r_max)
+        self.w_in = float(_fm_potential(self.a, self.l, np.array(spec.r_in), np.array(0.5 * math.pi)))
+        hm1_max = math.expm1(float(_fm_potential(self.a, self.l, np.array(spec.r_max),
+                                                 np.array(0.5 * math.pi))) - self.w_in)
+        self.hm1_max = hm1_max
+        # rho = (hm1 (g-1) / (kappa g))^(1/(g-1)) normalised at the maximum -> kappa drops out
+        self.K = spec.u_K if spec.u_K > 0.0 else hm1_max * (self.gam - 1.0) / self.gam
+
+    def lnh(self, r, th):
+        out = _fm_potential(self.a, self.l, r, th) - self.w_in
+        return np.where(r > self.r_in, out, -1.0)
+
+    @np.errstate(invalid="ignore", over="ignore")
+    def rho(self, r, th):
+        """Normalised torus density (0 outside the torus)."""
+        lnh = self.lnh(r, th)
+        inside = lnh > 0.0
+        hm1 = np.expm1(np.where(inside, lnh, 0.0))
+        return np.where(inside, (hm1 / self.hm1_max) ** (1.0 / (self.gam - 1.0)), 0.0)
+
+    @np.errstate(invalid="ignore", divide="ignore")
+    def uphi(self, r, th):
+        """Boyer-Lindquist u^phi of the torus (= Kerr-Schild u^phi since u^r = 0)."""
+        a, l = self.a, self.l
+        s, c = np.sin(th), np.cos(th)
+        dd = r * r - 2.0 * r + a * a
+        aa = (r * r + a * a) ** 2 - dd * a * a * s * s
+        ss = r * r + a * a * c * c
+        em2chi = ss * ss * dd / (aa * aa * s * s)
+        up1 = np.sqrt((-1.0 + np.sqrt(1.0 + 4.0 * l * l * em2chi)) / 2.0)
+        return 2.0 * a * r * np.sqrt(1.0 + up1 * up1) / np.sqrt(aa * ss * dd) + np.sqrt(ss / aa) * up1 / s
+
+
+# --------------------------------------------------------------------------- forest
+def morton3(i, j, k):
+    code = 0
+    for b in range(21):
+        code |= ((i >> b) & 1) << (3 * b) | ((j >> b) & 1) << (3 * b + 1) | ((k >> b) & 1) << (3 * b + 2)
+    return code
+
+
+class Layout:
+    """Block geometry of the forest (``bhac_io.hpp:436-453``)."""
+
+    def __init__(self, spec: SnapshotSpec):
+        self.spec = spec
+        nxlone, xmin, xmax, _ = spec.grid_values()
+        self.nx = np.array(spec.block, dtype=np.int64)
+        self.nxlone = np.array(nxlone, dtype=np.int64)
+        if np.any(self.nxlone % self.nx):
+            raise ValueError("nxlone must be a multiple of the block size")
+        self.ng = self.nxlone // self.nx
+        tp = np.array([1.0, 2.0 * math.pi, 2.0 * math.pi])
+        self.xmin = np.array(xmin) * tp
+        self.xmax = np.array(xmax) * tp
+        self.dx1 = (self.xmax - self.xmin) / self.ng
+        self.dxc0 = (self.xmax - self.xmin) / self.nxlone
+        nx0, nx1, nx2 = (int(v) for v in self.nx)
+        cell = np.arange(nx0 * nx1 * nx2)
+        self.loc = np.stack([cell % nx0, (cell // nx0) % nx1, cell // (nx0 * nx1)], axis=1)
+
+    def block(self, level, ind):
+        inv = 1.0 / float(1 << (level - 1))
+        lb = self.xmin + np.array(ind) * self.dx1 * inv
+        dxc = self.dxc0 * inv
+        return lb, dxc
+
+    def centres(self, level, ind):
+        lb, dxc = self.block(level, ind)
+        return lb + (self.loc + 0.5) * dxc, dxc
+
+
+def build_forest(spec: SnapshotSpec, layout: Layout, torus: Torus):
+    ng = [int(v) for v in layout.ng]
+    if spec.sfc:
+        base = sorted(((i, j, k) for i in range(ng[0]) for j in range(ng[1]) for k in range(ng[2])),
+                      key=lambda t: morton3(*t))
+    else:
+        base = [(i, j, k) for k in range(ng[2]) for j in range(ng[1]) for i in range(ng[0])]
+    flags, leaves = [], []
+
+    def block_mean_rho(level, ind):
+        X, _ = layout.centres(level, ind)
+        r = np.exp(X[:, 0])
+        th = theta_of_x2(X[:, 1], spec.hslope)
+        return float(np.mean(torus.rho(r, th)))
+
+    # a block is refined when the mean torus density over its cell centres exceeds the threshold
+    def visit(level, ind):
+        if level - 1 < len(spec.refine) and block_mean_rho(level, ind) > spec.refine[level - 1]:
+            flags.append(0)
+            i, j, k = ind
+            for c in range(8):
+                visit(level + 1, (2 * i + c % 2, 2 * j + (c // 2) % 2, 2 * k + c // 4))
+        else:
+            flags.append(1)
+            leaves.append((level, ind))
+
+    for ind in base:
+        visit(1, ind)
+    return flags, leaves
+
+
+# --------------------------------------------------------------------------- fluid
+def primitives(spec: SnapshotSpec, torus: Torus, X, rng_u):
+    """rho, u, Gamma v^i, B^i (unscaled) at cell centres X [N,3]."""
+    a, h, gam = spec.spin, spec.hslope, spec.gamma
+    x1, x2 = X[:, 0], X[:, 1]
+    r = np.exp(x1)
+    th = theta_of_x2(x2, h)
+    rho_t = torus.rho(r, th)
+    inside = rho_t > 0.0
+    u_t = torus.K * rho_t ** gam
+    u_t = u_t * (1.0 + spec.pert * (2.0 * rng_u - 1.0))
+    rho = np.maximum(rho_t, 1e-4 * r ** -1.5)
+    u = np.maximum(u_t, 1e-6 * r ** -2.5)
+
+    # velocity: u^mu = u^t (1, 0, 0, Omega) inside the torus, normal observer outside
+    up = np.where(inside, torus.uphi(r, np.where(inside, th, 0.5 * math.pi)), 0.0)
+    gd = metric_dd(a, h, x1, x2)
+    gu = metric_uu(a, h, x1, x2)
+    # g_tt ut^2 + 2 g_tphi ut up + g_phiphi up^2 = -1   (KS components, u^r = u^x2 = 0)
+    A = gd["00"]
+    B = 2.0 * gd["03"] * up
+    C = gd["33"] * up * up + 1.0
+    disc = np.sqrt(np.maximum(B * B - 4.0 * A * C, 0.0))
+    ut = np.where(inside, (-B - disc) / (2.0 * A), 0.0)
+    ut = np.where(inside & (A >= 0.0), 0.0, ut)
+    inside = inside & (ut > 0.0)
+    # Gamma v^i = U^i + shift^i U^0 with shift^i = g^{0i} / (-g^{00})
+    w = np.zeros_like(X)
+    shift1 = gu["01"] / (-gu["00"])
+    w[:, 0] = np.where(inside, shift1 * ut, 0.0)
+    w[:, 2] = np.where(inside, up, 0.0)
+
+    # one poloidal loop A_phi = max(rho_t - cut, 0): B^1 = -d2 A / sqrt(-g), B^2 = d1 A / sqrt(-g)
+    eps = 1e-5
+
+    def aphi(y1, y2):
+        return np.maximum(torus.rho(np.exp(y1), theta_of_x2(y2, h)) - spec.aphi_cut, 0.0)
+
+    d1 = (aphi(x1 + eps, x2) - aphi(x1 - eps, x2)) / (2 * eps)
+    d2 = (aphi(x1, x2 + eps) - aphi(x1, x2 - eps)) / (2 * eps)
+    sig = r * r + a * a * np.cos(th) ** 2
+    sqrtg = sig * np.abs(np.sin(th)) * r * (1.0 + h * np.cos(2.0 * x2))
+    Bv = np.zeros_like(X)
+    Bv[:, 0] = -d2 / sqrtg
+    Bv[:, 1] = d1 / sqrtg
+    return rho, u, w, Bv
+
+
+def b_squared(a, h, X, w, Bv):
+    """b^mu b_mu of the comoving field, from Gamma v^i and B^i."""
+    gd = metric_dd(a, h, X[:, 0], X[:, 1])
+    gu = metric_uu(a, h, X[:, 0], X[:, 1])
+    gs = spatial(gd)
+    lfac = np.sqrt(1.0 + np.einsum("ni,nij,nj->n", w, gs, w))
+    V = w / lfac[:, None]
+    alpha = 1.0 / np.sqrt(-gu["00"])
+    Vl = np.einsum("nij,nj->ni", gs, V)
+    b0 = lfac * np.einsum("ni,ni->n", Bv, Vl) / alpha
+    shift = np.stack([gu["01"] / (-gu["00"]), np.zeros_like(alpha), np.zeros_like(alpha)], axis=1)
+    U0 = lfac / alpha
+    Ui = V * lfac[:, None] - shift * (lfac / alpha)[:, None]
+    bi = (Bv + (alpha * b0)[:, None] * Ui) / lfac[:, None]
+    # b.b with the full metric
+    b = np.concatenate([b0[:, None], bi], axis=1)
+    g4 = np.zeros((X.shape[0], 4, 4))
+    g4[:, 0, 0] = gd["00"]
+    g4[:, 0, 1] = g4[:, 1, 0] = gd["01"]
+    g4[:, 0, 3] = g4[:, 3, 0] = gd["03"]
+    g4[:, 1:, 1:] = gs
+    del U0
+    return np.einsum("ni,nij,nj->n", b, g4, b)
+
+
+def conserved(spec: SnapshotSpec, Xbar, rho, u, w, Bv):
+    """Valencia conserved variables with the metric at the centroid Xbar (inverse of ``bhac.hpp:247-346``)."""
+    a, h, gam = spec.spin, spec.hslope, spec.gamma
+    gd = metric_dd(a, h, Xbar[:, 0], Xbar[:, 1])
+    gs = spatial(gd)
+    lfac = np.sqrt(1.0 + np.einsum("ni,nij,nj->n", w, gs, w))
+    v = w / lfac[:, None]
+    v_d = np.einsum("nij,nj->ni", gs, v)
+    B_d = np.einsum("nij,nj->ni", gs, Bv)
+    B2 = np.einsum("ni,ni->n", Bv, B_d)
+    Bv_dot = np.einsum("ni,ni->n", Bv, v_d)
+    v2 = np.einsum("ni,ni->n", v, v_d)
+    P = (gam - 1.0) * u
+    xi = lfac * lfac * (rho + gam * u)
+    D = lfac * rho
+    S_d = (xi + B2)[:, None] * v_d - Bv_dot[:, None] * B_d
+    tau = xi - P - D + 0.5 * (B2 + B2 * v2 - Bv_dot * Bv_dot)
+    cons = np.zeros((NWINI, rho.shape[0]))
+    cons[0] = D
+    cons[1:4] = S_d.T
+    cons[4] = tau
+    cons[5:8] = Bv.T
+    cons[8] = D * P / rho ** gam
+    cons[NWINI - 2] = lfac
+    cons[NWINI - 1] = xi
+    return cons
+
+
+# --------------------------------------------------------------------------- writer
+def write_grid_file(path: str, spec: SnapshotSpec):
+    nxlone, xmin, xmax, h = spec.grid_values()
+    with open(path, "w") as f:
+        for d in range(3):
+            f.write(f"nxlone{d + 1} = {nxlone[d]}\n")
+        f.write("\n")
+        for d in range(3):
+            f.write(f"xprobminx{d + 1} = {xmin[d]!r}\n")
+        f.write("\n")
+        for d in range(3):
+            f.write(f"xprobmax{d + 1} = {xmax[d]!r}\n")
+        f.write("\n")
+        f.write(f"hslope = {h!r}\n")
+
+
+def generate(spec: SnapshotSpec):
+    """Returns (flags, leaves, prims[nleafs, cells, 8], cons[nleafs, NWINI, cells])."""
+    layout = Layout(spec)
+    torus = Torus(spec)
+    flags, leaves = build_forest(spec, layout, torus)
+    cells = int(np.prod(layout.nx))
+    nleafs = len(leaves)
+    rng = np.random.default_rng(spec.seed)
+    pert = rng.random((nleafs, cells))
+    X = np.empty((nleafs, cells, 3))
+    dxcs = np.empty((nleafs, 3))
+    for q, (level, ind) in enumerate(leaves):
+        X[q], dxcs[q] = layout.centres(level, ind)
+    Xf = X.reshape(-1, 3)
+    rho, u, w, Bv = primitives(spec, torus, Xf, pert.reshape(-1))
+    b2 = b_squared(spec.spin, spec.hslope, Xf, w, Bv)
+    P = (spec.gamma - 1.0) * u
+    pos = b2 > 0.0
+    if np.any(pos):
+        beta = P[pos] / (0.5 * b2[pos])
+        Bv = Bv * math.sqrt(float(np.min(beta)) / spec.beta_min)
+    prims = np.zeros((nleafs * cells, NPRIM))
+    prims[:, 0] = rho
+    prims[:, 1] = u
+    prims[:, 2:5] = w
+    prims[:, 5:8] = Bv
+    cons = np.empty((nleafs, NWINI, cells))
+    for q in range(nleafs):
+        sl = slice(q * cells, (q + 1) * cells)
+        Xbar = centroid(spec.spin, spec.hslope, X[q], dxcs[q])
+        cons[q] = conserved(spec, Xbar, rho[sl], u[sl], w[sl], Bv[sl])
+    return flags, leaves, prims.reshape(nleafs, cells, NPRIM), cons, layout
+
+
+def write_snapshot(dump_path: str, grid_path: str, spec: SnapshotSpec) -> dict:
+    flags, leaves, prims, cons, layout = generate(spec)
+    nx = [int(v) for v in layout.nx]
+    stag = np.zeros((nx[0] + 1) * (nx[1] + 1) * (nx[2] + 1) * NWS)
+    tmp = dump_path + ".tmp"
+    with open(tmp, "wb") as f:
+        for q in range(len(leaves)):
+            f.write(np.ascontiguousarray(cons[q], dtype="<f8").tobytes())
+            f.write(stag.astype("<f8").tobytes())
+        f.write(np.asarray(flags, dtype="<i4").tobytes())
+        f.write(np.asarray(nx, dtype="<i4").tobytes())
+        neqpar = np.zeros(max(4, spec.nspin + 1))
+        neqpar[0] = spec.gamma
+        neqpar[1] = 1.0
+        neqpar[spec.nspin] = spec.spin
+        f.write(neqpar.astype("<f8").tobytes())
+        levmax = max(level for level, _ in leaves)
+        f.write(struct.pack("<8i", len(leaves), levmax, 3, 3, NWINI, NWS, len(neqpar), 2000))
+        f.write(struct.pack("<d", 10000.0))
+    os.replace(tmp, dump_path)
+    write_grid_file(grid_path, spec)
+    h = hashlib.sha256()
+    with open(dump_path, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 20), b""):
+            h.update(chunk)
+    return {"nleafs": len(leaves), "cells": len(leaves) * int(np.prod(layout.nx)),
+            "levmax": max(level for level, _ in leaves), "sha256": h.hexdigest()}
+
+
+def _portable_env() -> dict:
+    """numpy's AVX-512 float64 kernels round differently from glibc's libm; the
+    generator runs with them off so that every x86-64 host writes the same bytes."""
+    env = dict(os.environ)
+    try:
+        from numpy._core._multiarray_umath import __cpu_features__ as feats
+    except ImportError:  # numpy < 2
+        from numpy.core._multiarray_umath import __cpu_features__ as feats
+    off = [k for k, v in feats.items() if v and k.startswith("AVX512")]
+    if off:
+        env["NPY_DISABLE_CPU_FEATURES"] = " ".join(off)
+    return env
+
+
+def ensure_snapshot(name: str, directory: str) -> tuple:
+    """Writes the named preset under ``directory`` once; returns (dump, grid, info)."""
+    import subprocess
+    import sys
+
+    os.makedirs(directory, exist_ok=True)
+    dump = os.path.join(directory, f"{name}.dat")
+    grid = os.path.join(directory, f"{name}_grid.in")
+    stamp = os.path.join(directory, f"{name}.sha256")
+    if not (os.path.exists(dump) and os.path.exists(grid) and os.path.exists(stamp)):
+        root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+        subprocess.run([sys.executable, "-m", "phoenix_amd.snapshot", name, "--dir", directory, "--write"],
+                       check=True, cwd=root, env=_portable_env())
+    with open(stamp) as f:
+        parts = f.read().split()
+    return dump, grid, {"sha256": parts[0], "nleafs": int(parts[1]), "cells": int(parts[2])}
+
+
+def _write_preset(name: str, directory: str) -> dict:
+    dump = os.path.join(directory, f"{name}.dat")
+    grid = os.path.join(directory, f"{name}_grid.in")
+    info = write_snapshot(dump, grid, PRESETS[name])
+    with open(os.path.join(directory, f"{name}.sha256"), "w") as f:
+        f.write(f"{info['sha256']} {info['nleafs']} {info['cells']}\n")
+    return info
+
+
+def default_dir() -> str:
+    return os.environ.get("PHX_SNAPSHOT_DIR", os.path.join(os.path.dirname(os.path.dirname(
+        os.path.abspath(__file__))), "_snapshots"))
+
+
+if __name__ == "__main__":
+    import argparse
+    import time
+
+    ap = argparse.ArgumentParser(description="write a synthetic BHAC snapshot")
+    ap.add_argument("preset", choices=sorted(PRESETS))
+    ap.add_argument("--dir", default=default_dir())
+    ap.add_argument("--write", action="store_true", help="write in this process (no portable-env child)")
+    args = ap.parse_args()
+    t0 = time.time()
+    if args.write:
+        os.makedirs(args.dir, exist_ok=True)
+        print(_write_preset(args.preset, args.dir), f"{time.time() - t0:.1f}s")
+    else:
+        print(ensure_snapshot(args.preset, args.dir), f"{time.time() - t0:.1f}s")
