@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""Throughput of the polarized GRRT raymarch on one MI355X, in rays x frequencies per second.
+
+Workload (one "step" = one full render of one image on a snapshot resident in HBM):
+the 2-level AMR torus snapshot (96x48x48 base cells in 12^3 blocks, refined where
+rho > 0.1 and rho > 0.5 of the maximum: 464 leaves, 0.80M cells, spin 0.9375),
+a 1024 x 1024 image at 230 GHz, inclination 10 deg, m_unit 1.2e18, R_high 3,
+camera at r = 1e4, fov 40 x 40 Rg.  The snapshot is generated (seeded) on first
+use, decoded on the device once and kept resident; the timed region renders
+into device memory (no host copies).
+
+Prints one JSON line with the roofline of the march kernel (algorithmic flops
+per launch from the counted operation mix in tests/golden/op_counts.json times
+the launch's own step / segment counts, over its HIP-event duration) and a CPU
+baseline: the reference's CPU build (oracle/_ref, when present) or the oracle,
+timed on a strided sample of the same image on the host's cores.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--precision fp32|fp64]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "rays·freq/s at 1024² (mixed vs plain FP32, same GPU); p50/p99 request latency"
+PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}  # MI355X vector peaks (FP32 counts packed dual-issue)
+W = H = 1024
+FREQ = 230e9
+
+
+def workload_config(tmpdir: str):
+    from phoenix_amd import configs, snapshot as S
+
+    dump, grid, info = S.ensure_snapshot("c2", S.default_dir())
+    cfg = configs.write_config(os.path.join(tmpdir, "bench.toml"), dump, grid, camera__img_width=str(W),
+                               camera__img_height=str(H), camera__frequency="230e9",
+                               camera__inclination="10", plasma__m_unit="1.2e18", plasma__r_high="3")
+    return cfg, dump, grid, info
+
+
+def algorithmic_flops(info: dict, nf: int, precision: str) -> float:
+    with open(os.path.join(ROOT, "tests", "golden", "op_counts.json")) as f:
+        oc = json.load(f)["c2_1024_230GHz"][precision]
+    useful_segf = info["nplasma"] * nf - info["bad"]
+    return info["steps"] * oc["F_step"] + useful_segf * oc["F_segf"]
+
+
+def cpu_baseline(cfg: str, dump: str, grid: str, threads: int, stride: int) -> dict:
+    """Reference CPU build (or the oracle) on every stride-th pixel of the same image."""
+    npix = (W * H + stride - 1) // stride
+    sample = f"every {stride}th pixel of the 1024^2 image ({npix} rays x 1 frequency)"
+    from oracle import oracle as O
+
+    if O.have_ref():
+        with tempfile.TemporaryDirectory() as td:
+            _, st, secs = O.run_ref(32, cfg, os.path.join(td, "o.bin"), pixels=(0, npix, stride), threads=threads)
+        kind = "reference"
+    else:
+        o = O.Oracle(dump, grid, 32, img_width=W, img_height=H, freqs=(FREQ,))
+        t0 = time.time()
+        o.render(np.arange(0, W * H, stride, dtype=np.int64), threads=threads)
+        secs = time.time() - t0
+        o.close()
+        kind = "port"
+    return {"value": npix / secs, "unit": "rays·freq/s", "cores": threads, "kind": kind,
+            "sample": sample + f", {secs:.1f} s, FP32 build"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 side measurement")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-stride", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the path shards by image tiles with no "
+              "exchange inside the march, and multi-GPU runs are not part of this build.", file=sys.stderr)
+        return 2
+
+    from phoenix_amd import capi
+
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing only; the library synchronises its own stream
+        sync = lambda: None  # noqa: E731
+
+    prec = capi.FP32 if args.precision == "fp32" else capi.FP64
+    with tempfile.TemporaryDirectory() as td:
+        cfg, dump, grid, sinfo = workload_config(td)
+        print(f"# snapshot c2: {sinfo['nleafs']} leaves, {sinfo['cells']} cells", file=sys.stderr, flush=True)
+        run = capi.Run(cfg)
+        snap = capi.Snapshot(run, capi.TABLE_FP32 | capi.TABLE_FP64)
+        nf = run.model.nfreqs
+        print(f"# resident snapshot {snap.bytes / 2**20:.1f} MiB", file=sys.stderr, flush=True)
+        for i in range(args.warmup):
+            _, info = snap.render(None, prec, to_host=False)
+            print(f"# warmup {i}: kernel {info['kernel_s'] * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        sync()
+        t0 = time.perf_counter()
+        kern, infos = [], []
+        for i in range(args.steps):
+            _, info = snap.render(None, prec, to_host=False)
+            kern.append(info["kernel_s"])
+            infos.append(info)
+        sync()
+        wall = time.perf_counter() - t0
+        rays = W * H * nf
+        value = args.steps * rays / wall
+        info = infos[-1]
+        flops = algorithmic_flops(info, nf, args.precision)
+        achieved = flops / float(np.mean(kern)) / 1e12
+        peak = PEAK_TFLOPS[args.precision]
+        extra = {"kernel_ms_mean": float(np.mean(kern) * 1e3), "kernel_ms_min": float(np.min(kern) * 1e3),
+                 "steps_per_ray": info["steps"] / (W * H), "segments_per_ray": info["nplasma"] / (W * H),
+                 "dropped_segments": info["bad"], "vgprs": info["vgprs"], "scratch_bytes": info["scratch_bytes"],
+                 "waves_per_cu": info["waves_per_cu"], "grid": [info["blocks"], info["block"]],
+                 "device": info["device"], "decode_s": info["decode_s"],
+                 "algorithmic_tflop_per_launch": flops / 1e12,
+                 "frac_of_unpacked_fp32_peak": achieved / 78.6 if args.precision == "fp32" else None}
+        if not args.no_fp64 and args.precision == "fp32":
+            _, i64 = snap.render(None, capi.FP64, to_host=False)
+            extra["fp64_rays_freq_per_s"] = rays / i64["kernel_s"]
+            extra["fp64_kernel_ms"] = i64["kernel_s"] * 1e3
+        print(f"# timed {args.steps} renders: {wall:.3f} s, {value:.4g} rays*freq/s", file=sys.stderr, flush=True)
+        snap.close()
+        run.close()
+        cpu = None
+        if not args.no_cpu_baseline:
+            print("# cpu baseline ...", file=sys.stderr, flush=True)
+            cpu = cpu_baseline(cfg, dump, grid, args.cpu_threads, args.cpu_stride)
+    out = {
+        "metric": METRIC, "value": value, "unit": "rays·freq/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.precision == "fp32" else "f64",
+        "data": "synthetic (seeded Fishbone-Moncrief torus snapshot, BHAC format)",
+        "config": {"workload": "c2 snapshot (96x48x48 in 12^3 blocks, 2 AMR levels, 0.80M cells), 1024x1024, "
+                               "230 GHz, i=10, m_unit 1.2e18, R_high 3, plain " + args.precision.upper(),
+                   "image": [W, H], "frequencies_hz": [FREQ], "rays_per_step": W * H * nf},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

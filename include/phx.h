@@ -1,0 +1,184 @@
+/*
+ * phx.h — C ABI of the MI355X-native PHOENIX raymarch (libphx.so).
+ *
+ * Plain C types only: no C++ or torch types cross this boundary.  Every entry
+ * point returns 0 on success or a negative PHX_E* code; phx_last_error() holds
+ * the message of the last failure on the calling thread (exceptions never
+ * cross the ABI).  Device work runs on one GPU (device 0, or the device set
+ * with phx_set_device).
+ *
+ * Reference interfaces each entry point replaces are cited as file:line into
+ * nohgz/phoenix.
+ */
+#ifndef PHX_H
+#define PHX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PHX_MAX_FREQS 32  /* frequencies per request (src/core/numerics/macros.hpp:110-112, CPU value) */
+#define PHX_MAX_LEVELS 16 /* AMR levels (src/core/model/bhac.hpp:154-161) */
+
+/* error codes */
+#define PHX_OK 0
+#define PHX_E_ARG -1     /* invalid argument / configuration */
+#define PHX_E_IO -2      /* file missing, short read, bad format */
+#define PHX_E_DEVICE -3  /* HIP runtime error */
+#define PHX_E_RANGE -4   /* size beyond a supported maximum */
+
+/* arithmetic of a render (the reference's PHX_ACCURACY, CMakeLists.txt:16-22, as a run-time choice) */
+#define PHX_FP32 32  /* scalar = float, compensated (hi, lo) geodesic state */
+#define PHX_FP64 64  /* scalar = double */
+#define PHX_MIXED 96 /* FP32 bulk; FP64 near the horizon / poles and for the AMR inversion; Kahan Stokes sums */
+
+/* One pixel at one frequency: core::pixel::FreqOut (src/core/render/pixel.hpp:26-29).
+ * res[(i * img_height + j) * nfreqs + f]; I_nu in cgs already scaled by nu^3. */
+typedef struct phx_freqout {
+  double IQUV[4];
+  double tau, tauF;
+} phx_freqout;
+
+/* What a render reports: device::Info (src/device/kernel.cuh:23-30), with CU/VGPR
+ * names and the launch's own HIP-event time. */
+typedef struct phx_info {
+  char device[256];
+  int32_t cus, blocks, block;          /* compute units; persistent grid x workgroup size */
+  int32_t vgprs, sgprs, scratch_bytes; /* per-lane registers and private-segment bytes of the march kernel */
+  int32_t lds_bytes, waves_per_cu;     /* static LDS per workgroup; occupancy the launch assumed */
+  double upload_s, kernel_s, decode_s; /* table upload, march kernels (HIP events), device decode */
+  int64_t steps, nplasma, bad;         /* sums over all rays (src/core/render/pixel.hpp:34-37) */
+  int64_t rays;                        /* rays marched (x frequency chunks) */
+} phx_info;
+
+/* device::Progress (src/device/kernel.cuh:32): called on the caller's thread after each chunk */
+typedef void (*phx_progress_fn)(int32_t jobs_done_in_chunk, void* user);
+
+/* The run description: core::bhac::Model (src/core/model/bhac.hpp:131-173) with
+ * UserParams (:62-129), GridParams (:50-55) and PlasmaUnits (:38-45) flattened,
+ * every scalar held as double (a FP32 render rounds them exactly as the
+ * reference's `scalar(x)` does) and explicit widths.  Filled by phx_load_config +
+ * phx_load_grid + phx_load_dump, or by the caller. */
+typedef struct phx_model {
+  /* [camera] */
+  double inclination, fov_x, fov_y, r_cam, t_init, pa, phi_cam;
+  int32_t img_width, img_height, nfreqs, pad0;
+  double frequencies[PHX_MAX_FREQS];
+  /* [black_hole] */
+  double mass, distance;
+  /* [integration] */
+  double stepsize, horizon_margin;
+  int32_t max_steps, max_order;
+  /* [plasma] */
+  double m_unit, r_low, r_high, beta_c, sigma_cut, thetae_min, thetae_max, rt_outer;
+  /* [grmhd] */
+  int32_t nspin, bpol, nlevels, sfc;
+  /* grid file and dump header; xprobmin/max of x2, x3 in radians (x 2 pi, bhac_io.hpp:200-203) */
+  int32_t nxlone[3], nx[3], ng[3];
+  int32_t nleafs, ndim, nwini, nws, cells;
+  double xprobmin[3], xprobmax[3], hslope_grid;
+  /* derived scalars (src/host/io/bhac_io.hpp:183-251) */
+  double a, hslope, gam, Ne, Bu, rhigh, rlow, ibc, r_out, cut_in, cos_pa, sin_pa;
+  double L, T_unit, rho_unit, U_unit, B_unit, Ne_unit;
+  double freq[PHX_MAX_FREQS], hconst[PHX_MAX_FREQS], jscale[PHX_MAX_FREQS];
+  double lvl_w[PHX_MAX_LEVELS][3], x_lo[4], x_hi[4];
+  int32_t level_dims[PHX_MAX_LEVELS][3];
+  /* host tables, owned by phx_load_dump (freed by phx_free_model) */
+  double* lb;                          /* [nleafs][3] block lower corners */
+  double* dxc;                         /* [nleafs][3] cell sizes */
+  double* prim;                        /* [nleafs][cells][8] primitives, or NULL: the device decodes the dump */
+  int32_t* level_map[PHX_MAX_LEVELS];  /* leaf per (level, i, j, k), -1 = go finer */
+  int32_t* leaf_level;                 /* [nleafs] 1-based AMR level */
+  int32_t* leaf_index;                 /* [nleafs][3] block index on its level */
+} phx_model;
+
+/* [output]/[visual]/[grmhd] keys that are not part of the model (host::config::Params, config.hpp:35-50) */
+typedef struct phx_config {
+  char outdir[1024];
+  char image[16];      /* hdf5 | text | both | none */
+  int32_t block_size;  /* HDF5 pixels per block side */
+  char dump_file[1024];
+  char grid_file[1024];
+  int32_t gpu_decode, show_header;
+} phx_config;
+
+/* ---------------------------------------------------------------- host layer */
+
+const char* phx_last_error(void);
+const char* phx_version(void);
+
+/* host::config::load_config (src/host/io/config.hpp:229-381): TOML subset, section.key=value
+ * overrides, frequency forms, unknown / missing / duplicate key errors.  Fills the [camera],
+ * [black_hole], [integration], [plasma], [grmhd] fields of *model. */
+int phx_load_config(const char* path, const char* const* overrides, int32_t noverrides, phx_config* cfg,
+                    phx_model* model);
+
+/* host::bhac_io::load_bhac_grid (src/host/io/bhac_io.hpp:143-174) */
+int phx_load_grid(const char* grid_file, phx_model* model);
+
+/* host::bhac_io::load (src/host/io/bhac_io.hpp:331-499) without the host decode: header,
+ * finalize_model units, forest walk, block geometry, level map.  model->prim stays NULL and
+ * the device decodes the dump (the reference's grmhd.gpu_decode = true path). */
+int phx_load_dump(phx_model* model, const char* dump_file);
+
+/* Re-derives the per-request scalars (units, plasma, camera rotation, cut_in, r_out, per-frequency
+ * hconst/jscale) from the user fields after a caller changed them: the per-request half of
+ * host::bhac_io::finalize_model (src/host/io/bhac_io.hpp:224-250).  Grid fields are untouched. */
+int phx_model_derive(phx_model* model);
+
+/* host::bhac_io::free_model (src/host/io/bhac_io.hpp:314-322) */
+void phx_free_model(phx_model* model);
+
+/* host::jobs::schedule (src/host/render/jobs.hpp:18-46): i * img_height + j, Morton-ordered if morton */
+int phx_schedule(const phx_model* model, int32_t morton, int32_t* jobs, int32_t njobs);
+
+/* host::output::compute_spec (src/host/io/output.hpp:43-53): flux [Jy] of frequency fi, non-finite pixels skipped */
+int phx_compute_spec(const phx_model* model, const phx_freqout* res, int32_t fi, double S[4]);
+
+/* ---------------------------------------------------------------- device path */
+
+int phx_set_device(int32_t device);
+
+/* device::warmup_start (src/device/kernel.cu:160-162): creates the HIP context off the caller's thread */
+int phx_dev_warmup(void);
+
+/* device::render (src/device/kernel.cu:284-360; declared src/device/kernel.cuh:57-66).
+ * Uploads the tables, decodes the dump on the device when model->prim is NULL, marches the
+ * jobs (pixel indices i * img_height + j, any order) and copies res back.  Device memory is
+ * per call.  progress (may be NULL) is called after each chunk of PHX_CHUNK jobs. */
+int phx_dev_render(const phx_model* model, const char* dump_file, int32_t precision, const int32_t* jobs,
+                   int32_t njobs, phx_freqout* res, phx_info* info, phx_progress_fn progress, void* user);
+
+/* ---------------------------------------------------------------- resident snapshots
+ * No counterpart in the reference (its device memory is per call, kernel.cu:131-155).
+ * A snapshot keeps the decoded cell table, the block table and the flattened AMR map in HBM
+ * across renders; requests then differ only in camera / plasma / frequency parameters. */
+typedef struct phx_snapshot phx_snapshot;
+
+/* precisions: bit set of PHX_FP32 -> 1, PHX_FP64 -> 2 (PHX_MIXED uses both tables) */
+#define PHX_TABLE_FP32 1
+#define PHX_TABLE_FP64 2
+int phx_snapshot_open(const phx_model* model, const char* dump_file, int32_t tables, phx_snapshot** out);
+int phx_snapshot_close(phx_snapshot* snap);
+/* bytes of HBM the snapshot holds */
+int64_t phx_snapshot_bytes(const phx_snapshot* snap);
+
+/* One render of `request` (camera / plasma / integration / frequency fields; the grid fields must
+ * describe the snapshot's dump) on a resident snapshot.  res: host array of
+ * img_width*img_height*nfreqs entries, or NULL to keep the result on the device (read it with
+ * phx_snapshot_result).  jobs NULL = all pixels in Morton order. */
+int phx_snapshot_render(phx_snapshot* snap, const phx_model* request, int32_t precision, const int32_t* jobs,
+                        int32_t njobs, phx_freqout* res, phx_info* info);
+/* copies the device result of the last phx_snapshot_render into res */
+int phx_snapshot_result(phx_snapshot* snap, phx_freqout* res, int64_t nres);
+
+/* decoded primitives of a snapshot, [nleafs][cells][8] as double (for checks) */
+int phx_snapshot_prims(phx_snapshot* snap, int32_t table, double* out, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHX_H */

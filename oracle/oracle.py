@@ -201,7 +201,7 @@ def run_ref(precision: int, config: str, out: str, pixels=None, threads: int = 8
     """Runs the reference CPU build (oracle/_ref) and returns (res [W*H, nf, 6], stats, seconds)."""
     cmd = [ref_binary(precision), config, out]
     if pixels is not None:
-        cmd += ["--pixels", f"{pixels[0]},{pixels[1]}"]
+        cmd += ["--pixels", ",".join(str(int(v)) for v in pixels)]
     cmd += list(overrides)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     p = subprocess.run(cmd, env=env, capture_output=True, text=True)

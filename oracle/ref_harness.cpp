@@ -12,7 +12,8 @@
 // needs hdf5.h, which is not part of the path).
 //
 // Usage: phx_ref_fpXX config.toml out.bin [section.key=value ...]
-//        phx_ref_fpXX config.toml out.bin --pixels i0,n ...   (pixel subset)
+//        phx_ref_fpXX config.toml out.bin --pixels first,count[,stride] ...
+//        (pixel subset first + k*stride, k < count)
 // out.bin: int32 {W, H, nf, 0} int64 {steps, nplasma, bad} f64 {raymarch_s}
 //          then W*H*nf*6 float64 (IQUV, tau, tauF), res[(i*H + j)*nf + f].
 #include <omp.h>
@@ -35,10 +36,10 @@ int main(int argc, char** argv) {
   }
   const std::string cfg = argv[1], outp = argv[2];
   std::vector<std::string> ov;
-  long first = 0, count = -1;
+  long first = 0, count = -1, stride = 1;
   for (int a = 3; a < argc; a++) {
     if (!std::strcmp(argv[a], "--pixels") && a + 1 < argc) {
-      std::sscanf(argv[++a], "%ld,%ld", &first, &count);
+      std::sscanf(argv[++a], "%ld,%ld,%ld", &first, &count, &stride);
       continue;
     }
     ov.push_back(argv[a]);
@@ -55,13 +56,15 @@ int main(int argc, char** argv) {
   }
   const core::bhac::UserParams& up = model.uparams;
   const long npix = (long)up.img_width * up.img_height;
-  if (count < 0 || first + count > npix) count = npix - first;
+  if (stride < 1) stride = 1;
+  if (count < 0 || first + (count - 1) * stride >= npix) count = (npix - first + stride - 1) / stride;
   std::vector<core::pixel::FreqOut> res((size_t)npix * up.nfreqs);
   std::memset(res.data(), 0, res.size() * sizeof(res[0]));
   long steps = 0, segs = 0, bad = 0;
   const auto t0 = std::chrono::steady_clock::now();
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : steps, segs, bad)
-  for (long n = first; n < first + count; n++) {
+  for (long kk = 0; kk < count; kk++) {
+    const long n = first + kk * stride;
     const int i = (int)(n / up.img_height), j = (int)(n % up.img_height);
     double a0, b0;
     core::camera::pixel_impact(i, j, up.img_width, up.img_height, up.fov_x, up.fov_y, &a0, &b0);

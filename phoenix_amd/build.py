@@ -1,0 +1,56 @@
+"""Builds phoenix_amd/libphx.so (the C ABI of include/phx.h) in-tree.
+
+hipcc compiles the device TU for gfx950 only; g++ compiles the host TU; hipcc
+links both into one shared library next to this file.  Packed FP32 VALU ops are
+disabled (``-Xclang -target-feature -Xclang -packed-fp32-ops``): some of their
+forms have produced wrong results on the MI355X boxes this runs on.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libphx.so")
+BUILD = os.path.join(HERE, "_build")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-Wall", "-Wno-unused-function",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics",
+             f"-I{ROOT}/include"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{ROOT}/include"]
+
+
+def _run(cmd):
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode:
+        sys.stderr.write(" ".join(cmd) + "\n" + p.stdout + p.stderr)
+        raise RuntimeError(f"build step failed: {cmd[0]} (exit {p.returncode})")
+    return p
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    dev_o = os.path.join(BUILD, "phx_device.o")
+    host_o = os.path.join(BUILD, "phx_host.o")
+    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "phx_device.hip"), "-o", dev_o])
+    _run([os.environ.get("CXX", "g++"), *CXX_FLAGS, "-c", os.path.join(CSRC, "phx_host.cpp"), "-o", host_o])
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, dev_o, host_o])
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+def clean() -> None:
+    shutil.rmtree(BUILD, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    build(verbose=True)

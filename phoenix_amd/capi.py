@@ -1,0 +1,257 @@
+"""ctypes binding of libphx.so (include/phx.h).
+
+The raymarch runs only through the HIP library: loading fails loudly when
+libphx.so is missing, and a render fails with the library's own error when no
+GPU is present.  There is no CPU path here.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphx.so")
+
+MAX_FREQS = 32
+MAX_LEVELS = 16
+FP32, FP64, MIXED = 32, 64, 96
+TABLE_FP32, TABLE_FP64 = 1, 2
+
+c_double, c_int, c_int64, c_char = ctypes.c_double, ctypes.c_int32, ctypes.c_int64, ctypes.c_char
+P_dbl, P_int = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)
+
+
+class FreqOut(ctypes.Structure):
+    _fields_ = [("IQUV", c_double * 4), ("tau", c_double), ("tauF", c_double)]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [("device", c_char * 256), ("cus", c_int), ("blocks", c_int), ("block", c_int),
+                ("vgprs", c_int), ("sgprs", c_int), ("scratch_bytes", c_int), ("lds_bytes", c_int),
+                ("waves_per_cu", c_int), ("upload_s", c_double), ("kernel_s", c_double),
+                ("decode_s", c_double), ("steps", c_int64), ("nplasma", c_int64), ("bad", c_int64),
+                ("rays", c_int64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["device"] = self.device.decode(errors="replace")
+        return d
+
+
+class Model(ctypes.Structure):
+    _fields_ = [
+        ("inclination", c_double), ("fov_x", c_double), ("fov_y", c_double), ("r_cam", c_double),
+        ("t_init", c_double), ("pa", c_double), ("phi_cam", c_double),
+        ("img_width", c_int), ("img_height", c_int), ("nfreqs", c_int), ("pad0", c_int),
+        ("frequencies", c_double * MAX_FREQS),
+        ("mass", c_double), ("distance", c_double),
+        ("stepsize", c_double), ("horizon_margin", c_double),
+        ("max_steps", c_int), ("max_order", c_int),
+        ("m_unit", c_double), ("r_low", c_double), ("r_high", c_double), ("beta_c", c_double),
+        ("sigma_cut", c_double), ("thetae_min", c_double), ("thetae_max", c_double), ("rt_outer", c_double),
+        ("nspin", c_int), ("bpol", c_int), ("nlevels", c_int), ("sfc", c_int),
+        ("nxlone", c_int * 3), ("nx", c_int * 3), ("ng", c_int * 3),
+        ("nleafs", c_int), ("ndim", c_int), ("nwini", c_int), ("nws", c_int), ("cells", c_int),
+        ("xprobmin", c_double * 3), ("xprobmax", c_double * 3), ("hslope_grid", c_double),
+        ("a", c_double), ("hslope", c_double), ("gam", c_double), ("Ne", c_double), ("Bu", c_double),
+        ("rhigh", c_double), ("rlow", c_double), ("ibc", c_double), ("r_out", c_double), ("cut_in", c_double),
+        ("cos_pa", c_double), ("sin_pa", c_double),
+        ("L", c_double), ("T_unit", c_double), ("rho_unit", c_double), ("U_unit", c_double),
+        ("B_unit", c_double), ("Ne_unit", c_double),
+        ("freq", c_double * MAX_FREQS), ("hconst", c_double * MAX_FREQS), ("jscale", c_double * MAX_FREQS),
+        ("lvl_w", (c_double * 3) * MAX_LEVELS), ("x_lo", c_double * 4), ("x_hi", c_double * 4),
+        ("level_dims", (c_int * 3) * MAX_LEVELS),
+        ("lb", P_dbl), ("dxc", P_dbl), ("prim", P_dbl),
+        ("level_map", P_int * MAX_LEVELS), ("leaf_level", P_int), ("leaf_index", P_int),
+    ]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("outdir", c_char * 1024), ("image", c_char * 16), ("block_size", c_int),
+                ("dump_file", c_char * 1024), ("grid_file", c_char * 1024), ("gpu_decode", c_int),
+                ("show_header", c_int)]
+
+
+class PhxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"phx error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+# every entry point of include/phx.h, with its ctypes signature
+SIGNATURES = {
+    "phx_last_error": (ctypes.c_char_p, []),
+    "phx_version": (ctypes.c_char_p, []),
+    "phx_load_config": (c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), c_int, ctypes.POINTER(Config),
+                                ctypes.POINTER(Model)]),
+    "phx_load_grid": (c_int, [ctypes.c_char_p, ctypes.POINTER(Model)]),
+    "phx_load_dump": (c_int, [ctypes.POINTER(Model), ctypes.c_char_p]),
+    "phx_model_derive": (c_int, [ctypes.POINTER(Model)]),
+    "phx_free_model": (None, [ctypes.POINTER(Model)]),
+    "phx_schedule": (c_int, [ctypes.POINTER(Model), c_int, P_int, c_int]),
+    "phx_compute_spec": (c_int, [ctypes.POINTER(Model), ctypes.POINTER(FreqOut), c_int, P_dbl]),
+    "phx_set_device": (c_int, [c_int]),
+    "phx_dev_warmup": (c_int, []),
+    "phx_dev_render": (c_int, [ctypes.POINTER(Model), ctypes.c_char_p, c_int, P_int, c_int, ctypes.POINTER(FreqOut),
+                               ctypes.POINTER(Info), ctypes.c_void_p, ctypes.c_void_p]),
+    "phx_snapshot_open": (c_int, [ctypes.POINTER(Model), ctypes.c_char_p, c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "phx_snapshot_close": (c_int, [ctypes.c_void_p]),
+    "phx_snapshot_bytes": (c_int64, [ctypes.c_void_p]),
+    "phx_snapshot_render": (c_int, [ctypes.c_void_p, ctypes.POINTER(Model), c_int, P_int, c_int,
+                                    ctypes.POINTER(FreqOut), ctypes.POINTER(Info)]),
+    "phx_snapshot_result": (c_int, [ctypes.c_void_p, ctypes.POINTER(FreqOut), c_int64]),
+    "phx_snapshot_prims": (c_int, [ctypes.c_void_p, c_int, P_dbl, c_int64]),
+}
+
+
+def lib():
+    """The loaded libphx.so; raises if it was not built (there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m phoenix_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise PhxError(rc, lib().phx_last_error().decode(errors="replace"))
+    return rc
+
+
+def _res_array(n):
+    arr = np.zeros((n, 6), dtype=np.float64)
+    return arr, arr.ctypes.data_as(ctypes.POINTER(FreqOut))
+
+
+class Run:
+    """config.toml + grid + dump loaded by the library's host layer (phx_load_*)."""
+
+    def __init__(self, config_path: str, overrides=()):
+        self.model = Model()
+        self.config = Config()
+        ov = [o.encode() for o in overrides]
+        arr = (ctypes.c_char_p * max(1, len(ov)))(*ov)
+        check(lib().phx_load_config(config_path.encode(), arr, len(ov), ctypes.byref(self.config),
+                                    ctypes.byref(self.model)))
+        check(lib().phx_load_grid(self.config.grid_file, ctypes.byref(self.model)))
+        check(lib().phx_load_dump(ctypes.byref(self.model), self.config.dump_file))
+        self._loaded = True
+
+    @property
+    def dump(self) -> str:
+        return self.config.dump_file.decode()
+
+    def close(self):
+        if getattr(self, "_loaded", False):
+            lib().phx_free_model(ctypes.byref(self.model))
+            self._loaded = False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def request(self, **kw) -> Model:
+        """A copy of the model with user fields replaced and per-request scalars re-derived."""
+        m = Model()
+        ctypes.pointer(m)[0] = self.model
+        for k, v in kw.items():
+            if k == "freqs":
+                m.nfreqs = len(v)
+                for i, f in enumerate(v):
+                    m.frequencies[i] = f
+            else:
+                setattr(m, k, v)
+        check(lib().phx_model_derive(ctypes.byref(m)))
+        return m
+
+    def schedule(self, morton: bool = True, model: Model | None = None) -> np.ndarray:
+        m = model or self.model
+        n = m.img_width * m.img_height
+        jobs = np.zeros(n, dtype=np.int32)
+        check(lib().phx_schedule(ctypes.byref(m), int(morton), jobs.ctypes.data_as(P_int), n))
+        return jobs
+
+    def dev_render(self, precision: int = FP32, model: Model | None = None, jobs=None):
+        """phx_dev_render: the drop-in one-shot device path.  Returns (res [W*H, nf, 6], info)."""
+        m = model or self.model
+        if jobs is None:
+            jobs = self.schedule(True, m)
+        jobs = np.ascontiguousarray(jobs, dtype=np.int32)
+        n = m.img_width * m.img_height * m.nfreqs
+        res, rp = _res_array(n)
+        info = Info()
+        check(lib().phx_dev_render(ctypes.byref(m), self.config.dump_file, precision, jobs.ctypes.data_as(P_int),
+                                   jobs.size, rp, ctypes.byref(info), None, None))
+        return res.reshape(m.img_width * m.img_height, m.nfreqs, 6), info.as_dict()
+
+    def flux(self, res: np.ndarray, model: Model | None = None) -> np.ndarray:
+        m = model or self.model
+        r = np.ascontiguousarray(res.reshape(-1, 6))
+        out = np.zeros((m.nfreqs, 4))
+        for f in range(m.nfreqs):
+            S = np.zeros(4)
+            check(lib().phx_compute_spec(ctypes.byref(m), r.ctypes.data_as(ctypes.POINTER(FreqOut)), f,
+                                         S.ctypes.data_as(P_dbl)))
+            out[f] = S
+        return out
+
+
+class Snapshot:
+    """A snapshot resident in HBM (phx_snapshot_*)."""
+
+    def __init__(self, run: Run, tables: int = TABLE_FP32 | TABLE_FP64):
+        self.run = run
+        self.h = ctypes.c_void_p()
+        check(lib().phx_snapshot_open(ctypes.byref(run.model), run.config.dump_file, tables, ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            check(lib().phx_snapshot_close(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def bytes(self) -> int:
+        return int(lib().phx_snapshot_bytes(self.h))
+
+    def render(self, request: Model | None = None, precision: int = FP32, jobs=None, to_host: bool = True):
+        m = request or self.run.model
+        n = m.img_width * m.img_height * m.nfreqs
+        jp, nj = None, 0
+        if jobs is not None:
+            jobs = np.ascontiguousarray(jobs, dtype=np.int32)
+            jp, nj = jobs.ctypes.data_as(P_int), jobs.size
+        info = Info()
+        if to_host:
+            res, rp = _res_array(n)
+        else:
+            res, rp = None, None
+        check(lib().phx_snapshot_render(self.h, ctypes.byref(m), precision, jp, nj, rp, ctypes.byref(info)))
+        if res is not None:
+            res = res.reshape(m.img_width * m.img_height, m.nfreqs, 6)
+        return res, info.as_dict()
+
+    def prims(self, table: int) -> np.ndarray:
+        m = self.run.model
+        n = m.nleafs * m.cells * 8
+        out = np.zeros(n)
+        check(lib().phx_snapshot_prims(self.h, table, out.ctypes.data_as(P_dbl), n))
+        return out.reshape(m.nleafs, m.cells, 8)

@@ -1,0 +1,855 @@
+// Device runtime of libphx: resident snapshots (device decode, block table,
+// flattened AMR map), the persistent lane-refill march kernel and the C ABI of
+// the device path (include/phx.h).  gfx950 only.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "phx_internal.h"
+#include "phx_march.h"
+
+using namespace phxd;
+
+namespace phx {
+
+#define HIPCHECK(call)                                                                                 \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess)                                                                              \
+      throw Error(PHX_E_DEVICE, std::string(hipGetErrorString(e_)) + " (" #call ", " __FILE__ ":" +   \
+                                    std::to_string(__LINE__) + ")");                                   \
+  } while (0)
+
+constexpr int BLOCK = 256;  // 4 wave64 per workgroup
+
+// ============================================================================ kernels
+template <class T, int NF>
+struct MarchArgs {
+  Grid<T> G;
+  Plasma<T> P;
+  Cam C;
+  T stepsize, r_out, cut_in;
+  int max_steps, max_order;
+  int nf_total, f0, nf, count_stats;
+  T freq[NF], hconst[NF], jscale[NF];
+  const int* jobs;
+  int j0, j1;
+  int* counter;
+  phx_freqout* res;
+  unsigned long long* tot;
+};
+
+// Persistent march: each lane owns one ray at a time; a lane whose ray ends writes it
+// out and takes the next job (wave-aggregated atomic on the job counter), so rays that
+// hit the horizon or escape free their lane at once instead of idling until the wave's
+// longest ray ends.  One pass of the loop is one pass of render_pixel's loop
+// (src/core/render/pixel.hpp:179-211): view, process the pending sample with the
+// current position's leaf as its cache, turning points, step size, RK4.
+template <class T, int NF>
+__global__ __launch_bounds__(BLOCK) void k_march(const MarchArgs<T, NF> A) {
+  constexpr bool F32 = Lim<T>::fp32;
+  const int lane = threadIdx.x & 63;
+  Ray<T> g;
+  T px[4], pk[4], pf[4], pdl = T(0);
+  int pi = -1;
+  T AA[NF][16], SS[NF][4], tau[NF], tauF[NF];
+  int steps = 0, turns = 0, stop = 0, np = 0, bad = 0, idx = 0;
+  T thdot = T(0), dlp = T(0);
+  bool bneg = false, bpos = false, have = false;
+  unsigned long long s_steps = 0, s_np = 0, s_bad = 0;
+
+  for (;;) {
+    if (!have) {
+      const unsigned long long m = __ballot(1);
+      const int leader = __ffsll((long long)m) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(A.counter, (int)__popcll(m));
+      base = __shfl(base, leader);
+      const int n = A.j0 + base + (int)__popcll(m & ((1ull << lane) - 1ull));
+      if (n >= A.j1) break;
+      idx = A.jobs[n];
+      // pixel -> impact parameters with the position-angle rotation (camera.hpp:11-23, jobs.hpp:64-76)
+      const int i = idx / A.C.H, j = idx % A.C.H;
+      const double a0 = -0.5 * A.C.fov_x + (i + 0.5) * A.C.fov_x / A.C.W;
+      const double b0 = -0.5 * A.C.fov_y + (j + 0.5) * A.C.fov_y / A.C.H;
+      const double alpha = a0 * A.C.cos_pa + b0 * A.C.sin_pa;
+      const double beta = -a0 * A.C.sin_pa + b0 * A.C.cos_pa;
+      double ph[8];
+      camera_photon(A.C, alpha, beta, ph);
+      T F1[4];
+      camera_F1<T>(A.P.a, A.P.hs, ph, F1);
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        g.hi[q] = T(ph[q]);
+        g.lo[q] = F32 ? T(ph[q] - (double)g.hi[q]) : T(0);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        g.hi[8 + q] = F1[q];
+        g.lo[8 + q] = T(0);
+      }
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+#pragma unroll
+        for (int q = 0; q < 16; q++) AA[f][q] = (q % 5 == 0) ? T(1) : T(0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) SS[f][q] = T(0);
+        tau[f] = tauF[f] = T(0);
+      }
+      steps = turns = stop = np = bad = 0;
+      thdot = dlp = T(0);
+      pi = -1;
+      bneg = beta < 0.;
+      bpos = beta > 0.;
+      have = true;
+    }
+
+    // ---- view of the current state
+    T cx[4], ck[4], cf[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      cx[q] = F32 ? g.hi[q] + g.lo[q] : g.hi[q];
+      ck[q] = F32 ? g.hi[4 + q] + g.lo[4 + q] : g.hi[4 + q];
+      cf[q] = F32 ? g.hi[8 + q] + g.lo[8 + q] : g.hi[8 + q];
+    }
+    const T r = exp(cx[1]);
+    const bool alive = r < A.r_out && r > A.cut_in && steps < A.max_steps && !stop;
+
+    // ---- one AMR lookup serves the pending sample's cache (leaf_of) and the step's cell cap (grid_dxc)
+    int leaf_next = -1;
+    bool have_dxc = false;
+    T dxc[3] = {T(0), T(0), T(0)};
+    if (alive) {
+      T wx[4] = {cx[0], cx[1], cx[2], cx[3]};
+      wrap(wx);
+      const bool need_dxc = r < A.P.rt_outer;
+      const bool need_leaf = pi >= 1 && !off_domain(A.G, wx);
+      if (need_dxc || need_leaf) {
+        const int L = find_leaf(A.G, wx);
+        if (need_leaf) leaf_next = L;
+        if (need_dxc && L >= 0) {
+          have_dxc = true;
+#pragma unroll
+          for (int d = 0; d < 3; d++) dxc[d] = A.G.blk[12 * L + 6 + d];
+        }
+      }
+    }
+
+    // ---- the pending sample (src/core/render/pixel.hpp:52-123)
+    if (pi >= 1) {
+      T k[4] = {pk[0], pk[1], pk[2], pk[3]};
+      const T rs = exp(px[1]);
+      const Met<T> gs = metric(A.P.a, A.P.hs, px[1], px[2]);
+      if (fabs(dot(gs, k, k)) > T(1e-6) && rs > T(2)) {  // re-solve k^t for k.k = 0 (geodesic.hpp:297-306)
+        const T aa = gs.d00;
+        const T bc = -(gs.d01 * k[1]) / aa - (gs.d03 * k[3]) / aa;
+        const T gz = -(gs.d11 * k[1] * k[1]) / aa - (T(2) * gs.d13 * k[1] * k[3]) / aa - (gs.d22 * k[2] * k[2]) / aa -
+                     (gs.d33 * k[3] * k[3]) / aa;
+        k[0] = bc + sqrt(bc * bc + gz);
+      }
+      Fluid<T> fl;
+      const int in_vol = sample_fluid(A.G, A.P, px, leaf_next, fl);
+      if (in_vol && rs < A.P.rt_outer) {
+        const T pa = pitch_angle(fl.g, k, fl.b, fl.U);
+        if (!(pa == T(0) || pa == T(PI_D))) {
+          T perp[4], para[4];
+          frame(fl.g, A.P.a, k, fl.U, fl.b, perp, para);
+          const T Cp = dot(fl.g, perp, pf), Sp = dot(fl.g, para, pf);
+          const T nn = Cp * Cp + Sp * Sp;
+          const T c2 = (Cp * Cp - Sp * Sp) / nn, s2 = T(2) * Cp * Sp / nn;
+          const T ratio = -dot(fl.g, k, fl.U);
+          np++;
+          if (__builtin_isnan(c2) || __builtin_isnan(s2) || __builtin_isnan(ratio)) {
+            // a NaN rotation or frequency ratio makes every frequency's operator non-finite: the
+            // reference drops each of them (pixel.hpp:107-116); count them without computing them
+            bad += A.nf;
+          } else {
+            T sth, cth;
+            scs(pa, &sth, &cth);
+            T kr1, kr0;
+            kratios(fl.te, &kr1, &kr0);
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+              if (f >= A.nf) break;
+              const T nup = A.freq[f] * ratio;
+              const Coef<T> seg =
+                  invariant_rotated(thermal(fl.te, fl.ne, nup, fl.B, sth, cth, kr1, kr0), nup, A.jscale[f], c2, s2);
+              const T h = pdl * A.hconst[f];
+              T OA[16], OS[4];
+              segment_op(seg, h, OA, OS);
+              bool fin = true;
+#pragma unroll
+              for (int q = 0; q < 16; q++) fin = fin && __builtin_isfinite(OA[q]);
+#pragma unroll
+              for (int q = 0; q < 4; q++) fin = fin && __builtin_isfinite(OS[q]);
+              if (!fin) {
+                bad++;
+                continue;
+              }
+              // S += A O_S; A <- A O_A (operators.hpp:74-86)
+              T nA[16];
+#pragma unroll
+              for (int i = 0; i < 4; i++) {
+                T s = T(0);
+#pragma unroll
+                for (int j = 0; j < 4; j++) s += AA[f][4 * i + j] * OS[j];
+                SS[f][i] += s;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                  T t = T(0);
+#pragma unroll
+                  for (int kq = 0; kq < 4; kq++) t += AA[f][4 * i + kq] * OA[4 * kq + j];
+                  nA[4 * i + j] = t;
+                }
+              }
+#pragma unroll
+              for (int q = 0; q < 16; q++) AA[f][q] = nA[q];
+              tau[f] += seg.aI * h;
+              tauF[f] += fabs(seg.rV * h);
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      px[q] = cx[q];
+      pk[q] = ck[q];
+      pf[q] = cf[q];
+    }
+    pdl = dlp;
+    pi = steps;
+
+    if (!alive) {
+      // ---- ray done: write it out and free the lane (pixel.hpp:213-223)
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        if (f >= A.nf) break;
+        double2* o = reinterpret_cast<double2*>(A.res + ((size_t)idx * A.nf_total + A.f0 + f));
+        if (np > 0) {
+          o[0] = make_double2((double)SS[f][0], (double)SS[f][1]);
+          o[1] = make_double2((double)SS[f][2], (double)SS[f][3]);
+          o[2] = make_double2((double)tau[f], (double)tauF[f]);
+        } else {
+          o[0] = o[1] = o[2] = make_double2(0., 0.);
+        }
+      }
+      s_steps += steps;
+      s_np += np;
+      s_bad += bad;
+      have = false;
+      continue;
+    }
+
+    // ---- turning points of theta (pixel.hpp:193-197)
+    if (thdot * ck[2] < T(0) && steps > 2) turns++;
+    thdot = ck[2];
+    if ((bneg && turns > A.max_order) || (bpos && turns > A.max_order + 1)) stop = 1;
+
+    // ---- grid-aware step and RK4 (pixel.hpp:200-210)
+    const T dl = step_len(A.stepsize, pole_x2(g), ck, have_dxc, dxc);
+    rk4(A.P.a, A.P.hs, g, dl);
+    dlp = fabs(dl);
+    steps++;
+  }
+
+  if (A.count_stats) {
+    // wave reduction, then one atomic per wave
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      s_steps += __shfl_xor(s_steps, off);
+      s_np += __shfl_xor(s_np, off);
+      s_bad += __shfl_xor(s_bad, off);
+    }
+    if (lane == 0) {
+      atomicAdd(&A.tot[0], s_steps);
+      atomicAdd(&A.tot[1], s_np);
+      atomicAdd(&A.tot[2], s_bad);
+    }
+  }
+}
+
+// cons -> prim of one cell in FP64 (src/core/model/bhac.hpp:178-346): metric at the
+// sqrt(gamma)-weighted 27-point Simpson centroid, Valencia inversion, entropy fallback.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_decode(const double* __restrict__ raw, int leaf0, int nl, int cells,
+                                                  int nwini, int nx0, int nx1, const T* __restrict__ blk, T a,
+                                                  T hs, T gam, int bpol, T* __restrict__ prim) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nl * cells) return;
+  const int q = (int)(t / cells), cell = (int)(t % cells), leaf = leaf0 + q;
+  const double* cons = raw + (size_t)q * nwini * cells;
+  auto CV = [&](int nw) { return cons[(size_t)nw * cells + cell]; };
+  const int loc[3] = {cell % nx0, (cell / nx0) % nx1, cell / (nx0 * nx1)};
+  double X[3], dx[3];
+  for (int d = 0; d < 3; d++) {
+    dx[d] = (double)blk[12 * leaf + 6 + d];
+    X[d] = (double)blk[12 * leaf + d] + (loc[d] + 0.5) * dx[d];
+  }
+  T* out = prim + ((size_t)leaf * cells + cell) * 8;
+  if (!(exp(X[0]) > 1.0)) {
+    for (int v = 0; v < 8; v++) out[v] = T(0);
+    return;
+  }
+  const double ad = (double)a, hd = (double)hs;
+  const double wt[3] = {1., 4., 1.}, off[3] = {-1., 0., 1.};
+  double nrm = 0., xb = 0., yb = 0.;  // the metric does not depend on x3
+  for (int k = 0; k < 3; k++)
+    for (int j = 0; j < 3; j++)
+      for (int i = 0; i < 3; i++) {
+        const double w = wt[i] * wt[j] * wt[k];
+        const double x = X[0] + dx[0] / 2. * off[i], y = X[1] + dx[1] / 2. * off[j];
+        const Met<double> g = metric<double>(ad, hd, x, y);
+        const double dg = sqrt(g.d11 * (g.d22 * g.d33) - g.d13 * (g.d22 * g.d13));
+        nrm += dg * w;
+        xb += dg * w * x;
+        yb += dg * w * y;
+      }
+  const double vol = (dx[0] / 6.0) * (dx[1] / 6.0) * (dx[2] / 6.0);
+  nrm *= vol;
+  const double Xb1 = xb * (vol / nrm), Xb2 = yb * (vol / nrm);
+  const Met<double> g = metric<double>(ad, hd, Xb1, Xb2);
+  const MetU<double> u = metric_up<double>(ad, g);
+  const double S1 = CV(1), S2 = CV(2), S3 = CV(3), B1 = CV(5), B2 = CV(6), B3 = CV(7);
+  const double BS = S1 * B1 + S2 * B2 + S3 * B3;
+  // gamma^ij = g^ij + g^0i g^0j / (-g^00): only the (1,1) entry picks up the shift term
+  const double y11 = u.u11 + u.u01 * u.u01 / (-u.u00), y13 = u.u13, y22 = u.u22, y33 = u.u33;
+  const double Su1 = y11 * S1 + y13 * S3, Su2 = y22 * S2, Su3 = y13 * S1 + y33 * S3;
+  const double Bd1 = g.d11 * B1 + g.d13 * B3, Bd2 = g.d22 * B2, Bd3 = g.d13 * B1 + g.d33 * B3;
+  const double Bsq = Bd1 * B1 + Bd2 * B2 + Bd3 * B3;
+  const double gm = (double)gam, lf = CV(nwini - 2), xi = CV(nwini - 1);
+  const double rho = CV(0) / lf;
+  double uu = (gm - 1.) / gm * (xi / (lf * lf) - rho) / (gm - 1.);
+  if (uu < 0) {
+    uu = (CV(8) / CV(0)) * pow(rho, gm - 1) / (gm - 1.);
+    if (uu < 0) uu = 1e-14;
+  }
+  const double den = xi + Bsq;
+  out[0] = T(rho);
+  out[1] = T(uu);
+  out[2] = T((Su1 / den + B1 * BS / (xi * den)) * lf);
+  out[3] = T((Su2 / den + B2 * BS / (xi * den)) * lf);
+  out[4] = T((Su3 / den + B3 * BS / (xi * den)) * lf);
+  out[5] = T(bpol * B1);
+  out[6] = T(bpol * B2);
+  out[7] = T(bpol * B3);
+}
+
+// ============================================================================ host runtime
+static double now_s() {
+  using clk = std::chrono::steady_clock;
+  return std::chrono::duration<double>(clk::now().time_since_epoch()).count();
+}
+
+template <class T>
+struct Tables {
+  T* blk = nullptr;
+  T* prim = nullptr;
+};
+
+}  // namespace phx
+
+struct phx_snapshot {
+  int tables = 0;
+  phx_model geo{};  // grid description (no host pointers)
+  phx::Tables<float> t32;
+  phx::Tables<double> t64;
+  int* map = nullptr;
+  int fd[3] = {0, 0, 0};
+  int64_t bytes = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  phx_freqout* d_res = nullptr;
+  size_t res_cap = 0;
+  int* d_jobs = nullptr;
+  size_t jobs_cap = 0;
+  int* d_counter = nullptr;
+  unsigned long long* d_tot = nullptr;
+  double decode_s = 0., upload_s = 0.;
+  int64_t last_nres = 0;
+  int device = 0;
+  std::vector<void*> allocs;
+};
+
+namespace phx {
+
+static int g_device = 0;
+static std::thread* g_warm = nullptr;
+
+static void warm_wait() {
+  if (g_warm) {
+    g_warm->join();
+    delete g_warm;
+    g_warm = nullptr;
+  }
+}
+
+template <class P>
+static P* dalloc(phx_snapshot* s, size_t bytes) {
+  void* d = nullptr;
+  HIPCHECK(hipMalloc(&d, bytes ? bytes : 16));
+  s->allocs.push_back(d);
+  s->bytes += (int64_t)bytes;
+  return (P*)d;
+}
+
+static void snapshot_free(phx_snapshot* s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (void* p : s->allocs) (void)hipFree(p);
+  if (s->d_res) (void)hipFree(s->d_res);
+  if (s->d_jobs) (void)hipFree(s->d_jobs);
+  if (s->e0) (void)hipEventDestroy(s->e0);
+  if (s->e1) (void)hipEventDestroy(s->e1);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+// block records: lower corner, upper corner (lo + nx * dxc in the table's precision, as
+// src/core/model/sampler.hpp:93-97 forms it), cell size, padding
+template <class T>
+static std::vector<T> block_records(const phx_model* m) {
+#pragma clang fp contract(off)
+  std::vector<T> rec((size_t)m->nleafs * 12, T(0));
+  for (int q = 0; q < m->nleafs; q++)
+    for (int d = 0; d < 3; d++) {
+      const T lo = T(m->lb[3 * q + d]), dx = T(m->dxc[3 * q + d]);
+      const T span = T(m->nx[d]) * dx;
+      rec[12 * q + d] = lo;
+      rec[12 * q + 3 + d] = lo + span;
+      rec[12 * q + 6 + d] = dx;
+    }
+  return rec;
+}
+
+// finest-level map whose entries are the first leaf of the coarse-to-fine walk of
+// find_leaf_map (src/core/model/sampler.hpp:76-80)
+static std::vector<int> flat_map(const phx_model* m, int fd[3]) {
+  const int top = m->nlevels - 1;
+  for (int d = 0; d < 3; d++) fd[d] = m->level_dims[top][d];
+  const size_t n = (size_t)fd[0] * fd[1] * fd[2];
+  if (n > ((size_t)1 << 28)) throw Error(PHX_E_RANGE, "flattened AMR map above 2^28 entries");
+  std::vector<int> out(n, -1);
+  for (int i = 0; i < fd[0]; i++)
+    for (int j = 0; j < fd[1]; j++)
+      for (int k = 0; k < fd[2]; k++) {
+        int leaf = -1;
+        for (int L = 0; L <= top && leaf < 0; L++) {
+          const int s = top - L;
+          leaf = m->level_map[L][((size_t)(i >> s) * m->level_dims[L][1] + (j >> s)) * m->level_dims[L][2] + (k >> s)];
+        }
+        out[((size_t)i * fd[1] + j) * fd[2] + k] = leaf;
+      }
+  return out;
+}
+
+template <class T>
+static void upload_tables(phx_snapshot* s, const phx_model* m, const char* dump, Tables<T>& tb,
+                          const std::vector<double>* raw, int per, int nchunks) {
+  const std::vector<T> rec = block_records<T>(m);
+  tb.blk = dalloc<T>(s, rec.size() * sizeof(T));
+  HIPCHECK(hipMemcpyAsync(tb.blk, rec.data(), rec.size() * sizeof(T), hipMemcpyHostToDevice, s->stream));
+  const size_t nprim = (size_t)m->nleafs * m->cells * 8;
+  tb.prim = dalloc<T>(s, nprim * sizeof(T));
+  if (m->prim) {
+    std::vector<T> p(nprim);
+    for (size_t i = 0; i < nprim; i++) p[i] = T(m->prim[i]);
+    HIPCHECK(hipMemcpyAsync(tb.prim, p.data(), nprim * sizeof(T), hipMemcpyHostToDevice, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return;
+  }
+  (void)dump;
+  (void)raw;
+  (void)per;
+  (void)nchunks;
+}
+
+// Streams the dump's leaf blocks through two pinned buffers and decodes them on the device
+// (the reference's decode_to_device, src/device/kernel.cu:200-260).
+static void decode_dump(phx_snapshot* s, const phx_model* m, const char* dump) {
+  const double t0 = now_s();
+  const size_t per_leaf = (size_t)m->nwini * m->cells;
+  const long stag = (long)(m->nx[0] + 1) * (m->nx[1] + 1) * (m->nx[2] + 1) * m->nws * 8;
+  int per = (int)((64u << 20) / (per_leaf * sizeof(double)));
+  per = std::max(1, std::min(per, m->nleafs));
+  FILE* f = std::fopen(dump, "rb");
+  if (!f) throw Error(PHX_E_IO, std::string("cannot open ") + dump);
+  struct Closer {
+    FILE* f;
+    ~Closer() { std::fclose(f); }
+  } closer{f};
+  double* h_raw[2] = {nullptr, nullptr};
+  double* d_raw[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(s->stream);
+    for (int b = 0; b < 2; b++) {
+      if (h_raw[b]) (void)hipHostFree(h_raw[b]);
+      if (d_raw[b]) (void)hipFree(d_raw[b]);
+      if (ev[b]) (void)hipEventDestroy(ev[b]);
+    }
+  };
+  try {
+    for (int b = 0; b < 2; b++) {
+      HIPCHECK(hipHostMalloc((void**)&h_raw[b], (size_t)per * per_leaf * sizeof(double), hipHostMallocDefault));
+      HIPCHECK(hipMalloc((void**)&d_raw[b], (size_t)per * per_leaf * sizeof(double)));
+      HIPCHECK(hipEventCreateWithFlags(&ev[b], hipEventDisableTiming));
+      HIPCHECK(hipEventRecord(ev[b], s->stream));
+    }
+    int nb = 0;
+    for (int l0 = 0; l0 < m->nleafs; l0 += per, nb ^= 1) {
+      const int nl = std::min(per, m->nleafs - l0);
+      HIPCHECK(hipEventSynchronize(ev[nb]));  // buffer nb free again
+      for (int q = 0; q < nl; q++) {
+        if (std::fread(h_raw[nb] + (size_t)q * per_leaf, sizeof(double), per_leaf, f) != per_leaf)
+          throw Error(PHX_E_IO, "short read in the data, leaf " + std::to_string(l0 + q));
+        std::fseek(f, stag, SEEK_CUR);
+      }
+      HIPCHECK(hipMemcpyAsync(d_raw[nb], h_raw[nb], (size_t)nl * per_leaf * sizeof(double), hipMemcpyHostToDevice,
+                              s->stream));
+      const size_t cells = (size_t)nl * m->cells;
+      const unsigned grid = (unsigned)((cells + BLOCK - 1) / BLOCK);
+      if (s->tables & PHX_TABLE_FP32)
+        k_decode<float><<<grid, BLOCK, 0, s->stream>>>(d_raw[nb], l0, nl, m->cells, m->nwini, m->nx[0], m->nx[1],
+                                                       s->t32.blk, (float)m->a, (float)m->hslope, (float)m->gam,
+                                                       m->bpol, s->t32.prim);
+      if (s->tables & PHX_TABLE_FP64)
+        k_decode<double><<<grid, BLOCK, 0, s->stream>>>(d_raw[nb], l0, nl, m->cells, m->nwini, m->nx[0], m->nx[1],
+                                                        s->t64.blk, m->a, m->hslope, m->gam, m->bpol, s->t64.prim);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipEventRecord(ev[nb], s->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(s->stream));
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+  cleanup();
+  s->decode_s = now_s() - t0;
+}
+
+static void check_model(const phx_model* m) {
+  if (!m) throw Error(PHX_E_ARG, "null model");
+  if (m->nleafs <= 0 || m->cells <= 0 || !m->lb || !m->dxc) throw Error(PHX_E_ARG, "model has no block tables");
+  if (m->nlevels < 1 || m->nlevels > PHX_MAX_LEVELS) throw Error(PHX_E_ARG, "model nlevels out of range");
+  for (int l = 0; l < m->nlevels; l++)
+    if (!m->level_map[l]) throw Error(PHX_E_ARG, "model level map missing");
+  if (m->nx[0] * m->nx[1] * m->nx[2] != m->cells) throw Error(PHX_E_ARG, "cells != nx0 nx1 nx2");
+  if (m->nwini < 9) throw Error(PHX_E_ARG, "nwini < 9");
+}
+
+static phx_snapshot* snapshot_open(const phx_model* m, const char* dump, int tables) {
+  check_model(m);
+  if (!(tables & (PHX_TABLE_FP32 | PHX_TABLE_FP64))) throw Error(PHX_E_ARG, "no table precision requested");
+  if (!m->prim && !dump) throw Error(PHX_E_ARG, "no primitives and no dump file");
+  warm_wait();
+  HIPCHECK(hipSetDevice(g_device));
+  std::unique_ptr<phx_snapshot, void (*)(phx_snapshot*)> s(new phx_snapshot(), snapshot_free);
+  const double t0 = now_s();
+  s->device = g_device;
+  s->tables = tables;
+  s->geo = *m;
+  s->geo.lb = s->geo.dxc = s->geo.prim = nullptr;
+  s->geo.leaf_level = s->geo.leaf_index = nullptr;
+  for (int l = 0; l < PHX_MAX_LEVELS; l++) s->geo.level_map[l] = nullptr;
+  HIPCHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  HIPCHECK(hipEventCreate(&s->e0));
+  HIPCHECK(hipEventCreate(&s->e1));
+  s->d_counter = dalloc<int>(s.get(), 64 * sizeof(int));
+  s->d_tot = dalloc<unsigned long long>(s.get(), 4 * sizeof(unsigned long long));
+  const std::vector<int> fm = flat_map(m, s->fd);
+  s->map = dalloc<int>(s.get(), fm.size() * sizeof(int));
+  HIPCHECK(hipMemcpyAsync(s->map, fm.data(), fm.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+  if (tables & PHX_TABLE_FP32) upload_tables<float>(s.get(), m, dump, s->t32, nullptr, 0, 0);
+  if (tables & PHX_TABLE_FP64) upload_tables<double>(s.get(), m, dump, s->t64, nullptr, 0, 0);
+  HIPCHECK(hipStreamSynchronize(s->stream));
+  s->upload_s = now_s() - t0;
+  if (!m->prim) decode_dump(s.get(), m, dump);
+  return s.release();
+}
+
+template <class T>
+static Grid<T> grid_of(const phx_snapshot* s, const Tables<T>& tb) {
+  const phx_model& m = s->geo;
+  Grid<T> G;
+  G.blk = tb.blk;
+  G.prim = tb.prim;
+  G.map = s->map;
+  G.nleafs = m.nleafs;
+  G.cells = m.cells;
+  G.nx0 = m.nx[0];
+  G.nx1 = m.nx[1];
+  G.nx2 = m.nx[2];
+  G.fd0 = s->fd[0];
+  G.fd1 = s->fd[1];
+  G.fd2 = s->fd[2];
+  G.w0 = T(m.lvl_w[0][0]);
+  G.w1 = T(m.lvl_w[0][1]);
+  G.w2 = T(m.lvl_w[0][2]);
+  G.up = T(1 << (m.nlevels - 1));
+  G.lo1 = T(m.x_lo[1]);
+  G.lo2 = T(m.x_lo[2]);
+  G.lo3 = T(m.x_lo[3]);
+  G.hi1 = T(m.x_hi[1]);
+  G.hi2 = T(m.x_hi[2]);
+  G.hi3 = T(m.x_hi[3]);
+  return G;
+}
+
+template <class T, int NF>
+static void launch_march(phx_snapshot* s, const Tables<T>& tb, const phx_model* rq, int f0, int nf,
+                         const int* d_jobs, int j0, int j1, int count_stats, phx_info* info) {
+  MarchArgs<T, NF> A;
+  std::memset(&A, 0, sizeof A);
+  A.G = grid_of<T>(s, tb);
+  const phx_model& m = s->geo;
+  A.P.a = T(m.a);
+  A.P.hs = T(m.hslope);
+  A.P.gam = T(m.gam);
+  A.P.Ne = T(rq->Ne);
+  A.P.Bu = T(rq->Bu);
+  A.P.rhigh = T(rq->rhigh);
+  A.P.rlow = T(rq->rlow);
+  A.P.ibc = T(rq->ibc);
+  A.P.sigma_cut = T(rq->sigma_cut);
+  A.P.te_min = T(rq->thetae_min);
+  A.P.te_max = T(rq->thetae_max);
+  A.P.rt_outer = T(rq->rt_outer);
+  A.C.a = (double)T(m.a);  // the camera uses the table-precision spin and stretching (camera.hpp:77)
+  A.C.hs = (double)T(m.hslope);
+  A.C.incl = rq->inclination;
+  A.C.r_cam = rq->r_cam;
+  A.C.t_init = rq->t_init;
+  A.C.phi_cam = rq->phi_cam;
+  A.C.fov_x = rq->fov_x;
+  A.C.fov_y = rq->fov_y;
+  A.C.cos_pa = rq->cos_pa;
+  A.C.sin_pa = rq->sin_pa;
+  A.C.W = rq->img_width;
+  A.C.H = rq->img_height;
+  A.stepsize = T(rq->stepsize);
+  A.r_out = T(rq->r_out);
+  A.cut_in = T(rq->cut_in);
+  A.max_steps = rq->max_steps;
+  A.max_order = rq->max_order;
+  A.nf_total = rq->nfreqs;
+  A.f0 = f0;
+  A.nf = nf;
+  A.count_stats = count_stats;
+  for (int f = 0; f < nf; f++) {
+    A.freq[f] = T(rq->freq[f0 + f]);
+    A.hconst[f] = T(rq->hconst[f0 + f]);
+    A.jscale[f] = T(rq->jscale[f0 + f]);
+  }
+  A.jobs = d_jobs;
+  A.j0 = j0;
+  A.j1 = j1;
+  A.counter = s->d_counter;
+  A.res = s->d_res;
+  A.tot = s->d_tot;
+  int cus = 0, per_cu = 0;
+  HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
+  HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march<T, NF>, BLOCK, 0));
+  per_cu = std::max(per_cu, 1);
+  int grid = per_cu * cus;
+  const int njobs = j1 - j0;
+  grid = std::max(1, std::min(grid, (njobs + BLOCK - 1) / BLOCK));
+  HIPCHECK(hipMemsetAsync(s->d_counter, 0, sizeof(int), s->stream));
+  k_march<T, NF><<<grid, BLOCK, 0, s->stream>>>(A);
+  HIPCHECK(hipGetLastError());
+  if (info) {
+    hipFuncAttributes fa;
+    HIPCHECK(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_march<T, NF>)));
+    info->vgprs = fa.numRegs;
+    info->scratch_bytes = (int32_t)fa.localSizeBytes;
+    info->lds_bytes = (int32_t)fa.sharedSizeBytes;
+    info->cus = cus;
+    info->blocks = grid;
+    info->block = BLOCK;
+    info->waves_per_cu = per_cu * BLOCK / 64;
+  }
+}
+
+template <class T>
+static void march_all(phx_snapshot* s, const Tables<T>& tb, const phx_model* rq, const int* d_jobs, int njobs,
+                      phx_info* info, phx_progress_fn progress, void* user) {
+  int chunk = njobs;
+  if (const char* e = std::getenv("PHX_CHUNK"))
+    if (std::atoi(e) > 0) chunk = std::atoi(e);
+  const int nfr = rq->nfreqs;
+  for (int f0 = 0; f0 < nfr; f0 += 8) {
+    const int nf = std::min(8, nfr - f0);
+    for (int j0 = 0; j0 < njobs; j0 += chunk) {
+      const int j1 = std::min(njobs, j0 + chunk);
+      const int stats = f0 == 0;
+      if (nf == 1)
+        launch_march<T, 1>(s, tb, rq, f0, nf, d_jobs, j0, j1, stats, info);
+      else if (nf == 2)
+        launch_march<T, 2>(s, tb, rq, f0, nf, d_jobs, j0, j1, stats, info);
+      else if (nf <= 4)
+        launch_march<T, 4>(s, tb, rq, f0, nf, d_jobs, j0, j1, stats, info);
+      else
+        launch_march<T, 8>(s, tb, rq, f0, nf, d_jobs, j0, j1, stats, info);
+      if (progress) {
+        HIPCHECK(hipStreamSynchronize(s->stream));
+        if (f0 + nf >= nfr) progress(j1 - j0, user);
+      }
+    }
+  }
+}
+
+static void check_request(const phx_snapshot* s, const phx_model* rq, int precision) {
+  if (!rq) throw Error(PHX_E_ARG, "null request");
+  if (rq->img_width <= 0 || rq->img_height <= 0) throw Error(PHX_E_ARG, "image size must be positive");
+  if ((int64_t)rq->img_width * rq->img_height > (int64_t)1 << 30) throw Error(PHX_E_RANGE, "image too large");
+  if (rq->nfreqs < 1 || rq->nfreqs > PHX_MAX_FREQS) throw Error(PHX_E_ARG, "nfreqs must be in 1..32");
+  if (precision == PHX_FP32 && !(s->tables & PHX_TABLE_FP32)) throw Error(PHX_E_ARG, "snapshot has no FP32 table");
+  if (precision == PHX_FP64 && !(s->tables & PHX_TABLE_FP64)) throw Error(PHX_E_ARG, "snapshot has no FP64 table");
+  if (precision != PHX_FP32 && precision != PHX_FP64)
+    throw Error(PHX_E_ARG, "precision must be PHX_FP32 or PHX_FP64");
+}
+
+static void render(phx_snapshot* s, const phx_model* rq, int precision, const int32_t* jobs, int32_t njobs,
+                   phx_freqout* res, phx_info* info, phx_progress_fn progress, void* user) {
+  check_request(s, rq, precision);
+  HIPCHECK(hipSetDevice(s->device));
+  const int64_t npix = (int64_t)rq->img_width * rq->img_height;
+  std::vector<int32_t> order;
+  if (!jobs) {
+    order.resize(npix);
+    if (phx_schedule(rq, 1, order.data(), (int32_t)npix) != PHX_OK) throw Error(PHX_E_ARG, phx_last_error());
+    jobs = order.data();
+    njobs = (int32_t)npix;
+  }
+  if (njobs < 0 || njobs > npix) throw Error(PHX_E_ARG, "njobs out of range");
+  for (int32_t n = 0; n < njobs; n++)
+    if (jobs[n] < 0 || jobs[n] >= npix) throw Error(PHX_E_ARG, "job index outside the image");
+  const size_t nres = (size_t)npix * rq->nfreqs;
+  if (nres > s->res_cap) {
+    if (s->d_res) HIPCHECK(hipFree(s->d_res));
+    s->d_res = nullptr;
+    HIPCHECK(hipMalloc((void**)&s->d_res, nres * sizeof(phx_freqout)));
+    s->res_cap = nres;
+  }
+  if ((size_t)njobs > s->jobs_cap) {
+    if (s->d_jobs) HIPCHECK(hipFree(s->d_jobs));
+    s->d_jobs = nullptr;
+    HIPCHECK(hipMalloc((void**)&s->d_jobs, std::max<size_t>(njobs, 1) * sizeof(int32_t)));
+    s->jobs_cap = std::max<size_t>(njobs, 1);
+  }
+  phx_info local;
+  if (!info) info = &local;
+  std::memset(info, 0, sizeof *info);
+  hipDeviceProp_t prop;
+  HIPCHECK(hipGetDeviceProperties(&prop, s->device));
+  std::snprintf(info->device, sizeof info->device, "%s (%s)", prop.name, prop.gcnArchName);
+  HIPCHECK(hipMemcpyAsync(s->d_jobs, jobs, (size_t)njobs * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+  if ((int64_t)njobs < npix) HIPCHECK(hipMemsetAsync(s->d_res, 0, nres * sizeof(phx_freqout), s->stream));
+  HIPCHECK(hipMemsetAsync(s->d_tot, 0, 4 * sizeof(unsigned long long), s->stream));
+  HIPCHECK(hipEventRecord(s->e0, s->stream));
+  if (precision == PHX_FP32)
+    march_all<float>(s, s->t32, rq, s->d_jobs, njobs, info, progress, user);
+  else
+    march_all<double>(s, s->t64, rq, s->d_jobs, njobs, info, progress, user);
+  HIPCHECK(hipEventRecord(s->e1, s->stream));
+  HIPCHECK(hipEventSynchronize(s->e1));
+  float ms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&ms, s->e0, s->e1));
+  unsigned long long tot[4] = {0, 0, 0, 0};
+  HIPCHECK(hipMemcpy(tot, s->d_tot, sizeof tot, hipMemcpyDeviceToHost));
+  info->kernel_s = ms * 1e-3;
+  info->steps = (int64_t)tot[0];
+  info->nplasma = (int64_t)tot[1];
+  info->bad = (int64_t)tot[2];
+  info->rays = (int64_t)njobs * rq->nfreqs;
+  info->decode_s = s->decode_s;
+  info->upload_s = s->upload_s;
+  s->last_nres = (int64_t)nres;
+  if (res) HIPCHECK(hipMemcpy(res, s->d_res, nres * sizeof(phx_freqout), hipMemcpyDeviceToHost));
+}
+
+}  // namespace phx
+
+using namespace phx;
+
+extern "C" {
+
+int phx_set_device(int32_t device) {
+  return guarded([&] {
+    int n = 0;
+    HIPCHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) throw Error(PHX_E_ARG, "no such device");
+    g_device = device;
+  });
+}
+
+int phx_dev_warmup(void) {
+  return guarded([&] {
+    if (!g_warm) g_warm = new std::thread([] { (void)hipFree(nullptr); });
+  });
+}
+
+int phx_dev_render(const phx_model* model, const char* dump, int32_t precision, const int32_t* jobs, int32_t njobs,
+                   phx_freqout* res, phx_info* info, phx_progress_fn progress, void* user) {
+  return guarded([&] {
+    if (!res) throw Error(PHX_E_ARG, "res must not be NULL");
+    const int tables = precision == PHX_FP64 ? PHX_TABLE_FP64 : PHX_TABLE_FP32;
+    std::unique_ptr<phx_snapshot, void (*)(phx_snapshot*)> s(snapshot_open(model, dump, tables), snapshot_free);
+    render(s.get(), model, precision, jobs, njobs, res, info, progress, user);
+  });
+}
+
+int phx_snapshot_open(const phx_model* model, const char* dump, int32_t tables, phx_snapshot** out) {
+  return guarded([&] {
+    if (!out) throw Error(PHX_E_ARG, "out must not be NULL");
+    *out = snapshot_open(model, dump, tables);
+  });
+}
+
+int phx_snapshot_close(phx_snapshot* s) {
+  return guarded([&] { snapshot_free(s); });
+}
+
+int64_t phx_snapshot_bytes(const phx_snapshot* s) { return s ? s->bytes : 0; }
+
+int phx_snapshot_render(phx_snapshot* s, const phx_model* rq, int32_t precision, const int32_t* jobs, int32_t njobs,
+                        phx_freqout* res, phx_info* info) {
+  return guarded([&] {
+    if (!s) throw Error(PHX_E_ARG, "null snapshot");
+    render(s, rq, precision, jobs, njobs, res, info, nullptr, nullptr);
+  });
+}
+
+int phx_snapshot_result(phx_snapshot* s, phx_freqout* res, int64_t nres) {
+  return guarded([&] {
+    if (!s || !res || nres != s->last_nres) throw Error(PHX_E_ARG, "result size mismatch");
+    HIPCHECK(hipMemcpy(res, s->d_res, (size_t)nres * sizeof(phx_freqout), hipMemcpyDeviceToHost));
+  });
+}
+
+int phx_snapshot_prims(phx_snapshot* s, int32_t table, double* out, int64_t n) {
+  return guarded([&] {
+    const int64_t want = (int64_t)s->geo.nleafs * s->geo.cells * 8;
+    if (!s || !out || n != want) throw Error(PHX_E_ARG, "prims size mismatch");
+    if (table == PHX_TABLE_FP64 && s->t64.prim) {
+      HIPCHECK(hipMemcpy(out, s->t64.prim, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else if (table == PHX_TABLE_FP32 && s->t32.prim) {
+      std::vector<float> tmp(n);
+      HIPCHECK(hipMemcpy(tmp.data(), s->t32.prim, n * sizeof(float), hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < n; i++) out[i] = tmp[i];
+    } else {
+      throw Error(PHX_E_ARG, "snapshot has no such table");
+    }
+  });
+}
+
+}  // extern "C"

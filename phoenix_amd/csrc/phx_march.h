@@ -1,0 +1,773 @@
+// Device functions of the polarized GRRT march (gfx950), templated on the
+// scalar type T (float or double).  The physics is that of the reference's
+// src/core/ (cited per function); the layout is MI355X-first: per-lane state in
+// VGPRs, the sparse Kerr-Schild metric (7 non-zero covariant components), one
+// AMR lookup per geodesic step, a flattened finest-level leaf map, 32-byte cell
+// records gathered with 16-byte loads, and a fully unrolled Gauss-Jordan
+// inverse whose partial pivoting is done with selects (no scratch memory).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "phx_bessel_table.h"
+
+namespace phxd {
+
+#define PHX_DEV __device__ __forceinline__
+
+// ------------------------------------------------------------------ constants
+// CGS constants of the reference (src/core/numerics/constants.hpp:22-81)
+constexpr double MP = 1.6726219e-24, ME = 9.1093829e-28, QE = 4.80320425e-10;
+constexpr double CL = 2.99792458e10, HP = 6.62606885e-27;
+constexpr double H_OVER_MEC2 = HP / (ME * CL * CL);
+constexpr double NU_CRIT = 3. * QE / (4. * M_PI * ME * CL);
+constexpr double JNU = QE * QE / (2. * 1.73205080756887729353 * CL);
+constexpr double WP2 = 4. * M_PI * QE * QE / ME;
+constexpr double GYRO = QE / (ME * CL);
+constexpr double PI_D = 3.14159265358979323846;
+
+template <class T> struct Lim;
+template <> struct Lim<float> {  // src/core/numerics/precision.hpp:46-51
+  static constexpr float tiny = 1e-30f, big = 1e30f, tol_series = 1e-9f, tol_asym = 1e-8f;
+  static constexpr bool fp32 = true;
+};
+template <> struct Lim<double> {  // precision.hpp:53-58
+  static constexpr double tiny = 1e-160, big = 1e100, tol_series = 1e-18, tol_asym = 1e-17;
+  static constexpr bool fp32 = false;
+};
+
+PHX_DEV void scs(float x, float* s, float* c) { sincosf(x, s, c); }
+PHX_DEV void scs(double x, double* s, double* c) { sincos(x, s, c); }
+
+// ------------------------------------------------------------------ metric
+// Kerr-Schild in MKS-BHAC coordinates (t, ln r, x2, phi), theta = x2 + h/2 sin 2x2
+// (src/core/geometry/metric.hpp:26-142).  Only the 7 non-zero covariant and 6
+// non-zero contravariant components are kept.
+template <class T>
+struct Met {
+  T d00, d01, d03, d11, d13, d22, d33;  // g_mu nu
+  T r, s, c, hf;                        // r, sin/cos theta, d theta / d x2
+};
+template <class T>
+PHX_DEV Met<T> metric(T a, T hs, T x1, T x2) {
+  Met<T> g;
+  const T r = exp(x1);
+  T s2x, c2x, s, c;
+  scs(T(2) * x2, &s2x, &c2x);
+  scs(x2 + T(0.5) * hs * s2x, &s, &c);
+  const T s2 = s * s, rho2 = r * r + a * a * c * c, hf = T(1) + hs * c2x;
+  const T z = T(2) * r / rho2;
+  g.d00 = -T(1) + z;
+  g.d01 = z * r;
+  g.d03 = -T(2) * a * r * s2 / rho2;
+  g.d11 = (T(1) + z) * r * r;
+  g.d13 = -a * s2 * (T(1) + z) * r;
+  g.d22 = rho2 * hf * hf;
+  g.d33 = s2 * (rho2 + a * a * s2 * (T(1) + z));
+  g.r = r;
+  g.s = s;
+  g.c = c;
+  g.hf = hf;
+  return g;
+}
+template <class T>
+struct MetU {  // contravariant part (metric.hpp:287-292)
+  T u00, u01, u11, u13, u22, u33;
+};
+template <class T>
+PHX_DEV MetU<T> metric_up(T a, const Met<T>& g) {
+  MetU<T> u;
+  const T r = g.r, rho2 = r * r + a * a * g.c * g.c, ir = T(1) / rho2;
+  u.u00 = -T(1) - T(2) * r / rho2;
+  u.u01 = T(2) * ir;
+  u.u11 = ir * (r * (r - T(2)) + a * a) / (r * r);
+  u.u13 = a * ir / r;
+  u.u22 = ir / (g.hf * g.hf);
+  u.u33 = ir / (g.s * g.s);
+  return u;
+}
+template <class T>
+PHX_DEV T dot(const Met<T>& g, const T A[4], const T B[4]) {
+  return g.d00 * A[0] * B[0] + g.d01 * (A[0] * B[1] + A[1] * B[0]) + g.d03 * (A[0] * B[3] + A[3] * B[0]) +
+         g.d11 * A[1] * B[1] + g.d13 * (A[1] * B[3] + A[3] * B[1]) + g.d22 * A[2] * B[2] + g.d33 * A[3] * B[3];
+}
+template <class T>
+PHX_DEV void lower(const Met<T>& g, const T V[4], T out[4]) {
+  out[0] = g.d00 * V[0] + g.d01 * V[1] + g.d03 * V[3];
+  out[1] = g.d01 * V[0] + g.d11 * V[1] + g.d13 * V[3];
+  out[2] = g.d22 * V[2];
+  out[3] = g.d03 * V[0] + g.d13 * V[1] + g.d33 * V[3];
+}
+
+// ------------------------------------------------------------------ geodesic
+// Per-point quantities of the RHS (src/core/geometry/geodesic.hpp:81-109)
+template <class T>
+struct Geo {
+  T a, a2, r, r2, tp, tp2, tpp, s, c, s2, s2th, cot, Sig, Sb, Del, A, B, C;
+};
+template <class T>
+PHX_DEV Geo<T> geo_at(T a, T hs, T x1, T x2) {
+  Geo<T> G;
+  G.a = a;
+  G.a2 = a * a;
+  G.r = exp(x1);
+  G.r2 = G.r * G.r;
+  T s2x, c2x;
+  scs(T(2) * x2, &s2x, &c2x);
+  G.tp = T(1) + hs * c2x;
+  G.tp2 = G.tp * G.tp;
+  G.tpp = -T(2) * hs * s2x;
+  scs(x2 + T(0.5) * hs * s2x, &G.s, &G.c);
+  G.s2 = G.s * G.s;
+  G.s2th = T(2) * G.s * G.c;
+  G.cot = G.c / G.s;
+  G.Sig = G.r2 + G.a2 * G.c * G.c;
+  G.Sb = T(1) / G.Sig;
+  G.Del = G.r2 - T(2) * G.r + G.a2;
+  G.A = G.Sig * G.Del + T(2) * G.r * (G.r2 + G.a2);
+  G.B = (T(2) * G.r2 - G.Sig) * G.Sb * G.Sb * G.Sb;
+  G.C = G.r * G.Sb - G.a2 * G.B * G.s2;
+  return G;
+}
+// -Gamma^i_jk U^j V^k in closed form (geodesic.hpp:114-169)
+template <class T>
+PHX_DEV void chris(const Geo<T>& G, const T U[4], const T V[4], T out[4]) {
+  const T h = T(0.5), two = T(2), four = T(4);
+  const T P00 = U[0] * V[0];
+  const T P01 = h * (U[0] * V[1] + U[1] * V[0]);
+  const T P02 = h * (U[0] * V[2] + U[2] * V[0]);
+  const T P03 = h * (U[0] * V[3] + U[3] * V[0]);
+  const T P11 = U[1] * V[1];
+  const T P12 = h * (U[1] * V[2] + U[2] * V[1]);
+  const T P13 = h * (U[1] * V[3] + U[3] * V[1]);
+  const T P22 = U[2] * V[2];
+  const T P23 = h * (U[2] * V[3] + U[3] * V[2]);
+  const T P33 = U[3] * V[3];
+  const T a = G.a, a2 = G.a2, r = G.r, r2 = G.r2, tp = G.tp, tp2 = G.tp2, s = G.s, c = G.c, s2 = G.s2;
+  const T s2th = G.s2th, Sig = G.Sig, Sb = G.Sb, Sb2 = Sb * Sb, Sb3 = Sb2 * Sb;
+  const T Del = G.Del, A = G.A, B = G.B, C = G.C, cot = G.cot;
+  const T K = a2 * r * s2th * Sb2 * tp;
+  out[0] = -two * (-r2 * Sb * tp2 * P22 - r * C * s2 * P33 + K * (-P02 - r * P12 + a * s2 * P23) +
+                   B * ((Sig + two * r) * r * P01 + r * P00 + (Sig + r) * r2 * P11 - two * a * r * s2 * P03 -
+                        a * s2 * (Sig + two * r) * r * P13));
+  out[1] = -((T(1) - (two * Sig - Del) * B * r) * P11 +
+             (Del / r) * (B * P00 - two * a * B * s2 * P03 - r * Sb * tp2 * P22 - C * s2 * P33) +
+             two * ((Del - Sig) * B * P01 - a2 * s * c * Sb * tp * P12 + a * (r * Sb + (Sig - Del) * B) * s2 * P13));
+  out[2] = -((G.tpp / tp - a2 * s * c * Sb * tp) * P22 +
+             Sb3 *
+                 ((-a2 * r * s2th) * (P00 + r2 * P11) - s * c * (Del * Sig * Sig + two * r * (r2 + a2) * (r2 + a2)) * P33 -
+                  two * a2 * r * s2th * r * P01 + two * a * s * c * (A + Sig * (Sig - Del)) * r * P13 +
+                  two * a * r * (r2 + a2) * s2th * P03) /
+                 tp +
+             two * r * Sb * r * P12);
+  out[3] = -(a * (B * P00 + B * r2 * P11 - r * Sb * tp2 * P22 - C * s2 * P33) + two * a * B * r * P01 -
+             four * a * r * cot * Sb2 * tp * P02 - two * a * (Sig + two * r) * cot * Sb2 * r * tp * P12 -
+             two * a2 * B * s2 * P03 + two * (cot + a2 * r * s2th * Sb2) * tp * P23 + two * C * r * P13);
+}
+
+// Ray state: (hi, lo) pairs of x, k, F1; lo is only carried in FP32 (geodesic.hpp:11-14)
+template <class T>
+struct Ray {
+  T hi[12], lo[12];
+};
+template <class T>
+PHX_DEV void two_sum_add(T& hi, T& lo, T d) {  // TwoSum + Fast2Sum (precision.hpp:65-73)
+  const T s = hi + d;
+  const T bb = s - hi;
+  const T err = (hi - (s - bb)) + (d - bb);
+  lo += err;
+  const T s2 = s + lo;
+  lo = lo - (s2 - s);
+  hi = s2;
+}
+// RK4 of x, k (geodesic) and F1 (parallel transport) with the FP32 south-hemisphere
+// fold (geodesic.hpp:222-256).  Keeps k1 and k2 + k3 instead of all four stages.
+template <class T>
+PHX_DEV void rk4(T a, T hs, Ray<T>& g, T dt) {
+  constexpr bool F32 = Lim<T>::fp32;
+  const T pi_hi = T(PI_D), pi_lo = T(PI_D - (double)T(PI_D));
+  const bool south = F32 && g.hi[2] > T(0.5 * PI_D);
+  T k1[12], k23[12], inc[12];
+#pragma unroll
+  for (int i = 0; i < 12; i++) inc[i] = T(0);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    T y[12];
+#pragma unroll
+    for (int i = 0; i < 12; i++) y[i] = F32 ? g.hi[i] + (g.lo[i] + inc[i]) : g.hi[i] + inc[i];
+    if (south) {
+      y[2] = (pi_hi - g.hi[2]) + ((pi_lo - g.lo[2]) - inc[2]);
+      y[6] = -y[6];
+      y[10] = -y[10];
+    }
+    T d[12];
+    const Geo<T> G = geo_at(a, hs, y[1], y[2]);
+#pragma unroll
+    for (int i = 0; i < 4; i++) d[i] = y[4 + i];
+    chris(G, y + 4, y + 4, d + 4);
+    chris(G, y + 4, y + 8, d + 8);
+    if (south) {
+      d[2] = -d[2];
+      d[6] = -d[6];
+      d[10] = -d[10];
+    }
+    if (q == 0) {
+#pragma unroll
+      for (int i = 0; i < 12; i++) k1[i] = d[i];
+    } else if (q == 1) {
+#pragma unroll
+      for (int i = 0; i < 12; i++) k23[i] = d[i];
+    } else if (q == 2) {
+#pragma unroll
+      for (int i = 0; i < 12; i++) k23[i] = k23[i] + d[i];
+    } else {
+      const T dt6 = dt / T(6);
+#pragma unroll
+      for (int i = 0; i < 12; i++) {
+        const T incr = ((k1[i] + d[i]) + T(2) * k23[i]) * dt6;
+        if (F32)
+          two_sum_add(g.hi[i], g.lo[i], incr);
+        else
+          g.hi[i] += incr;
+      }
+      break;
+    }
+    const T wdt = (q == 2 ? T(1.0) : T(0.5)) * dt;
+#pragma unroll
+    for (int i = 0; i < 12; i++) inc[i] = wdt * d[i];
+  }
+}
+template <class T>
+PHX_DEV T pole_x2(const Ray<T>& g) {  // geodesic.hpp:200-205
+  constexpr bool F32 = Lim<T>::fp32;
+  const T pi_hi = T(PI_D), pi_lo = T(PI_D - (double)T(PI_D));
+  if (F32 && g.hi[2] > T(0.5 * PI_D)) return (pi_hi - g.hi[2]) + (pi_lo - g.lo[2]);
+  return F32 ? g.hi[2] + g.lo[2] : g.hi[2];
+}
+// RAPTOR step rule (geodesic.hpp:267-292): negative step, capped at 1/4 cell inside the grid
+template <class T>
+PHX_DEV T step_len(T stepsize, T x2, const T k[4], bool have_dxc, const T dxc[3]) {
+  const T S2 = Lim<T>::tiny;
+  const T ak1 = fabs(k[1]) + S2, ak2 = fabs(k[2]) + S2, ak3 = fabs(k[3]) + S2;
+  const T d1 = stepsize / ak1;
+  const T d2 = stepsize * fmin(x2, T(PI_D) - x2) / ak2;
+  const T d3 = stepsize / ak3;
+  const T i1 = T(1) / (fabs(d1) + S2), i2 = T(1) / (fabs(d2) + S2), i3 = T(1) / (fabs(d3) + S2);
+  T grid = Lim<T>::big;
+  if (have_dxc) {
+    const T s1 = dxc[0] / ak1, s2 = dxc[1] / ak2, s3 = dxc[2] / ak3;
+    const T j1 = T(1) / (fabs(s1) + S2), j2 = T(1) / (fabs(s2) + S2), j3 = T(1) / (fabs(s3) + S2);
+    grid = T(1) / (j1 + j2 + j3) / T(4);
+  }
+  return -fmin(T(1) / (i1 + i2 + i3), grid);
+}
+
+// ------------------------------------------------------------------ AMR lookup
+// The snapshot's device tables: one 12-scalar record per leaf (lower corner, upper
+// corner = lo + nx * dxc formed on the host exactly as the reference forms it,
+// cell size), 8-scalar cell records, and the finest-level leaf map in which each
+// entry already holds the first leaf of the reference's coarse-to-fine walk
+// (src/core/model/sampler.hpp:55-82).
+template <class T>
+struct Grid {
+  const T* blk;    // [nleafs][12]
+  const T* prim;   // [nleafs * cells][8]
+  const int* map;  // [fd0][fd1][fd2]
+  int nleafs, cells, nx0, nx1, nx2, fd0, fd1, fd2;
+  T w0, w1, w2, up;  // base-level block widths, 2^(nlevels-1)
+  T lo1, lo2, lo3, hi1, hi2, hi3;
+};
+template <class T>
+PHX_DEV void wrap(T x[4]) {  // sampler.hpp:29-38
+  const T TP = T(2. * PI_D), PI = T(PI_D);
+  x[3] = fmod(x[3], TP);
+  x[2] = fmod(x[2], PI) - T(1e-6);
+  if (x[3] < T(0)) x[3] = TP + x[3];
+  if (x[2] < T(0)) {
+    x[2] = -x[2];
+    x[3] = PI + x[3];
+  }
+}
+template <class T>
+PHX_DEV bool off_domain(const Grid<T>& G, const T x[4]) {  // sampler.hpp:43-50
+  return x[1] > G.hi1 || x[1] < G.lo1 || x[2] < G.lo2 || x[2] > G.hi2 || x[3] < G.lo3 || x[3] > G.hi3;
+}
+template <class T>
+PHX_DEV bool in_leaf(const Grid<T>& G, int q, const T x[4], T small) {  // half-open (sampler.hpp:87-101)
+  const T* b = G.blk + 12 * q;
+  return !(x[1] + small < b[0] || x[1] + small >= b[3] || x[2] + small < b[1] || x[2] + small >= b[4] ||
+           x[3] + small < b[2] || x[3] + small >= b[5]);
+}
+template <class T>
+PHX_DEV bool in_leaf_closed(const Grid<T>& G, int q, const T x[4]) {  // inclusive (sampler.hpp:109-122)
+  const T* b = G.blk + 12 * q;
+  return !(x[1] < b[0] || x[1] > b[3] || x[2] < b[1] || x[2] > b[4] || x[3] < b[2] || x[3] > b[5]);
+}
+// find_leaf (sampler.hpp:143-180) on an already wrapped position
+template <class T>
+PHX_DEV int find_leaf(const Grid<T>& G, const T x[4]) {
+  const T small = x[2] > T(PI_D / 2.) ? T(-1e-9) : T(1e-9);
+  int cand = -1;
+  {
+    const T fi = (x[1] + small - G.lo1) / G.w0 * G.up;
+    const T fj = (x[2] + small - G.lo2) / G.w1 * G.up;
+    const T fk = (x[3] + small - G.lo3) / G.w2 * G.up;
+    if (!(fi < T(0) || fj < T(0) || fk < T(0))) {
+      const int i = (int)fi, j = (int)fj, k = (int)fk;
+      if (i < G.fd0 && j < G.fd1 && k < G.fd2) cand = G.map[((size_t)i * G.fd1 + j) * G.fd2 + k];
+    }
+  }
+  if (cand >= 0 && in_leaf(G, cand, x, small)) return cand;
+  const T eps = T(1e-6);
+  if (cand < 0 && (x[1] < G.lo1 - eps || x[1] > G.hi1 + eps || x[2] < G.lo2 - eps || x[2] > G.hi2 + eps ||
+                   x[3] < G.lo3 - eps || x[3] > G.hi3 + eps))
+    return -1;
+  for (int q = 0; q < G.nleafs; q++)  // the reference's linear fallback scan
+    if (in_leaf(G, q, x, small)) return q;
+  return -1;
+}
+
+// ------------------------------------------------------------------ fluid
+template <class T>
+struct Fluid {
+  T U[4], b[4], ne, B, te;
+  T x[4];  // the (wrapped) position the sampler used
+  Met<T> g;
+};
+template <class T>
+struct Plasma {  // per-request plasma parameters (src/core/model/bhac.hpp:150-151)
+  T a, hs, gam, Ne, Bu, rhigh, rlow, ibc, sigma_cut, te_min, te_max, rt_outer;
+};
+PHX_DEV void load8(const float* p, float v[8]) {
+  const float4 a = *reinterpret_cast<const float4*>(p), b = *reinterpret_cast<const float4*>(p + 4);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+PHX_DEV void load8(const double* p, double v[8]) {
+  const double4 a = *reinterpret_cast<const double4*>(p), b = *reinterpret_cast<const double4*>(p + 4);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+// get_fluid_params + derive_fluid (src/core/model/sampler.hpp:305-427): wrap, cached-leaf test,
+// re-wrap and lookup on a miss, cell by truncation, weights from the cell's lower corner,
+// block-local trilinear gather of the 8 primitives, then U^mu, b^mu, |B|, Theta_e and the cuts.
+template <class T>
+PHX_DEV int sample_fluid(const Grid<T>& G, const Plasma<T>& P, const T X[4], int cache, Fluid<T>& fl) {
+  constexpr bool F32 = Lim<T>::fp32;
+  T x[4] = {X[0], X[1], X[2], X[3]};
+  wrap(x);
+  if (exp(x[1]) < T(1) || off_domain(G, x)) return 0;
+  int q = cache;
+  if (q == -1 || !in_leaf_closed(G, q, x)) {
+    wrap(x);
+    q = find_leaf(G, x);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) fl.x[i] = x[i];
+  if (q == -1) return 0;
+  const T* b = G.blk + 12 * q;
+  const int nxs[3] = {G.nx0, G.nx1, G.nx2};
+  int c[3];
+  T del[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const T lb = b[d], dx = b[6 + d];
+    const T u = (x[1 + d] - lb) / dx;
+    c[d] = (int)u;
+    if (c[d] >= nxs[d]) c[d] = nxs[d] - 1;
+    const int ix = F32 ? (int)floor(u) : (int)(u + T(1000)) - 1000;
+    del[d] = (ix < 0) ? T(0) : (ix > nxs[d] - 2) ? T(1) : (x[1 + d] - (T(ix) * dx + lb)) / dx;
+  }
+  const int ip = c[0] + 1 < G.nx0 ? c[0] + 1 : c[0];
+  const int jp = c[1] + 1 < G.nx1 ? c[1] + 1 : c[1];
+  const int kp = c[2] + 1 < G.nx2 ? c[2] + 1 : c[2];
+  const T w1 = T(1) - del[0], w2 = T(1) - del[1], w3 = T(1) - del[2];
+  const int ci[4] = {c[0], c[0], ip, ip}, cj[4] = {c[1], jp, c[1], jp};
+  const T wq[4] = {w1 * w2, w1 * del[1], del[0] * w2, del[0] * del[1]};
+  const T* base = G.prim + (size_t)q * G.cells * 8;
+  T qv[8];
+#pragma unroll
+  for (int a = 0; a < 8; a++) qv[a] = T(0);
+#pragma unroll
+  for (int z = 0; z < 2; z++) {
+    const int ck = z ? kp : c[2];
+    const T wz = z ? del[2] : w3;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      T v[8];
+      load8(base + (size_t)(ci[s] + cj[s] * G.nx0 + ck * G.nx0 * G.nx1) * 8, v);
+      const T w = wq[s] * wz;
+#pragma unroll
+      for (int a = 0; a < 8; a++) qv[a] += v[a] * w;
+    }
+  }
+  // derive_fluid
+  const T tiny = T(1.e-6);
+  const Met<T> g = metric(P.a, P.hs, x[1], x[2]);
+  const MetU<T> u = metric_up(P.a, g);
+  fl.g = g;
+  const T rho = qv[0], uu = qv[1];
+  const T gV[4] = {T(0), qv[2], qv[3], qv[4]};
+  const T Bp[4] = {T(0), qv[5], qv[6], qv[7]};
+  fl.ne = rho * P.Ne + tiny;
+  const T shift1 = u.u01 / (-u.u00);  // shift^2 = shift^3 = 0 in Kerr-Schild
+  const T alpha = T(1) / sqrt(-u.u00);
+  const T vv = g.d11 * gV[1] * gV[1] + T(2) * g.d13 * gV[1] * gV[3] + g.d22 * gV[2] * gV[2] + g.d33 * gV[3] * gV[3];
+  const T lf = sqrt(vv + T(1));
+  const T V1 = gV[1] / lf, V2 = gV[2] / lf, V3 = gV[3] / lf;
+  fl.U[0] = lf / alpha;
+  fl.U[1] = V1 * lf - shift1 * lf / alpha;
+  fl.U[2] = V2 * lf;
+  fl.U[3] = V3 * lf;
+  const T Vl1 = g.d11 * V1 + g.d13 * V3, Vl2 = g.d22 * V2, Vl3 = g.d13 * V1 + g.d33 * V3;
+  fl.b[0] = lf * (Bp[1] * Vl1 + Bp[2] * Vl2 + Bp[3] * Vl3) / alpha;
+#pragma unroll
+  for (int i = 1; i < 4; i++) fl.b[i] = (Bp[i] + alpha * fl.b[0] * fl.U[i]) / lf;
+  T bd[4];
+  lower(g, fl.b, bd);
+  const T Bsq = fabs(fl.b[0] * bd[0] + fl.b[1] * bd[1] + fl.b[2] * bd[2] + fl.b[3] * bd[3]) + tiny;
+  fl.B = sqrt(Bsq) * P.Bu;
+  const T beta = uu * (P.gam - T(1)) / (T(0.5) * (Bsq + tiny));
+  const T bb = beta * P.ibc;
+  const T b2 = bb * bb;
+  const T trat = P.rhigh * b2 / (T(1) + b2) + P.rlow / (T(1) + b2);
+  const T te_unit = T(1. / 3. * (MP / ME)) / (trat + T(1));
+  fl.te = (uu / rho) * te_unit;
+  if ((Bsq / (rho + T(1e-20)) > P.sigma_cut) || g.r > P.rt_outer || fl.te > P.te_max || fl.te < P.te_min) {
+    fl.ne = T(0);
+    return 0;
+  }
+  return 1;
+}
+
+// ------------------------------------------------------------------ tetrad
+// Plasma-frame tetrad columns e_(1) (perp) and e_(2) (para) (src/core/geometry/tetrad.hpp:51-101);
+// sqrt(-g) of Kerr-Schild/MKS in closed form, Sigma |sin theta| r (1 + h cos 2x2).
+template <class T>
+PHX_DEV void frame(const Met<T>& g, T a, const T k[4], const T U[4], const T b[4], T perp[4], T para[4]) {
+  const T om = -dot(g, k, U);
+  T eK[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) eK[i] = k[i] / om - U[i];
+  const T Be = dot(g, U, b);
+  const T Cc = dot(g, k, b) / om - Be;
+  const T b2 = dot(g, b, b);
+  const T Nc = sqrt(b2 + Be * Be - Cc * Cc);
+#pragma unroll
+  for (int i = 0; i < 4; i++) para[i] = (b[i] + Be * U[i] - Cc * eK[i]) / Nc;
+  T Ud[4], kd[4], bd[4];
+  lower(g, U, Ud);
+  lower(g, k, kd);
+  lower(g, b, bd);
+  const T sq = (g.r * g.r + a * a * g.c * g.c) * fabs(g.s) * g.r * g.hf;
+  const T s = T(-1) / (sq * om * Nc);
+  // Levi-Civita contraction eps^{i..} U_a k_c b_d (tetrad.hpp:70-80)
+  const T m01 = kd[0] * bd[1] - kd[1] * bd[0], m02 = kd[0] * bd[2] - kd[2] * bd[0], m03 = kd[0] * bd[3] - kd[3] * bd[0];
+  const T m12 = kd[1] * bd[2] - kd[2] * bd[1], m13 = kd[1] * bd[3] - kd[3] * bd[1], m23 = kd[2] * bd[3] - kd[3] * bd[2];
+  perp[0] = s * (Ud[1] * m23 - Ud[2] * m13 + Ud[3] * m12);
+  perp[1] = -s * (Ud[0] * m23 - Ud[2] * m03 + Ud[3] * m02);
+  perp[2] = s * (Ud[0] * m13 - Ud[1] * m03 + Ud[3] * m01);
+  perp[3] = -s * (Ud[0] * m12 - Ud[1] * m02 + Ud[2] * m01);
+}
+template <class T>
+PHX_DEV T pitch_angle(const Met<T>& g, const T k[4], const T b[4], const T U[4]) {  // tetrad.hpp:138-154
+  const T bn = sqrt(fabs(dot(g, b, b)));
+  if (bn == T(0)) return T(PI_D / 2.);
+  const T kk = fabs(dot(g, k, U));
+  T mu = dot(g, k, b) / (kk * bn);
+  if (fabs(mu) > T(1)) mu = mu / fabs(mu);
+  return acos(mu);
+}
+
+// ------------------------------------------------------------------ coefficients
+// Dexter (2016) thermal synchrotron fits and Faraday coefficients (src/core/transfer/coeffs.hpp:44-138)
+template <class T>
+struct Coef {
+  T jI, jQ, jU, jV, aI, aQ, aU, aV, rQ, rU, rV;
+};
+template <class T>
+PHX_DEV T bes_c(int which, int p, int j);
+template <>
+PHX_DEV float bes_c<float>(int which, int p, int j) { return which ? PHX_BES_K0K2_F[p][j] : PHX_BES_K1K2_F[p][j]; }
+template <>
+PHX_DEV double bes_c<double>(int which, int p, int j) { return which ? PHX_BES_K0K2[p][j] : PHX_BES_K1K2[p][j]; }
+template <class T>
+PHX_DEV T bes_ln(int p, int e);
+template <>
+PHX_DEV float bes_ln<float>(int p, int e) { return PHX_BES_LNEDGE_F[p][e]; }
+template <>
+PHX_DEV double bes_ln<double>(int p, int e) { return PHX_BES_LNEDGE[p][e]; }
+
+template <class T>
+PHX_DEV T hankel(int nu, T x) {  // src/core/numerics/bessel.hpp:99-108
+  const T mu = T(4 * nu * nu);
+  T term = T(1), sum = T(1);
+  for (int k = 1; k <= 30; k++) {
+    term = term * ((mu - T(2 * k - 1) * T(2 * k - 1)) / (T(k) * T(8) * x));
+    sum += term;
+    if (fabs(term) < Lim<T>::tol_asym) break;
+  }
+  return sum;
+}
+template <class T>
+PHX_DEV void bessel_scaled(T x, T* K0, T* K1, T* K2) {  // bessel.hpp:65-88
+  const T dt = fmin(T(0.2), T(0.7) / sqrt(x));
+  const T tmax = acosh(T(1) + T(42) / x) + T(2);
+  const int nmax = (int)(tmax / dt) + 1;
+  T s0 = T(0.5), s1 = T(0.5), s2 = T(0.5);
+  for (int k = 1; k <= nmax; k++) {
+    const T t = T(k) * dt;
+    const T sh = sinh(T(0.5) * t);
+    const T ch = T(1) + T(2) * sh * sh;
+    const T e = exp(-T(2) * x * sh * sh);
+    s0 += e;
+    s1 += e * ch;
+    s2 += e * (T(2) * ch * ch - T(1));
+    if (e * (T(2) * ch * ch) < Lim<T>::tol_series * T(0.1) * s2) break;
+  }
+  *K0 = dt * s0;
+  *K1 = dt * s1;
+  *K2 = dt * s2;
+}
+template <class T>
+PHX_DEV void bessel_ratios(T x, T* r10, T* r02) {  // K1/K2 and K0/K2 (bessel.hpp:118-148)
+  if (x >= T(30)) {
+    const T s2 = hankel(2, x);
+    *r10 = hankel(1, x) / s2;
+    *r02 = hankel(0, x) / s2;
+    return;
+  }
+  if (x < T(0.01)) {
+    T k0, k1, k2;
+    bessel_scaled(x, &k0, &k1, &k2);
+    *r10 = k1 / k2;
+    *r02 = k0 / k2;
+    return;
+  }
+  const int p = (x < T(0.1)) ? 0 : (x < T(1.0)) ? 1 : 2;
+  const T la = bes_ln<T>(p, 0), lb = bes_ln<T>(p, 1);
+  const T z = (T(2) * log(x) - (lb + la)) / (lb - la);
+  T b1a = T(0), b2a = T(0), b1b = T(0), b2b = T(0);
+  for (int j = PHX_BES_NTERM[p] - 1; j >= 1; j--) {  // Clenshaw, both fits at once
+    const T ta = T(2) * z * b1a - b2a + bes_c<T>(0, p, j);
+    b2a = b1a;
+    b1a = ta;
+    const T tb = T(2) * z * b1b - b2b + bes_c<T>(1, p, j);
+    b2b = b1b;
+    b1b = tb;
+  }
+  *r10 = bes_c<T>(0, p, 0) + z * b1a - b2a;
+  *r02 = bes_c<T>(1, p, 0) + z * b1b - b2b;
+}
+template <class T>
+PHX_DEV T fm_conv(T X) {  // f_m(X) (coeffs.hpp:63-70)
+  const T lx = log(X);
+  const T ex = exp(-X / T(47.2));
+  return T(2.011) * exp(-exp(T(1.035) * lx) / T(4.7)) - cos(X * T(0.5)) * exp(-exp(T(1.2) * lx) / T(2.73)) -
+         T(0.011) * ex +
+         (T(0.011) * ex - T(369.69342869090815) * exp(T(-8. / 3.) * lx)) * T(0.5) *
+             (T(1) + tanh(T(10) * (lx - T(4.787491742782046))));
+}
+// plasma-frame coefficients of one frequency (coeffs.hpp:95-138)
+template <class T>
+PHX_DEV Coef<T> thermal(T te, T ne, T nu, T B, T sth, T cth, T kr1, T kr0) {
+  Coef<T> r;
+  r.jU = r.aU = r.rU = T(0);
+  const T ite = T(1) / te;
+  const T nuc = T(NU_CRIT) * B * sth * te * te;
+  const T x = nu / nuc;
+  const T pref = T(JNU) * ne * nu * ite * ite;
+  const T cx = cbrt(x), ic = T(1) / cx, ex = exp(T(-1.8899) * cx);
+  r.jI = pref * (T(2.5651) * (T(1) + T(1.92) * ic + T(0.9977) * ic * ic) * ex);
+  r.jQ = pref * (T(2.5651) * (T(1) + T(0.93193) * ic + T(0.499873) * ic * ic) * ex);
+  r.jV = T(4. / 3.) * pref * ite * (cth / sth) *
+         ((T(1.81348) * ic * ic * ic + T(3.42319) * ic * ic + T(0.0292545) / sqrt(x) + T(2.03773) * ic) * ex);
+  const T pre = T(2. * HP) * nu * nu * nu / T(CL * CL);
+  const T xx = T(H_OVER_MEC2) * nu / te;
+  const T Bnu = Lim<T>::fp32 ? pre / expm1(xx) : pre / (exp(xx) - T(1));
+  const T iB = T(1) / Bnu;
+  r.aI = r.jI * iB;
+  r.aQ = r.jQ * iB;
+  r.aV = r.jV * iB;
+  const T iw = T(1) / (T(2 * PI_D) * nu);
+  const T wp2 = T(WP2) * ne;
+  const T om0 = T(GYRO) * B;
+  const T Xe = te * sqrt(T(1.4142135623730951) * sth * (T(1.e3 / (2. * PI_D)) * om0 / nu));
+  const T q = om0 * iw;
+  const T wpw = wp2 * iw;
+  r.rQ = (T(0.5) / T(CL)) * wpw * q * q * fm_conv(Xe) * (kr1 + T(6) * te) * sth * sth;
+  const T shgm = T(1) - T(0.11) * log(T(1) + T(0.035) * Xe);
+  r.rV = (T(1) / T(CL)) * wpw * q * (kr0 * shgm) * cth;
+  return r;
+}
+// K1/K2 and K0/K2 at 1/Theta_e (frequency independent; coeffs.hpp:122-131)
+template <class T>
+PHX_DEV void kratios(T te, T* kr1, T* kr0) {
+  const T ite = T(1) / te;
+  if (ite < T(1. / 5.)) {
+    const T k0 = -log(ite / T(2)) - T(0.5772), k1 = T(1) / ite, k2 = T(2) / ite / ite;
+    *kr1 = k1 / k2;
+    *kr0 = (k2 > T(0)) ? k0 / k2 : T(1);
+  } else {
+    bessel_ratios(ite, kr1, kr0);
+  }
+}
+// Lorentz-invariant form + polarization-fraction clamp + rotation into the camera basis
+// (coeffs.hpp:147-182)
+template <class T>
+PHX_DEV Coef<T> invariant_rotated(Coef<T> c, T nup, T jscale, T c2, T s2) {
+  const T js = jscale / nup / nup;
+  c.jI = c.jI * js;
+  c.jQ = c.jQ * js;
+  c.jV = c.jV * js;
+  c.aI = c.aI * nup;
+  c.aQ = c.aQ * nup;
+  c.aV = c.aV * nup;
+  c.rQ = c.rQ * nup;
+  c.rV = c.rV * nup;
+  const T pf = sqrt(c.jQ * c.jQ + c.jV * c.jV) / c.jI;
+  if (pf > T(1)) {
+    c.jQ = c.jQ / (pf + T(0.005));
+    c.jU = c.jU / (pf + T(0.005));
+    c.jV = c.jV / (pf + T(0.005));
+  }
+  Coef<T> r = c;
+  r.jQ = c2 * c.jQ + s2 * c.jU;
+  r.jU = -s2 * c.jQ + c2 * c.jU;
+  r.aQ = c2 * c.aQ + s2 * c.aU;
+  r.aU = -s2 * c.aQ + c2 * c.aU;
+  r.rQ = c2 * c.rQ + s2 * c.rU;
+  r.rU = -s2 * c.rQ + c2 * c.rU;
+  return r;
+}
+
+// ------------------------------------------------------------------ operator
+// O_A = (I + h/2 K)^-1 (I - h/2 K), O_S = h (I + h/2 K)^-1 j (src/core/transfer/operators.hpp:158-176),
+// Gauss-Jordan with the reference's successive-swap partial pivoting (operators.hpp:91-136)
+template <class T>
+PHX_DEV void segment_op(const Coef<T>& c, T h, T OA[16], T OS[4]) {
+  const T K[4][4] = {{c.aI, c.aQ, c.aU, c.aV}, {c.aQ, c.aI, c.rV, -c.rU}, {c.aU, -c.rV, c.aI, c.rQ},
+                     {c.aV, c.rU, -c.rQ, c.aI}};
+  const T hh = T(0.5) * h;
+  T A[4][8];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      A[i][j] = (i == j ? T(1) : T(0)) + hh * K[i][j];
+      A[i][4 + j] = (i == j) ? T(1) : T(0);
+    }
+#pragma unroll
+  for (int c0 = 0; c0 < 4; c0++) {
+#pragma unroll
+    for (int r = c0 + 1; r < 4; r++) {
+      const bool sw = fabs(A[r][c0]) > fabs(A[c0][c0]);
+#pragma unroll
+      for (int j = c0; j < 8; j++) {
+        const T t = A[c0][j];
+        A[c0][j] = sw ? A[r][j] : t;
+        A[r][j] = sw ? t : A[r][j];
+      }
+    }
+    const T d = A[c0][c0];
+#pragma unroll
+    for (int j = c0; j < 8; j++) A[c0][j] = A[c0][j] / d;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      if (r == c0) continue;
+      const T f = A[r][c0];
+#pragma unroll
+      for (int j = c0; j < 8; j++) A[r][j] = A[r][j] - f * A[c0][j];
+    }
+  }
+  // O_A = P^-1 N with N = I - h/2 K
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      T s = T(0);
+#pragma unroll
+      for (int k = 0; k < 4; k++) s += A[i][4 + k] * ((k == j ? T(1) : T(0)) - hh * K[k][j]);
+      OA[4 * i + j] = s;
+    }
+  const T jv[4] = {c.jI, c.jQ, c.jU, c.jV};
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    T s = T(0);
+#pragma unroll
+    for (int k = 0; k < 4; k++) s += A[i][4 + k] * jv[k];
+    OS[i] = h * s;
+  }
+}
+
+// ------------------------------------------------------------------ camera
+// RAPTOR camera photon in double (src/core/geometry/camera.hpp:37-135)
+struct Cam {
+  double a, hs, incl, r_cam, t_init, phi_cam, fov_x, fov_y, cos_pa, sin_pa;
+  int W, H;
+};
+PHX_DEV double sgnd(double x) { return (x < 0.) ? -1. : (x > 0.) ? 1. : 0.; }
+PHX_DEV void camera_photon(const Cam& C, double alpha, double beta, double ph[8]) {
+  const double a = C.a, rcam = C.r_cam;
+  const double mu0 = cos(C.incl / 180. * PI_D);
+  const double th0 = acos(mu0);
+  const double phi0 = C.phi_cam / 180. * PI_D;
+  const double ll = -alpha * sqrt(1. - mu0 * mu0);
+  const double qq = beta * beta + mu0 * mu0 * (alpha * alpha - 1.);
+  const double sint = sin(th0), sinc2 = sint * sint, cost = cos(th0), cosc2 = cost * cost;
+  const double kd0 = -1., kd3 = ll;
+  const double kd2 = sgnd(beta) * sqrt(fabs(qq - ll * ll * (cosc2 / sinc2) + cosc2));
+  const double r = rcam;
+  const double sig = r * r + a * a * cost * cost;
+  const double del = r * r + a * a - 2. * r;
+  const double A_ = (r * r + a * a) * (r * r + a * a) - del * a * a * sint * sint;
+  const double g11 = sig / del * r * r;
+  const double g00 = -A_ / (sig * del), g03 = -2. * a * r / (sig * del);
+  const double g33 = (del - a * a * sint * sint) / (sig * del * sint * sint), g22 = 1. / sig;
+  const double ku0 = g00 * kd0 + g03 * kd3;
+  const double ku3 = g33 * kd3 + g03 * kd0;
+  const double ku2 = g22 * kd2;
+  const double ku1 = sqrt((-ku0 * kd0 - ku2 * kd2 - ku3 * kd3) / g11);
+  const double lr = log(rcam), rp = exp(lr);
+  const double Dl = rp * rp - 2. * rp + a * a;
+  const double sq = sqrt(1. - a * a);
+  const double lg = log((rp - (1. + sq)) / (rp - (1. - sq)));
+  ph[0] = C.t_init + (log(Dl) + lg / sq);
+  ph[1] = lr;
+  ph[3] = phi0 + a / (2. * sq) * lg;
+  ph[4] = ku0 + 2. * rp / Dl * rp * ku1;
+  ph[5] = ku1;
+  ph[7] = ku3 + a / Dl * rp * ku1;
+  // theta -> x2 and k^theta -> k^x2 by Newton (camera.hpp:37-61)
+  double cur = th0, prev = 1.e-15;
+  for (int it = 0; it < 100 && fabs(cur - prev) > 1.e-15; it++) {
+    prev = cur;
+    cur = cur - (cur + 0.5 * C.hs * sin(2. * cur) - th0) / (1. + C.hs * cos(2. * cur));
+  }
+  ph[2] = cur;
+  const double dd = 1. + C.hs * cos(2. * cur);
+  double ck = ku2;
+  prev = 1.e-15;
+  for (int it = 0; it < 100 && fabs(ck - prev) > 1.e-15; it++) {
+    prev = ck;
+    ck = ck - (ck * dd - ku2) / dd;
+  }
+  ph[6] = ck;
+}
+// camera polarization basis F1 = e_(1) of the camera observer with up = (0,0,0,-1) (tetrad.hpp:27-133)
+template <class T>
+PHX_DEV void camera_F1(T a, T hs, const double ph[8], T F1[4]) {
+  T k[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) k[i] = T(ph[4 + i]);
+  const Met<T> g = metric(a, hs, T(ph[1]), T(ph[2]));
+  const MetU<T> u = metric_up(a, g);
+  const T B = T(0);  // -g^{03} U_0 / g^{33} with g^{03} = 0 in Kerr-Schild
+  const T C = -(T(1) + u.u00) / u.u33;
+  const T U3d = B + sqrt(B * B + C);
+  const T U[4] = {-u.u00, -u.u01 + u.u13 * U3d, T(0), u.u33 * U3d};  // raise (-1, 0, 0, U3d)
+  const T up[4] = {T(0), T(0), T(0), T(-1)};
+  T para[4];
+  frame(g, a, k, U, up, F1, para);
+}
+
+}  // namespace phxd

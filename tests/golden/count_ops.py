@@ -1,0 +1,78 @@
+"""Counts the algorithmic work of the march with the oracle's operation-counting
+scalar type and writes tests/golden/op_counts.json.
+
+The counted type runs the reference's arithmetic (FP32 or FP64 control flow) and
+tallies, per phase, additions/subtractions, multiplications, divisions, square
+roots, transcendental calls (exp, log, sin, cos, cbrt, expm1, tanh, acos, acosh,
+sinh, fmod) and comparisons.  Phases: ray setup, geodesic step (view, AMR lookups,
+step rule, RK4), plasma sample (frequency independent part of process_sample),
+and sample x frequency (coefficients, operator, accumulation).
+
+The per-unit constants that the benchmark's roofline uses:
+  F_step   = (setup + step + sample ops) / geodesic steps
+  F_segf   = (sample x frequency ops) / (plasma segments x frequencies)
+flop = add + mul + div + sqrt + transcendental (each counted once); comparisons
+are not flops.  Algorithmic work of a render with N steps, M plasma segments,
+B dropped segments and nf frequencies: W = N F_step + (M nf - B) F_segf.
+
+    python tests/golden/count_ops.py [--pixels 600]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle as O  # noqa: E402
+from phoenix_amd import snapshot as S  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "op_counts.json")
+OPS = ["add", "mul", "div", "sqrt", "trans", "cmp"]
+PHASES = ["setup", "step", "sample", "freq"]
+
+
+def count(preset: str, precision: int, npix: int, W: int = 1024, seed: int = 7, **params):
+    dump, grid, info = S.ensure_snapshot(preset, S.default_dir())
+    o = O.Oracle(dump, grid, precision, img_width=W, img_height=W, **params)
+    rng = np.random.default_rng(seed)
+    pix = np.sort(rng.choice(W * W, size=npix, replace=False))
+    O.op_counts_reset()
+    t0 = time.time()
+    res, st = o.render(pix, threads=1)
+    dt = time.time() - t0
+    c = O.op_counts()
+    o.close()
+    flops = c[:, :5].sum(1)  # per phase
+    nf = len(params.get("freqs", (230e9,)))
+    steps, segs, bad = (int(v) for v in st)
+    return {
+        "preset": preset, "precision": precision, "pixels": npix, "image": W, "nfreqs": nf, "seconds": dt,
+        "steps": steps, "nplasma": segs, "bad": bad,
+        "ops": {ph: dict(zip(OPS, (int(v) for v in c[i]))) for i, ph in enumerate(PHASES)},
+        "F_step": float((flops[0] + flops[1] + flops[2]) / steps),
+        "F_segf": float(flops[3] / (segs * nf)),
+        "div_per_step": float((c[0, 2] + c[1, 2] + c[2, 2]) / steps),
+        "trans_per_step": float((c[0, 4] + c[1, 4] + c[2, 4]) / steps),
+        "div_per_segf": float(c[3, 2] / (segs * nf)),
+        "trans_per_segf": float(c[3, 4] / (segs * nf)),
+    }
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--pixels", type=int, default=600)
+    a = ap.parse_args()
+    out = {"c2_1024_230GHz": {"fp32": count("c2", 132, a.pixels), "fp64": count("c2", 164, a.pixels)}}
+    with open(OUT, "w") as f:
+        json.dump(out, f, indent=1)
+    for k, v in out["c2_1024_230GHz"].items():
+        print(k, {x: v[x] for x in ("F_step", "F_segf", "div_per_step", "trans_per_step", "div_per_segf",
+                                     "trans_per_segf", "steps", "nplasma", "bad", "seconds")})

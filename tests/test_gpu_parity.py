@@ -1,0 +1,116 @@
+"""HIP path (libphx.so, through the C ABI) against the reference's outputs and the oracle.
+
+* device decode of the dump against the oracle's host decode;
+* every golden case at FP64 and FP32 against the reference's FP64 build (the accuracy
+  judge), gated per pixel and on total flux (tests/parity.py), with the run statistics
+  (geodesic steps, plasma segments, dropped segments) against the reference's;
+* a case outside the fixtures against the oracle rendered live on the host;
+* resident-snapshot reuse, partial job lists and error returns.
+"""
+import numpy as np
+import pytest
+
+from phoenix_amd import capi, configs, snapshot as S
+from tests import parity
+from tests.golden import make_golden as G
+
+pytestmark = pytest.mark.gpu
+
+CASES = sorted(G.CASES)
+
+
+def _run_for(case, snapdir, tmp_path):
+    preset, over = G.CASES[case]
+    dump, grid, info = S.ensure_snapshot(preset, snapdir)
+    cfg = configs.write_config(str(tmp_path / f"{case}.toml"), dump, grid, **over)
+    return capi.Run(cfg), info
+
+
+@pytest.fixture(scope="module")
+def c1_run(snapdir, tmp_path_factory):
+    dump, grid, _ = S.ensure_snapshot("c1", snapdir)
+    cfg = configs.write_config(str(tmp_path_factory.mktemp("c1") / "c1.toml"), dump, grid)
+    return capi.Run(cfg)
+
+
+@pytest.mark.parametrize("table", [capi.TABLE_FP64, capi.TABLE_FP32])
+def test_device_decode_matches_oracle(c1_run, snapdir, table, oracle_lib):
+    O = oracle_lib
+    dump, grid, _ = S.ensure_snapshot("c1", snapdir)
+    snap = capi.Snapshot(c1_run, table)
+    dev = snap.prims(table)
+    snap.close()
+    o = O.Oracle(dump, grid, 64 if table == capi.TABLE_FP64 else 32)
+    ref = o.tables()[0]
+    o.close()
+    if table == capi.TABLE_FP64:
+        np.testing.assert_allclose(dev, ref, rtol=1e-12, atol=1e-300)
+    else:
+        # float32 of the same double: identical up to a final rounding of a 1e-15-different value
+        np.testing.assert_allclose(dev, ref, rtol=2 ** -23, atol=1e-40)
+
+
+@pytest.mark.parametrize("precision", [capi.FP64, capi.FP32])
+@pytest.mark.parametrize("case", CASES)
+def test_golden_case(case, precision, snapdir, tmp_path):
+    run, info = _run_for(case, snapdir, tmp_path)
+    ref64, st64, meta = G.load(case, 64)
+    ref_same, st_same, _ = G.load(case, 32 if precision == capi.FP32 else 64)
+    res, inf = run.dev_render(precision)
+    figs = parity.compare(res, ref64)
+    print(f"\n{case} fp{precision}: {parity.summary(figs)}; steps {inf['steps']} ref {st_same[0]}; "
+          f"segments {inf['nplasma']} ref {st_same[1]}; bad {inf['bad']} ref {st_same[2]}; "
+          f"kernel {inf['kernel_s'] * 1e3:.2f} ms, vgprs {inf['vgprs']} scratch {inf['scratch_bytes']}")
+    assert parity.passes(figs), parity.summary(figs)
+    if info["sha256"] == meta["sha256"]:
+        for k, name in enumerate(("steps", "nplasma", "bad")):
+            assert abs(inf[name] - int(st_same[k])) <= max(2, 2e-3 * int(st_same[k])), (name, inf[name], st_same)
+
+
+def test_c2_inclination60_vs_oracle(snapdir, tmp_path, oracle_lib):
+    O = oracle_lib
+    dump, grid, _ = S.ensure_snapshot("c2", snapdir)
+    cfg = configs.write_config(str(tmp_path / "c2.toml"), dump, grid, camera__inclination="60",
+                               camera__img_width="40", camera__img_height="40",
+                               camera__frequency_list="[86e9, 230e9, 345e9, 690e9, 150e9]")
+    run = capi.Run(cfg)
+    o = O.Oracle(dump, grid, 64, inclination=60.0, img_width=40, img_height=40,
+                 freqs=(86e9, 230e9, 345e9, 690e9, 150e9))
+    ref, st = o.render()
+    o.close()
+    for precision in (capi.FP64, capi.FP32):
+        res, inf = run.dev_render(precision)
+        figs = parity.compare(res, ref)
+        print(f"\nc2 i60 5freq fp{precision}: {parity.summary(figs)}")
+        assert parity.passes(figs), parity.summary(figs)
+
+
+def test_resident_snapshot_reuse_and_partial_jobs(c1_run):
+    snap = capi.Snapshot(c1_run, capi.TABLE_FP32 | capi.TABLE_FP64)
+    rq = c1_run.request(img_width=32, img_height=32, inclination=30.0)
+    full, i1 = snap.render(rq, capi.FP32)
+    again, i2 = snap.render(rq, capi.FP32)
+    assert np.array_equal(full, again), "a resident render is not deterministic"
+    assert i1["steps"] == i2["steps"] and i1["nplasma"] == i2["nplasma"]
+    jobs = np.arange(0, 32 * 32, 7, dtype=np.int32)
+    part, _ = snap.render(rq, capi.FP32, jobs=jobs)
+    assert np.array_equal(part[jobs], full[jobs])
+    rest = np.setdiff1d(np.arange(32 * 32), jobs)
+    assert np.all(part[rest] == 0)
+    # a different request on the same snapshot: FP64 table, two frequencies
+    rq2 = c1_run.request(img_width=16, img_height=24, freqs=(230e9, 345e9), m_unit=2e18)
+    r64, _ = snap.render(rq2, capi.FP64)
+    assert r64.shape == (16 * 24, 2, 6) and np.isfinite(r64).all()
+    snap.close()
+
+
+def test_errors_are_reported(c1_run):
+    snap = capi.Snapshot(c1_run, capi.TABLE_FP32)
+    with pytest.raises(capi.PhxError) as e:
+        snap.render(None, capi.FP64)  # no FP64 table
+    assert e.value.code == -1
+    with pytest.raises(capi.PhxError):
+        snap.render(None, 48)  # not a precision
+    with pytest.raises(capi.PhxError):
+        snap.render(None, capi.FP32, jobs=np.array([-1], dtype=np.int32))
+    snap.close()
