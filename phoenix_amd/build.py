@@ -3,7 +3,10 @@
 hipcc compiles the device TU for gfx950 only; g++ compiles the host TU; hipcc
 links both into one shared library next to this file.  Packed FP32 VALU ops are
 disabled (``-Xclang -target-feature -Xclang -packed-fp32-ops``): some of their
-forms have produced wrong results on the MI355X boxes this runs on.
+forms have produced wrong results on the MI355X boxes this runs on.  FP32
+division and square root use the hardware reciprocal / square-root sequences
+(``-fno-hip-fp32-correctly-rounded-divide-sqrt``, ~1 ulp instead of correctly
+rounded); FP64 is unaffected.
 """
 from __future__ import annotations
 
@@ -22,6 +25,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-Wall", "-Wno-unused-function",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics",
+             "-fno-hip-fp32-correctly-rounded-divide-sqrt",
              f"-I{ROOT}/include"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{ROOT}/include"]
 
@@ -34,18 +38,18 @@ def _run(cmd):
     return p
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, out: str = LIB, extra_hip_flags=(), device_src: str | None = None) -> str:
     os.makedirs(BUILD, exist_ok=True)
     dev_o = os.path.join(BUILD, "phx_device.o")
     host_o = os.path.join(BUILD, "phx_host.o")
-    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "phx_device.hip"), "-o", dev_o])
+    _run([HIPCC, *HIP_FLAGS, *extra_hip_flags, "-c", device_src or os.path.join(CSRC, "phx_device.hip"), "-o", dev_o])
     _run([os.environ.get("CXX", "g++"), *CXX_FLAGS, "-c", os.path.join(CSRC, "phx_host.cpp"), "-o", host_o])
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, dev_o, host_o])
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", out)
+    return out
 
 
 def clean() -> None:

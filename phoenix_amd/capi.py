@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libphx.so")
+LIB_PATH = os.environ.get("PHX_LIB", os.path.join(HERE, "libphx.so"))
 
 MAX_FREQS = 32
 MAX_LEVELS = 16

@@ -54,18 +54,18 @@ struct MarchArgs {
 // longest ray ends.  One pass of the loop is one pass of render_pixel's loop
 // (src/core/render/pixel.hpp:179-211): view, process the pending sample with the
 // current position's leaf as its cache, turning points, step size, RK4.
+// FP32: two waves per SIMD (a 256-VGPR budget); FP64: one wave, its state does not fit in 256.
 template <class T, int NF>
-__global__ __launch_bounds__(BLOCK) void k_march(const MarchArgs<T, NF> A) {
+__global__ __launch_bounds__(BLOCK, Lim<T>::fp32 ? 2 : 1) void k_march(const MarchArgs<T, NF> A) {
   constexpr bool F32 = Lim<T>::fp32;
   const int lane = threadIdx.x & 63;
   Ray<T> g;
-  T px[4], pk[4], pf[4], pdl = T(0);
-  int pi = -1;
+  T px[4] = {T(0), T(0), T(0), T(0)}, pk[4], pf[4], pdl = T(0);
   T AA[NF][16], SS[NF][4], tau[NF], tauF[NF];
   int steps = 0, turns = 0, stop = 0, np = 0, bad = 0, idx = 0;
   T thdot = T(0), dlp = T(0);
   bool bneg = false, bpos = false, have = false;
-  unsigned long long s_steps = 0, s_np = 0, s_bad = 0;
+  unsigned s_steps = 0, s_np = 0, s_bad = 0;  // per-lane sums (a lane marches well under 2^32 steps)
 
   for (;;) {
     if (!have) {
@@ -77,14 +77,15 @@ __global__ __launch_bounds__(BLOCK) void k_march(const MarchArgs<T, NF> A) {
       const int n = A.j0 + base + (int)__popcll(m & ((1ull << lane) - 1ull));
       if (n >= A.j1) break;
       idx = A.jobs[n];
+      const Cam& C = A.C;
       // pixel -> impact parameters with the position-angle rotation (camera.hpp:11-23, jobs.hpp:64-76)
-      const int i = idx / A.C.H, j = idx % A.C.H;
-      const double a0 = -0.5 * A.C.fov_x + (i + 0.5) * A.C.fov_x / A.C.W;
-      const double b0 = -0.5 * A.C.fov_y + (j + 0.5) * A.C.fov_y / A.C.H;
-      const double alpha = a0 * A.C.cos_pa + b0 * A.C.sin_pa;
-      const double beta = -a0 * A.C.sin_pa + b0 * A.C.cos_pa;
+      const int i = idx / C.H, j = idx % C.H;
+      const double a0 = -0.5 * C.fov_x + (i + 0.5) * C.fov_x / C.W;
+      const double b0 = -0.5 * C.fov_y + (j + 0.5) * C.fov_y / C.H;
+      const double alpha = a0 * C.cos_pa + b0 * C.sin_pa;
+      const double beta = -a0 * C.sin_pa + b0 * C.cos_pa;
       double ph[8];
-      camera_photon(A.C, alpha, beta, ph);
+      camera_photon(C, alpha, beta, ph);
       T F1[4];
       camera_F1<T>(A.P.a, A.P.hs, ph, F1);
 #pragma unroll
@@ -107,7 +108,6 @@ __global__ __launch_bounds__(BLOCK) void k_march(const MarchArgs<T, NF> A) {
       }
       steps = turns = stop = np = bad = 0;
       thdot = dlp = T(0);
-      pi = -1;
       bneg = beta < 0.;
       bpos = beta > 0.;
       have = true;
@@ -123,6 +123,7 @@ __global__ __launch_bounds__(BLOCK) void k_march(const MarchArgs<T, NF> A) {
     }
     const T r = exp(cx[1]);
     const bool alive = r < A.r_out && r > A.cut_in && steps < A.max_steps && !stop;
+    const T rs = exp(px[1]);  // radius of the pending sample
 
     // ---- one AMR lookup serves the pending sample's cache (leaf_of) and the step's cell cap (grid_dxc)
     int leaf_next = -1;
@@ -132,7 +133,8 @@ __global__ __launch_bounds__(BLOCK) void k_march(const MarchArgs<T, NF> A) {
       T wx[4] = {cx[0], cx[1], cx[2], cx[3]};
       wrap(wx);
       const bool need_dxc = r < A.P.rt_outer;
-      const bool need_leaf = pi >= 1 && !off_domain(A.G, wx);
+      // the cache leaf only matters to a pending sample that can emit (r < rt_outer)
+      const bool need_leaf = steps >= 2 && rs < A.P.rt_outer && !off_domain(A.G, wx);
       if (need_dxc || need_leaf) {
         const int L = find_leaf(A.G, wx);
         if (need_leaf) leaf_next = L;
@@ -144,10 +146,12 @@ __global__ __launch_bounds__(BLOCK) void k_march(const MarchArgs<T, NF> A) {
       }
     }
 
-    // ---- the pending sample (src/core/render/pixel.hpp:52-123)
-    if (pi >= 1) {
+    // ---- the pending sample (src/core/render/pixel.hpp:52-123).  A sample at r >= rt_outer
+    // cannot emit (pixel.hpp:80) and changes no ray state, so nothing of it is computed.
+    // the pending sample is the state before the last step, i.e. sample number steps - 1; the
+    // camera sample (number 0) is never processed (pixel.hpp:186)
+    if (steps >= 2 && rs < A.P.rt_outer) {
       T k[4] = {pk[0], pk[1], pk[2], pk[3]};
-      const T rs = exp(px[1]);
       const Met<T> gs = metric(A.P.a, A.P.hs, px[1], px[2]);
       if (fabs(dot(gs, k, k)) > T(1e-6) && rs > T(2)) {  // re-solve k^t for k.k = 0 (geodesic.hpp:297-306)
         const T aa = gs.d00;
@@ -158,7 +162,7 @@ __global__ __launch_bounds__(BLOCK) void k_march(const MarchArgs<T, NF> A) {
       }
       Fluid<T> fl;
       const int in_vol = sample_fluid(A.G, A.P, px, leaf_next, fl);
-      if (in_vol && rs < A.P.rt_outer) {
+      if (in_vol) {
         const T pa = pitch_angle(fl.g, k, fl.b, fl.U);
         if (!(pa == T(0) || pa == T(PI_D))) {
           T perp[4], para[4];
@@ -195,24 +199,23 @@ __global__ __launch_bounds__(BLOCK) void k_march(const MarchArgs<T, NF> A) {
                 bad++;
                 continue;
               }
-              // S += A O_S; A <- A O_A (operators.hpp:74-86)
-              T nA[16];
+              // S += A O_S; A <- A O_A (operators.hpp:74-86), one row at a time in place
 #pragma unroll
               for (int i = 0; i < 4; i++) {
+                T row[4];
                 T s = T(0);
 #pragma unroll
-                for (int j = 0; j < 4; j++) s += AA[f][4 * i + j] * OS[j];
-                SS[f][i] += s;
-#pragma unroll
                 for (int j = 0; j < 4; j++) {
+                  s += AA[f][4 * i + j] * OS[j];
                   T t = T(0);
 #pragma unroll
                   for (int kq = 0; kq < 4; kq++) t += AA[f][4 * i + kq] * OA[4 * kq + j];
-                  nA[4 * i + j] = t;
+                  row[j] = t;
                 }
-              }
+                SS[f][i] += s;
 #pragma unroll
-              for (int q = 0; q < 16; q++) AA[f][q] = nA[q];
+                for (int j = 0; j < 4; j++) AA[f][4 * i + j] = row[j];
+              }
               tau[f] += seg.aI * h;
               tauF[f] += fabs(seg.rV * h);
             }
@@ -227,7 +230,6 @@ __global__ __launch_bounds__(BLOCK) void k_march(const MarchArgs<T, NF> A) {
       pf[q] = cf[q];
     }
     pdl = dlp;
-    pi = steps;
 
     if (!alive) {
       // ---- ray done: write it out and free the lane (pixel.hpp:213-223)
@@ -262,19 +264,21 @@ __global__ __launch_bounds__(BLOCK) void k_march(const MarchArgs<T, NF> A) {
     steps++;
   }
 
-  if (A.count_stats) {
-    // wave reduction, then one atomic per wave
+  // wave reduction, then one atomic per wave; steps and segments are counted by the first
+  // frequency chunk only, dropped segments (per frequency) by every chunk
+  unsigned long long w_steps = s_steps, w_np = s_np, w_bad = s_bad;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      s_steps += __shfl_xor(s_steps, off);
-      s_np += __shfl_xor(s_np, off);
-      s_bad += __shfl_xor(s_bad, off);
+  for (int off = 32; off > 0; off >>= 1) {
+    w_steps += __shfl_xor(w_steps, off);
+    w_np += __shfl_xor(w_np, off);
+    w_bad += __shfl_xor(w_bad, off);
+  }
+  if (lane == 0) {
+    if (A.count_stats) {
+      atomicAdd(&A.tot[0], w_steps);
+      atomicAdd(&A.tot[1], w_np);
     }
-    if (lane == 0) {
-      atomicAdd(&A.tot[0], s_steps);
-      atomicAdd(&A.tot[1], s_np);
-      atomicAdd(&A.tot[2], s_bad);
-    }
+    atomicAdd(&A.tot[2], w_bad);
   }
 }
 
@@ -376,6 +380,8 @@ struct phx_snapshot {
   unsigned long long* d_tot = nullptr;
   double decode_s = 0., upload_s = 0.;
   int64_t last_nres = 0;
+  int last_chunk_jobs = 0;
+  int morton_w = 0, morton_h = 0;  // image size whose Morton job list d_jobs holds (0: none)
   int device = 0;
   std::vector<void*> allocs;
 };
@@ -624,18 +630,20 @@ static void launch_march(phx_snapshot* s, const Tables<T>& tb, const phx_model* 
   A.P.te_min = T(rq->thetae_min);
   A.P.te_max = T(rq->thetae_max);
   A.P.rt_outer = T(rq->rt_outer);
-  A.C.a = (double)T(m.a);  // the camera uses the table-precision spin and stretching (camera.hpp:77)
-  A.C.hs = (double)T(m.hslope);
-  A.C.incl = rq->inclination;
-  A.C.r_cam = rq->r_cam;
-  A.C.t_init = rq->t_init;
-  A.C.phi_cam = rq->phi_cam;
-  A.C.fov_x = rq->fov_x;
-  A.C.fov_y = rq->fov_y;
-  A.C.cos_pa = rq->cos_pa;
-  A.C.sin_pa = rq->sin_pa;
-  A.C.W = rq->img_width;
-  A.C.H = rq->img_height;
+  Cam& C = A.C;
+  C.a = (double)T(m.a);  // the camera uses the table-precision spin and stretching (camera.hpp:77)
+  C.hs = (double)T(m.hslope);
+  C.incl = rq->inclination;
+  C.r_cam = rq->r_cam;
+  C.t_init = rq->t_init;
+  C.phi_cam = rq->phi_cam;
+  C.fov_x = rq->fov_x;
+  C.fov_y = rq->fov_y;
+  C.cos_pa = rq->cos_pa;
+  C.sin_pa = rq->sin_pa;
+  C.W = rq->img_width;
+  C.H = rq->img_height;
+
   A.stepsize = T(rq->stepsize);
   A.r_out = T(rq->r_out);
   A.cut_in = T(rq->cut_in);
@@ -663,12 +671,18 @@ static void launch_march(phx_snapshot* s, const Tables<T>& tb, const phx_model* 
   int grid = per_cu * cus;
   const int njobs = j1 - j0;
   grid = std::max(1, std::min(grid, (njobs + BLOCK - 1) / BLOCK));
+  hipFuncAttributes fa;
+  HIPCHECK(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_march<T, NF>)));
+  // a kernel whose private segment exceeds the per-lane stack limit does not run at all; raise the
+  // limit to what the kernel needs (the reference sets it explicitly too, kernel.cu:305)
+  size_t limit = 0;
+  HIPCHECK(hipDeviceGetLimit(&limit, hipLimitStackSize));
+  if (fa.localSizeBytes > limit) HIPCHECK(hipDeviceSetLimit(hipLimitStackSize, (fa.localSizeBytes + 255) & ~size_t(255)));
   HIPCHECK(hipMemsetAsync(s->d_counter, 0, sizeof(int), s->stream));
   k_march<T, NF><<<grid, BLOCK, 0, s->stream>>>(A);
   HIPCHECK(hipGetLastError());
+  s->last_chunk_jobs = njobs;
   if (info) {
-    hipFuncAttributes fa;
-    HIPCHECK(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_march<T, NF>)));
     info->vgprs = fa.numRegs;
     info->scratch_bytes = (int32_t)fa.localSizeBytes;
     info->lds_bytes = (int32_t)fa.sharedSizeBytes;
@@ -686,8 +700,9 @@ static void march_all(phx_snapshot* s, const Tables<T>& tb, const phx_model* rq,
   if (const char* e = std::getenv("PHX_CHUNK"))
     if (std::atoi(e) > 0) chunk = std::atoi(e);
   const int nfr = rq->nfreqs;
-  for (int f0 = 0; f0 < nfr; f0 += 8) {
-    const int nf = std::min(8, nfr - f0);
+  const int per = Lim<T>::fp32 ? 8 : 4;  // frequencies per march: keeps the FP64 state out of scratch
+  for (int f0 = 0; f0 < nfr; f0 += per) {
+    const int nf = std::min(per, nfr - f0);
     for (int j0 = 0; j0 < njobs; j0 += chunk) {
       const int j1 = std::min(njobs, j0 + chunk);
       const int stats = f0 == 0;
@@ -697,7 +712,7 @@ static void march_all(phx_snapshot* s, const Tables<T>& tb, const phx_model* rq,
         launch_march<T, 2>(s, tb, rq, f0, nf, d_jobs, j0, j1, stats, info);
       else if (nf <= 4)
         launch_march<T, 4>(s, tb, rq, f0, nf, d_jobs, j0, j1, stats, info);
-      else
+      else if constexpr (Lim<T>::fp32)
         launch_march<T, 8>(s, tb, rq, f0, nf, d_jobs, j0, j1, stats, info);
       if (progress) {
         HIPCHECK(hipStreamSynchronize(s->stream));
@@ -724,15 +739,20 @@ static void render(phx_snapshot* s, const phx_model* rq, int precision, const in
   HIPCHECK(hipSetDevice(s->device));
   const int64_t npix = (int64_t)rq->img_width * rq->img_height;
   std::vector<int32_t> order;
+  // the full-image Morton list is built once per image size and stays on the device
+  const bool reuse = !jobs && s->morton_w == rq->img_width && s->morton_h == rq->img_height;
   if (!jobs) {
-    order.resize(npix);
-    if (phx_schedule(rq, 1, order.data(), (int32_t)npix) != PHX_OK) throw Error(PHX_E_ARG, phx_last_error());
-    jobs = order.data();
     njobs = (int32_t)npix;
+    if (!reuse) {
+      order.resize(npix);
+      if (phx_schedule(rq, 1, order.data(), (int32_t)npix) != PHX_OK) throw Error(PHX_E_ARG, phx_last_error());
+      jobs = order.data();
+    }
+  } else {
+    if (njobs < 0 || njobs > npix) throw Error(PHX_E_ARG, "njobs out of range");
+    for (int32_t n = 0; n < njobs; n++)
+      if (jobs[n] < 0 || jobs[n] >= npix) throw Error(PHX_E_ARG, "job index outside the image");
   }
-  if (njobs < 0 || njobs > npix) throw Error(PHX_E_ARG, "njobs out of range");
-  for (int32_t n = 0; n < njobs; n++)
-    if (jobs[n] < 0 || jobs[n] >= npix) throw Error(PHX_E_ARG, "job index outside the image");
   const size_t nres = (size_t)npix * rq->nfreqs;
   if (nres > s->res_cap) {
     if (s->d_res) HIPCHECK(hipFree(s->d_res));
@@ -743,6 +763,7 @@ static void render(phx_snapshot* s, const phx_model* rq, int precision, const in
   if ((size_t)njobs > s->jobs_cap) {
     if (s->d_jobs) HIPCHECK(hipFree(s->d_jobs));
     s->d_jobs = nullptr;
+    s->morton_w = s->morton_h = 0;
     HIPCHECK(hipMalloc((void**)&s->d_jobs, std::max<size_t>(njobs, 1) * sizeof(int32_t)));
     s->jobs_cap = std::max<size_t>(njobs, 1);
   }
@@ -752,7 +773,14 @@ static void render(phx_snapshot* s, const phx_model* rq, int precision, const in
   hipDeviceProp_t prop;
   HIPCHECK(hipGetDeviceProperties(&prop, s->device));
   std::snprintf(info->device, sizeof info->device, "%s (%s)", prop.name, prop.gcnArchName);
-  HIPCHECK(hipMemcpyAsync(s->d_jobs, jobs, (size_t)njobs * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+  if (jobs) {
+    HIPCHECK(hipMemcpyAsync(s->d_jobs, jobs, (size_t)njobs * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+    s->morton_w = s->morton_h = 0;
+    if (!order.empty()) {
+      s->morton_w = rq->img_width;
+      s->morton_h = rq->img_height;
+    }
+  }
   if ((int64_t)njobs < npix) HIPCHECK(hipMemsetAsync(s->d_res, 0, nres * sizeof(phx_freqout), s->stream));
   HIPCHECK(hipMemsetAsync(s->d_tot, 0, 4 * sizeof(unsigned long long), s->stream));
   HIPCHECK(hipEventRecord(s->e0, s->stream));
@@ -766,6 +794,11 @@ static void render(phx_snapshot* s, const phx_model* rq, int precision, const in
   HIPCHECK(hipEventElapsedTime(&ms, s->e0, s->e1));
   unsigned long long tot[4] = {0, 0, 0, 0};
   HIPCHECK(hipMemcpy(tot, s->d_tot, sizeof tot, hipMemcpyDeviceToHost));
+  int taken = 0;
+  HIPCHECK(hipMemcpy(&taken, s->d_counter, sizeof taken, hipMemcpyDeviceToHost));
+  if (taken < s->last_chunk_jobs)  // every lane takes jobs until the list is exhausted
+    throw Error(PHX_E_DEVICE, "march kernel did not run to completion (" + std::to_string(taken) + " of " +
+                                  std::to_string(s->last_chunk_jobs) + " jobs taken)");
   info->kernel_s = ms * 1e-3;
   info->steps = (int64_t)tot[0];
   info->nplasma = (int64_t)tot[1];

@@ -180,13 +180,14 @@ PHX_DEV void two_sum_add(T& hi, T& lo, T d) {  // TwoSum + Fast2Sum (precision.h
   hi = s2;
 }
 // RK4 of x, k (geodesic) and F1 (parallel transport) with the FP32 south-hemisphere
-// fold (geodesic.hpp:222-256).  Keeps k1 and k2 + k3 instead of all four stages.
+// fold (geodesic.hpp:222-256).  The weighted stage sum k1 + 2 k2 + 2 k3 + k4 is
+// accumulated as the stages are formed, so only it and the next stage's offset are kept.
 template <class T>
 PHX_DEV void rk4(T a, T hs, Ray<T>& g, T dt) {
   constexpr bool F32 = Lim<T>::fp32;
   const T pi_hi = T(PI_D), pi_lo = T(PI_D - (double)T(PI_D));
   const bool south = F32 && g.hi[2] > T(0.5 * PI_D);
-  T k1[12], k23[12], inc[12];
+  T sum[12], inc[12];
 #pragma unroll
   for (int i = 0; i < 12; i++) inc[i] = T(0);
 #pragma unroll
@@ -210,30 +211,22 @@ PHX_DEV void rk4(T a, T hs, Ray<T>& g, T dt) {
       d[6] = -d[6];
       d[10] = -d[10];
     }
-    if (q == 0) {
+    const T wsum = (q == 0 || q == 3) ? T(1) : T(2);
 #pragma unroll
-      for (int i = 0; i < 12; i++) k1[i] = d[i];
-    } else if (q == 1) {
-#pragma unroll
-      for (int i = 0; i < 12; i++) k23[i] = d[i];
-    } else if (q == 2) {
-#pragma unroll
-      for (int i = 0; i < 12; i++) k23[i] = k23[i] + d[i];
-    } else {
-      const T dt6 = dt / T(6);
-#pragma unroll
-      for (int i = 0; i < 12; i++) {
-        const T incr = ((k1[i] + d[i]) + T(2) * k23[i]) * dt6;
-        if (F32)
-          two_sum_add(g.hi[i], g.lo[i], incr);
-        else
-          g.hi[i] += incr;
-      }
-      break;
-    }
+    for (int i = 0; i < 12; i++) sum[i] = q == 0 ? d[i] : sum[i] + wsum * d[i];
+    if (q == 3) break;
     const T wdt = (q == 2 ? T(1.0) : T(0.5)) * dt;
 #pragma unroll
     for (int i = 0; i < 12; i++) inc[i] = wdt * d[i];
+  }
+  const T dt6 = dt / T(6);
+#pragma unroll
+  for (int i = 0; i < 12; i++) {
+    const T incr = sum[i] * dt6;
+    if (F32)
+      two_sum_add(g.hi[i], g.lo[i], incr);
+    else
+      g.hi[i] += incr;
   }
 }
 template <class T>
@@ -679,16 +672,11 @@ PHX_DEV void segment_op(const Coef<T>& c, T h, T OA[16], T OS[4]) {
       for (int j = c0; j < 8; j++) A[r][j] = A[r][j] - f * A[c0][j];
     }
   }
-  // O_A = P^-1 N with N = I - h/2 K
+  // O_A = P^-1 N with N = I - h/2 K = 2 I - P, i.e. O_A = 2 P^-1 - I (no product needed)
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      T s = T(0);
-#pragma unroll
-      for (int k = 0; k < 4; k++) s += A[i][4 + k] * ((k == j ? T(1) : T(0)) - hh * K[k][j]);
-      OA[4 * i + j] = s;
-    }
+    for (int j = 0; j < 4; j++) OA[4 * i + j] = T(2) * A[i][4 + j] - (i == j ? T(1) : T(0));
   const T jv[4] = {c.jI, c.jQ, c.jU, c.jV};
 #pragma unroll
   for (int i = 0; i < 4; i++) {

@@ -503,7 +503,7 @@ def ensure_snapshot(name: str, directory: str) -> tuple:
     if not (os.path.exists(dump) and os.path.exists(grid) and os.path.exists(stamp)):
         root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
         subprocess.run([sys.executable, "-m", "phoenix_amd.snapshot", name, "--dir", directory, "--write"],
-                       check=True, cwd=root, env=_portable_env())
+                       check=True, cwd=root, env=_portable_env(), stdout=sys.stderr)
     with open(stamp) as f:
         parts = f.read().split()
     return dump, grid, {"sha256": parts[0], "nleafs": int(parts[1]), "cells": int(parts[2])}

@@ -43,11 +43,14 @@ def test_device_decode_matches_oracle(c1_run, snapdir, table, oracle_lib):
     o = O.Oracle(dump, grid, 64 if table == capi.TABLE_FP64 else 32)
     ref = o.tables()[0]
     o.close()
-    if table == capi.TABLE_FP64:
-        np.testing.assert_allclose(dev, ref, rtol=1e-12, atol=1e-300)
-    else:
-        # float32 of the same double: identical up to a final rounding of a 1e-15-different value
-        np.testing.assert_allclose(dev, ref, rtol=2 ** -23, atol=1e-40)
+    # scale per group (rho, u, Gamma v^i, B^i): a velocity component that vanishes analytically
+    # comes out as round-off of the others, so it is measured against the whole vector
+    per_var = np.abs(ref).max(axis=(0, 1))
+    scale = np.array([per_var[0], per_var[1]] + [per_var[2:5].max()] * 3 + [per_var[5:8].max()] * 3)
+    rtol, atol = (1e-12, 1e-14) if table == capi.TABLE_FP64 else (2.0 ** -23, 1e-12)
+    viol = np.abs(dev - ref) / (rtol * np.abs(ref) + atol * scale)
+    print(f"\ndecode table {table}: worst |dev - ref| / tolerance per variable", viol.max(axis=(0, 1)).round(3))
+    assert np.all(viol <= 1.0)
 
 
 @pytest.mark.parametrize("precision", [capi.FP64, capi.FP32])
@@ -65,6 +68,30 @@ def test_golden_case(case, precision, snapdir, tmp_path):
     if info["sha256"] == meta["sha256"]:
         for k, name in enumerate(("steps", "nplasma", "bad")):
             assert abs(inf[name] - int(st_same[k])) <= max(2, 2e-3 * int(st_same[k])), (name, inf[name], st_same)
+
+
+@pytest.mark.parametrize("nf", [1, 2, 3, 4, 5, 8, 9])
+def test_frequency_chunks_vs_oracle(nf, snapdir, tmp_path, oracle_lib):
+    """Every per-lane frequency template (1, 2, 4, 8 and chunks of 8) against the oracle."""
+    O = oracle_lib
+    dump, grid, _ = S.ensure_snapshot("c1", snapdir)
+    freqs = tuple(float(f) for f in np.geomspace(86e9, 690e9, nf)) if nf > 1 else (230e9,)
+    flist = "[" + ", ".join(repr(f) for f in freqs) + "]"
+    cfg = configs.write_config(str(tmp_path / "f.toml"), dump, grid, camera__img_width="20",
+                               camera__img_height="20", camera__frequency_list=flist)
+    run = capi.Run(cfg)
+    o = O.Oracle(dump, grid, 64, img_width=20, img_height=20, freqs=freqs)
+    ref, st = o.render()
+    o.close()
+    for precision in (capi.FP64, capi.FP32):
+        res, inf = run.dev_render(precision)
+        figs = parity.compare(res, ref)
+        print(f"\nnf={nf} fp{precision}: steps {inf['steps']} ref {st[0]} segs {inf['nplasma']} ref {st[1]} "
+              f"bad {inf['bad']} ref {st[2]} vgprs {inf['vgprs']} scratch {inf['scratch_bytes']} "
+              f"kernel {inf['kernel_s'] * 1e3:.2f} ms; {parity.summary(figs)}")
+        assert parity.passes(figs), parity.summary(figs)
+        assert (inf["steps"], inf["nplasma"]) == (st[0], st[1])
+        assert abs(inf["bad"] - st[2]) <= max(2, 1e-3 * st[2])
 
 
 def test_c2_inclination60_vs_oracle(snapdir, tmp_path, oracle_lib):
