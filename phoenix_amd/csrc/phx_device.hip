@@ -32,11 +32,14 @@ namespace phx {
 constexpr int BLOCK = 256;  // 4 wave64 per workgroup
 
 // ============================================================================ kernels
+// per-ray start record: hi[12] (x, k, F1), lo[8] (FP32 compensation of x, k), sign of beta, padding
+constexpr int RAY_REC = 24;
+
 template <class T, int NF>
 struct MarchArgs {
   Grid<T> G;
   Plasma<T> P;
-  Cam C;
+  const T* start;  // [njobs][RAY_REC], in job order
   T stepsize, r_out, cut_in;
   int max_steps, max_order;
   int nf_total, f0, nf, count_stats;
@@ -47,6 +50,85 @@ struct MarchArgs {
   phx_freqout* res;
   unsigned long long* tot;
 };
+
+// ---- one plasma sample, split into its frequency-independent part and the per-frequency
+// segment operator (src/core/render/pixel.hpp:52-123)
+template <class T>
+struct Prep {
+  T te, ne, B, sth, cth, kr1, kr0, c2, s2, ratio, dl;
+};
+// 0: no emission; 1: emits; 2: emits, but the rotation or the frequency ratio is NaN so every
+// frequency's operator will be non-finite and dropped
+template <class T, int NF>
+PHX_DEV int sample_prepare(const MarchArgs<T, NF>& A, const T px[4], const T pk[4], const T pf[4], T dl, int leaf,
+                           Prep<T>& sp) {
+  T k[4] = {pk[0], pk[1], pk[2], pk[3]};
+  const T rs = exp(px[1]);
+  const Met<T> gs = metric(A.P.a, A.P.hs, px[1], px[2]);
+  if (fabs(dot(gs, k, k)) > T(1e-6) && rs > T(2)) {  // re-solve k^t for k.k = 0 (geodesic.hpp:297-306)
+    const T aa = gs.d00;
+    const T bc = -(gs.d01 * k[1]) / aa - (gs.d03 * k[3]) / aa;
+    const T gz = -(gs.d11 * k[1] * k[1]) / aa - (T(2) * gs.d13 * k[1] * k[3]) / aa - (gs.d22 * k[2] * k[2]) / aa -
+                 (gs.d33 * k[3] * k[3]) / aa;
+    k[0] = bc + sqrt(bc * bc + gz);
+  }
+  Fluid<T> fl;
+  if (!sample_fluid(A.G, A.P, px, leaf, fl)) return 0;
+  const T pa = pitch_angle(fl.g, k, fl.b, fl.U);
+  if (pa == T(0) || pa == T(PI_D)) return 0;
+  T perp[4], para[4];
+  frame(fl.g, A.P.a, k, fl.U, fl.b, perp, para);
+  const T Cp = dot(fl.g, perp, pf), Sp = dot(fl.g, para, pf);
+  const T nn = Cp * Cp + Sp * Sp;
+  sp.c2 = (Cp * Cp - Sp * Sp) / nn;
+  sp.s2 = T(2) * Cp * Sp / nn;
+  sp.ratio = -dot(fl.g, k, fl.U);
+  if (__builtin_isnan(sp.c2) || __builtin_isnan(sp.s2) || __builtin_isnan(sp.ratio)) return 2;
+  sp.te = fl.te;
+  sp.ne = fl.ne;
+  sp.B = fl.B;
+  sp.dl = dl;
+  scs(pa, &sp.sth, &sp.cth);
+  kratios(fl.te, &sp.kr1, &sp.kr0);
+  return 1;
+}
+// segment operator of frequency f; false when it is not finite (the reference drops it)
+template <class T, int NF>
+PHX_DEV bool sample_op(const MarchArgs<T, NF>& A, const Prep<T>& sp, int f, T OA[16], T OS[4], T* dtau, T* dtauF) {
+  const T nup = A.freq[f] * sp.ratio;
+  const Coef<T> seg = invariant_rotated(thermal(sp.te, sp.ne, nup, sp.B, sp.sth, sp.cth, sp.kr1, sp.kr0), nup,
+                                        A.jscale[f], sp.c2, sp.s2);
+  const T h = sp.dl * A.hconst[f];
+  segment_op(seg, h, OA, OS);
+  bool fin = true;
+#pragma unroll
+  for (int q = 0; q < 16; q++) fin = fin && __builtin_isfinite(OA[q]);
+#pragma unroll
+  for (int q = 0; q < 4; q++) fin = fin && __builtin_isfinite(OS[q]);
+  *dtau = seg.aI * h;
+  *dtauF = fabs(seg.rV * h);
+  return fin;
+}
+// S += A O_S; A <- A O_A (operators.hpp:74-86), one row at a time in place
+template <class T>
+PHX_DEV void accumulate(T AA[16], T SS[4], const T OA[16], const T OS[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    T row[4];
+    T s = T(0);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      s += AA[4 * i + j] * OS[j];
+      T t = T(0);
+#pragma unroll
+      for (int kq = 0; kq < 4; kq++) t += AA[4 * i + kq] * OA[4 * kq + j];
+      row[j] = t;
+    }
+    SS[i] += s;
+#pragma unroll
+    for (int j = 0; j < 4; j++) AA[4 * i + j] = row[j];
+  }
+}
 
 // Persistent march: each lane owns one ray at a time; a lane whose ray ends writes it
 // out and takes the next job (wave-aggregated atomic on the job counter), so rays that
@@ -77,27 +159,15 @@ __global__ __launch_bounds__(BLOCK, Lim<T>::fp32 ? 2 : 1) void k_march(const Mar
       const int n = A.j0 + base + (int)__popcll(m & ((1ull << lane) - 1ull));
       if (n >= A.j1) break;
       idx = A.jobs[n];
-      const Cam& C = A.C;
-      // pixel -> impact parameters with the position-angle rotation (camera.hpp:11-23, jobs.hpp:64-76)
-      const int i = idx / C.H, j = idx % C.H;
-      const double a0 = -0.5 * C.fov_x + (i + 0.5) * C.fov_x / C.W;
-      const double b0 = -0.5 * C.fov_y + (j + 0.5) * C.fov_y / C.H;
-      const double alpha = a0 * C.cos_pa + b0 * C.sin_pa;
-      const double beta = -a0 * C.sin_pa + b0 * C.cos_pa;
-      double ph[8];
-      camera_photon(C, alpha, beta, ph);
-      T F1[4];
-      camera_F1<T>(A.P.a, A.P.hs, ph, F1);
+      // the ray's starting state, formed by k_rays (camera photon + polarization basis)
+      const T* st = A.start + (size_t)n * RAY_REC;
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
-        g.hi[q] = T(ph[q]);
-        g.lo[q] = F32 ? T(ph[q] - (double)g.hi[q]) : T(0);
-      }
+      for (int q = 0; q < 12; q++) g.hi[q] = st[q];
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        g.hi[8 + q] = F1[q];
-        g.lo[8 + q] = T(0);
-      }
+      for (int q = 0; q < 8; q++) g.lo[q] = F32 ? st[12 + q] : T(0);
+#pragma unroll
+      for (int q = 8; q < 12; q++) g.lo[q] = T(0);
+      const T bsign = st[20];
 #pragma unroll
       for (int f = 0; f < NF; f++) {
 #pragma unroll
@@ -108,8 +178,8 @@ __global__ __launch_bounds__(BLOCK, Lim<T>::fp32 ? 2 : 1) void k_march(const Mar
       }
       steps = turns = stop = np = bad = 0;
       thdot = dlp = T(0);
-      bneg = beta < 0.;
-      bpos = beta > 0.;
+      bneg = bsign < T(0);
+      bpos = bsign > T(0);
       have = true;
     }
 
@@ -151,75 +221,25 @@ __global__ __launch_bounds__(BLOCK, Lim<T>::fp32 ? 2 : 1) void k_march(const Mar
     // the pending sample is the state before the last step, i.e. sample number steps - 1; the
     // camera sample (number 0) is never processed (pixel.hpp:186)
     if (steps >= 2 && rs < A.P.rt_outer) {
-      T k[4] = {pk[0], pk[1], pk[2], pk[3]};
-      const Met<T> gs = metric(A.P.a, A.P.hs, px[1], px[2]);
-      if (fabs(dot(gs, k, k)) > T(1e-6) && rs > T(2)) {  // re-solve k^t for k.k = 0 (geodesic.hpp:297-306)
-        const T aa = gs.d00;
-        const T bc = -(gs.d01 * k[1]) / aa - (gs.d03 * k[3]) / aa;
-        const T gz = -(gs.d11 * k[1] * k[1]) / aa - (T(2) * gs.d13 * k[1] * k[3]) / aa - (gs.d22 * k[2] * k[2]) / aa -
-                     (gs.d33 * k[3] * k[3]) / aa;
-        k[0] = bc + sqrt(bc * bc + gz);
-      }
-      Fluid<T> fl;
-      const int in_vol = sample_fluid(A.G, A.P, px, leaf_next, fl);
-      if (in_vol) {
-        const T pa = pitch_angle(fl.g, k, fl.b, fl.U);
-        if (!(pa == T(0) || pa == T(PI_D))) {
-          T perp[4], para[4];
-          frame(fl.g, A.P.a, k, fl.U, fl.b, perp, para);
-          const T Cp = dot(fl.g, perp, pf), Sp = dot(fl.g, para, pf);
-          const T nn = Cp * Cp + Sp * Sp;
-          const T c2 = (Cp * Cp - Sp * Sp) / nn, s2 = T(2) * Cp * Sp / nn;
-          const T ratio = -dot(fl.g, k, fl.U);
-          np++;
-          if (__builtin_isnan(c2) || __builtin_isnan(s2) || __builtin_isnan(ratio)) {
-            // a NaN rotation or frequency ratio makes every frequency's operator non-finite: the
-            // reference drops each of them (pixel.hpp:107-116); count them without computing them
-            bad += A.nf;
-          } else {
-            T sth, cth;
-            scs(pa, &sth, &cth);
-            T kr1, kr0;
-            kratios(fl.te, &kr1, &kr0);
+      Prep<T> sp;
+      const int status = sample_prepare(A, px, pk, pf, pdl, leaf_next, sp);
+      if (status) np++;
+      if (status == 2) {
+        // a NaN rotation or frequency ratio makes every frequency's operator non-finite: the
+        // reference drops each of them (pixel.hpp:107-116); count them without computing them
+        bad += A.nf;
+      } else if (status == 1) {
 #pragma unroll
-            for (int f = 0; f < NF; f++) {
-              if (f >= A.nf) break;
-              const T nup = A.freq[f] * ratio;
-              const Coef<T> seg =
-                  invariant_rotated(thermal(fl.te, fl.ne, nup, fl.B, sth, cth, kr1, kr0), nup, A.jscale[f], c2, s2);
-              const T h = pdl * A.hconst[f];
-              T OA[16], OS[4];
-              segment_op(seg, h, OA, OS);
-              bool fin = true;
-#pragma unroll
-              for (int q = 0; q < 16; q++) fin = fin && __builtin_isfinite(OA[q]);
-#pragma unroll
-              for (int q = 0; q < 4; q++) fin = fin && __builtin_isfinite(OS[q]);
-              if (!fin) {
-                bad++;
-                continue;
-              }
-              // S += A O_S; A <- A O_A (operators.hpp:74-86), one row at a time in place
-#pragma unroll
-              for (int i = 0; i < 4; i++) {
-                T row[4];
-                T s = T(0);
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                  s += AA[f][4 * i + j] * OS[j];
-                  T t = T(0);
-#pragma unroll
-                  for (int kq = 0; kq < 4; kq++) t += AA[f][4 * i + kq] * OA[4 * kq + j];
-                  row[j] = t;
-                }
-                SS[f][i] += s;
-#pragma unroll
-                for (int j = 0; j < 4; j++) AA[f][4 * i + j] = row[j];
-              }
-              tau[f] += seg.aI * h;
-              tauF[f] += fabs(seg.rV * h);
-            }
+        for (int f = 0; f < NF; f++) {
+          if (f >= A.nf) break;
+          T OA[16], OS[4], dtau, dtauF;
+          if (!sample_op(A, sp, f, OA, OS, &dtau, &dtauF)) {
+            bad++;
+            continue;
           }
+          accumulate(AA[f], SS[f], OA, OS);
+          tau[f] += dtau;
+          tauF[f] += dtauF;
         }
       }
     }
@@ -266,6 +286,294 @@ __global__ __launch_bounds__(BLOCK, Lim<T>::fp32 ? 2 : 1) void k_march(const Mar
 
   // wave reduction, then one atomic per wave; steps and segments are counted by the first
   // frequency chunk only, dropped segments (per frequency) by every chunk
+  unsigned long long w_steps = s_steps, w_np = s_np, w_bad = s_bad;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    w_steps += __shfl_xor(w_steps, off);
+    w_np += __shfl_xor(w_np, off);
+    w_bad += __shfl_xor(w_bad, off);
+  }
+  if (lane == 0) {
+    if (A.count_stats) {
+      atomicAdd(&A.tot[0], w_steps);
+      atomicAdd(&A.tot[1], w_np);
+    }
+    atomicAdd(&A.tot[2], w_bad);
+  }
+}
+
+// Starting state of every job's ray: pixel -> impact parameters with the position-angle
+// rotation (camera.hpp:11-23, jobs.hpp:64-76), the RAPTOR camera photon in FP64
+// (camera.hpp:71-135), the camera polarization basis in the march's precision
+// (tetrad.hpp:111-133), and the compensated (hi, lo) split of the FP64 photon (geodesic.hpp:48-60).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_rays(const Cam C, T a, T hs, const int* __restrict__ jobs, int njobs,
+                                                T* __restrict__ out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= njobs) return;
+  const int idx = jobs[n];
+  const int i = idx / C.H, j = idx % C.H;
+  const double a0 = -0.5 * C.fov_x + (i + 0.5) * C.fov_x / C.W;
+  const double b0 = -0.5 * C.fov_y + (j + 0.5) * C.fov_y / C.H;
+  const double alpha = a0 * C.cos_pa + b0 * C.sin_pa;
+  const double beta = -a0 * C.sin_pa + b0 * C.cos_pa;
+  double ph[8];
+  camera_photon(C, alpha, beta, ph);
+  T F1[4];
+  camera_F1<T>(a, hs, ph, F1);
+  T* o = out + (size_t)n * RAY_REC;
+  for (int q = 0; q < 8; q++) {
+    const T hi = T(ph[q]);
+    o[q] = hi;
+    o[12 + q] = Lim<T>::fp32 ? T(ph[q] - (double)hi) : T(0);
+  }
+  for (int q = 0; q < 4; q++) o[8 + q] = F1[q];
+  o[20] = beta < 0. ? T(-1) : beta > 0. ? T(1) : T(0);
+  o[21] = o[22] = o[23] = T(0);
+}
+
+// ---------------------------------------------------------------------------- queued march
+// FP32 march with a per-wave sample queue in LDS.  Geodesic steps run on every lane each
+// pass; a lane whose pending sample can emit pushes it (position, wavevector, polarization
+// vector, step length, cache leaf) into its wave's queue instead of processing it at once.
+// When the queue holds 64 samples, when a lane holds KMAX of them, or when a ray with queued
+// samples ends, all active lanes of the wave compute the queued samples' segment operators
+// together, one sample per lane, and each owning lane then folds its own operators into its
+// accumulator in the order it queued them, which is the reference's order
+// (src/core/render/pixel.hpp:186-188).  The divergent, expensive part of a sample (fluid,
+// tetrad, coefficients, 4x4 inverse) thus runs on full waves.
+constexpr int QCAP = 128;  // slots per wave (at most 63 + 64 are in use at a flush)
+constexpr int QW = 24;     // scalars per slot: 14 in (x, k, F1, dl, leaf), 23 out (O_A, O_S, dtau, dtauF, flag, emits)
+constexpr int KMAX = 4;    // queued samples a lane may hold
+
+PHX_DEV void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class T, int NF>
+__global__ __launch_bounds__(BLOCK, 2) void k_march_q(const MarchArgs<T, NF> A) {
+  __shared__ T qbuf[BLOCK / 64][QCAP * QW];
+  T* const Q = qbuf[threadIdx.x >> 6];
+  constexpr bool F32 = Lim<T>::fp32;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  Ray<T> g;
+  T px[4] = {T(0), T(0), T(0), T(0)}, pk[4] = {T(0), T(0), T(0), T(0)}, pf[4] = {T(0), T(0), T(0), T(0)};
+  T pdl = T(0);
+  T AA[NF][16], SS[NF][4], tau[NF], tauF[NF];
+  int steps = 0, turns = 0, stop = 0, np = 0, bad = 0, idx = 0;
+  T thdot = T(0), dlp = T(0);
+  bool bneg = false, bpos = false, have = false, waiting = false;
+  unsigned s_steps = 0, s_np = 0, s_bad = 0;
+  int qcount = 0, nmine = 0;  // qcount is the same in every active lane of the wave
+  int myslot[KMAX];
+#pragma unroll
+  for (int j = 0; j < KMAX; j++) myslot[j] = 0;
+
+  for (;;) {
+    if (!have) {
+      const unsigned long long m = __ballot(1);
+      const int leader = __ffsll((long long)m) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(A.counter, (int)__popcll(m));
+      base = __shfl(base, leader);
+      const int n = A.j0 + base + (int)__popcll(m & lt);
+      if (n >= A.j1) break;
+      idx = A.jobs[n];
+      const T* st = A.start + (size_t)n * RAY_REC;
+#pragma unroll
+      for (int q = 0; q < 12; q++) g.hi[q] = st[q];
+#pragma unroll
+      for (int q = 0; q < 8; q++) g.lo[q] = F32 ? st[12 + q] : T(0);
+#pragma unroll
+      for (int q = 8; q < 12; q++) g.lo[q] = T(0);
+      const T bsign = st[20];
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+#pragma unroll
+        for (int q = 0; q < 16; q++) AA[f][q] = (q % 5 == 0) ? T(1) : T(0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) SS[f][q] = T(0);
+        tau[f] = tauF[f] = T(0);
+      }
+      steps = turns = stop = np = bad = 0;
+      thdot = dlp = T(0);
+      bneg = bsign < T(0);
+      bpos = bsign > T(0);
+      have = true;
+    }
+
+    // ---- view, liveness and the one AMR lookup of this pass
+    T cx[4], ck[4], cf[4];
+    bool alive = false, push = false, have_dxc = false;
+    int leaf_next = -1;
+    T dxc[3] = {T(0), T(0), T(0)};
+    if (!waiting) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        cx[q] = F32 ? g.hi[q] + g.lo[q] : g.hi[q];
+        ck[q] = F32 ? g.hi[4 + q] + g.lo[4 + q] : g.hi[4 + q];
+        cf[q] = F32 ? g.hi[8 + q] + g.lo[8 + q] : g.hi[8 + q];
+      }
+      const T r = exp(cx[1]);
+      alive = r < A.r_out && r > A.cut_in && steps < A.max_steps && !stop;
+      const T rs = exp(px[1]);
+      push = steps >= 2 && rs < A.P.rt_outer;
+      if (alive) {
+        T wx[4] = {cx[0], cx[1], cx[2], cx[3]};
+        wrap(wx);
+        const bool need_dxc = r < A.P.rt_outer;
+        const bool need_leaf = push && !off_domain(A.G, wx);
+        if (need_dxc || need_leaf) {
+          const int L = find_leaf(A.G, wx);
+          if (need_leaf) leaf_next = L;
+          if (need_dxc && L >= 0) {
+            have_dxc = true;
+#pragma unroll
+            for (int d = 0; d < 3; d++) dxc[d] = A.G.blk[12 * L + 6 + d];
+          }
+        }
+      }
+    }
+
+    // ---- queue the pending sample (every loop-resident lane takes part in the ballot)
+    const unsigned long long pm = __ballot(push);
+    if (push) {
+      const int slot = qcount + (int)__popcll(pm & lt);
+      T* e = Q + slot * QW;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        e[q] = px[q];
+        e[4 + q] = pk[q];
+        e[8 + q] = pf[q];
+      }
+      e[12] = pdl;
+      e[13] = T(leaf_next);
+#pragma unroll
+      for (int j = 0; j < KMAX; j++)
+        if (j == nmine) myslot[j] = slot;
+      nmine++;
+    }
+    qcount += (int)__popcll(pm);
+
+    if (!waiting) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        px[q] = cx[q];
+        pk[q] = ck[q];
+        pf[q] = cf[q];
+      }
+      pdl = dlp;
+      if (!alive) {
+        waiting = true;  // written out once its queued samples are folded in
+      } else {
+        if (thdot * ck[2] < T(0) && steps > 2) turns++;
+        thdot = ck[2];
+        if ((bneg && turns > A.max_order) || (bpos && turns > A.max_order + 1)) stop = 1;
+        const T dl = step_len(A.stepsize, pole_x2(g), ck, have_dxc, dxc);
+        rk4(A.P.a, A.P.hs, g, dl);
+        dlp = fabs(dl);
+        steps++;
+      }
+    }
+
+    // ---- flush: all active lanes compute the queued samples, owners accumulate in order
+    const bool want = qcount >= QCAP - 64 || nmine >= KMAX || (waiting && nmine > 0);
+    if (__ballot(want) != 0ull) {
+      const unsigned long long act = __ballot(1);
+      const int nact = (int)__popcll(act), rank = (int)__popcll(act & lt);
+      for (int base = 0; base < qcount; base += nact) {
+        const int e = base + rank;
+        const bool mine = e < qcount;
+        Prep<T> sp;
+        int status = 0;
+        if (mine) {
+          const T* in = Q + e * QW;
+          T x[4], k[4], f1[4];
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            x[q] = in[q];
+            k[q] = in[4 + q];
+            f1[q] = in[8 + q];
+          }
+          status = sample_prepare(A, x, k, f1, in[12], (int)in[13], sp);
+        }
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          if (f >= A.nf) break;
+          if (mine) {
+            T* out = Q + e * QW;
+            int flag = 0;  // 1: fold in, 2: dropped as non-finite
+            if (status == 1) {
+              T OA[16], OS[4], dtau, dtauF;
+              flag = sample_op(A, sp, f, OA, OS, &dtau, &dtauF) ? 1 : 2;
+              if (flag == 1) {
+#pragma unroll
+                for (int q = 0; q < 16; q++) out[q] = OA[q];
+#pragma unroll
+                for (int q = 0; q < 4; q++) out[16 + q] = OS[q];
+                out[20] = dtau;
+                out[21] = dtauF;
+              }
+            } else if (status == 2) {
+              flag = 2;
+            }
+            out[22] = T(flag);
+            out[23] = T(status != 0 ? 1 : 0);
+          }
+          wave_sync();
+#pragma unroll
+          for (int j = 0; j < KMAX; j++) {
+            const int sl = myslot[j];
+            if (j < nmine && sl >= base && sl < base + nact) {
+              const T* rr = Q + sl * QW;
+              const int flag = (int)rr[22];
+              if (f == 0 && rr[23] != T(0)) np++;
+              if (flag == 1) {
+                T OA[16], OS[4];
+#pragma unroll
+                for (int q = 0; q < 16; q++) OA[q] = rr[q];
+#pragma unroll
+                for (int q = 0; q < 4; q++) OS[q] = rr[16 + q];
+                accumulate(AA[f], SS[f], OA, OS);
+                tau[f] += rr[20];
+                tauF[f] += rr[21];
+              } else if (flag == 2) {
+                bad++;
+              }
+            }
+          }
+          wave_sync();
+        }
+      }
+      qcount = 0;
+      nmine = 0;
+    }
+
+    if (waiting) {
+      // ---- ray done: write it out and free the lane (pixel.hpp:213-223)
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        if (f >= A.nf) break;
+        double2* o = reinterpret_cast<double2*>(A.res + ((size_t)idx * A.nf_total + A.f0 + f));
+        if (np > 0) {
+          o[0] = make_double2((double)SS[f][0], (double)SS[f][1]);
+          o[1] = make_double2((double)SS[f][2], (double)SS[f][3]);
+          o[2] = make_double2((double)tau[f], (double)tauF[f]);
+        } else {
+          o[0] = o[1] = o[2] = make_double2(0., 0.);
+        }
+      }
+      s_steps += steps;
+      s_np += np;
+      s_bad += bad;
+      have = false;
+      waiting = false;
+    }
+  }
+
   unsigned long long w_steps = s_steps, w_np = s_np, w_bad = s_bad;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -376,6 +684,8 @@ struct phx_snapshot {
   size_t res_cap = 0;
   int* d_jobs = nullptr;
   size_t jobs_cap = 0;
+  void* d_start = nullptr;  // per-job ray starting states
+  size_t start_cap = 0;
   int* d_counter = nullptr;
   unsigned long long* d_tot = nullptr;
   double decode_s = 0., upload_s = 0.;
@@ -414,6 +724,7 @@ static void snapshot_free(phx_snapshot* s) {
   for (void* p : s->allocs) (void)hipFree(p);
   if (s->d_res) (void)hipFree(s->d_res);
   if (s->d_jobs) (void)hipFree(s->d_jobs);
+  if (s->d_start) (void)hipFree(s->d_start);
   if (s->e0) (void)hipEventDestroy(s->e0);
   if (s->e1) (void)hipEventDestroy(s->e1);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -630,20 +941,7 @@ static void launch_march(phx_snapshot* s, const Tables<T>& tb, const phx_model* 
   A.P.te_min = T(rq->thetae_min);
   A.P.te_max = T(rq->thetae_max);
   A.P.rt_outer = T(rq->rt_outer);
-  Cam& C = A.C;
-  C.a = (double)T(m.a);  // the camera uses the table-precision spin and stretching (camera.hpp:77)
-  C.hs = (double)T(m.hslope);
-  C.incl = rq->inclination;
-  C.r_cam = rq->r_cam;
-  C.t_init = rq->t_init;
-  C.phi_cam = rq->phi_cam;
-  C.fov_x = rq->fov_x;
-  C.fov_y = rq->fov_y;
-  C.cos_pa = rq->cos_pa;
-  C.sin_pa = rq->sin_pa;
-  C.W = rq->img_width;
-  C.H = rq->img_height;
-
+  A.start = (const T*)s->d_start;
   A.stepsize = T(rq->stepsize);
   A.r_out = T(rq->r_out);
   A.cut_in = T(rq->cut_in);
@@ -664,22 +962,38 @@ static void launch_march(phx_snapshot* s, const Tables<T>& tb, const phx_model* 
   A.counter = s->d_counter;
   A.res = s->d_res;
   A.tot = s->d_tot;
+  // FP32 with up to 4 frequencies per lane marches with the per-wave sample queue (PHX_MARCH=direct
+  // selects the direct kernel); FP64 and wider FP32 chunks march directly
+  bool queued = false;
+  const void* kfn = reinterpret_cast<const void*>(&k_march<T, NF>);
+  if constexpr (Lim<T>::fp32 && NF <= 4) {
+    const char* e = std::getenv("PHX_MARCH");
+    queued = !(e && !std::strcmp(e, "direct"));
+    if (queued) kfn = reinterpret_cast<const void*>(&k_march_q<T, NF>);
+  }
   int cus = 0, per_cu = 0;
   HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
-  HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march<T, NF>, BLOCK, 0));
+  HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, BLOCK, 0));
   per_cu = std::max(per_cu, 1);
   int grid = per_cu * cus;
   const int njobs = j1 - j0;
   grid = std::max(1, std::min(grid, (njobs + BLOCK - 1) / BLOCK));
   hipFuncAttributes fa;
-  HIPCHECK(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_march<T, NF>)));
+  HIPCHECK(hipFuncGetAttributes(&fa, kfn));
   // a kernel whose private segment exceeds the per-lane stack limit does not run at all; raise the
   // limit to what the kernel needs (the reference sets it explicitly too, kernel.cu:305)
   size_t limit = 0;
   HIPCHECK(hipDeviceGetLimit(&limit, hipLimitStackSize));
   if (fa.localSizeBytes > limit) HIPCHECK(hipDeviceSetLimit(hipLimitStackSize, (fa.localSizeBytes + 255) & ~size_t(255)));
   HIPCHECK(hipMemsetAsync(s->d_counter, 0, sizeof(int), s->stream));
-  k_march<T, NF><<<grid, BLOCK, 0, s->stream>>>(A);
+  if constexpr (Lim<T>::fp32 && NF <= 4) {
+    if (queued)
+      k_march_q<T, NF><<<grid, BLOCK, 0, s->stream>>>(A);
+    else
+      k_march<T, NF><<<grid, BLOCK, 0, s->stream>>>(A);
+  } else {
+    k_march<T, NF><<<grid, BLOCK, 0, s->stream>>>(A);
+  }
   HIPCHECK(hipGetLastError());
   s->last_chunk_jobs = njobs;
   if (info) {
@@ -696,6 +1010,34 @@ static void launch_march(phx_snapshot* s, const Tables<T>& tb, const phx_model* 
 template <class T>
 static void march_all(phx_snapshot* s, const Tables<T>& tb, const phx_model* rq, const int* d_jobs, int njobs,
                       phx_info* info, phx_progress_fn progress, void* user) {
+  // ray starting states for every job (shared by all frequency chunks)
+  const size_t need = (size_t)std::max(njobs, 1) * RAY_REC * sizeof(T);
+  if (need > s->start_cap) {
+    if (s->d_start) HIPCHECK(hipFree(s->d_start));
+    s->d_start = nullptr;
+    HIPCHECK(hipMalloc(&s->d_start, need));
+    s->start_cap = need;
+  }
+  {
+    const phx_model& m = s->geo;
+    Cam C;
+    C.a = (double)T(m.a);  // the camera uses the table-precision spin and stretching (camera.hpp:77)
+    C.hs = (double)T(m.hslope);
+    C.incl = rq->inclination;
+    C.r_cam = rq->r_cam;
+    C.t_init = rq->t_init;
+    C.phi_cam = rq->phi_cam;
+    C.fov_x = rq->fov_x;
+    C.fov_y = rq->fov_y;
+    C.cos_pa = rq->cos_pa;
+    C.sin_pa = rq->sin_pa;
+    C.W = rq->img_width;
+    C.H = rq->img_height;
+    if (njobs > 0)
+      k_rays<T><<<(njobs + BLOCK - 1) / BLOCK, BLOCK, 0, s->stream>>>(C, T(m.a), T(m.hslope), d_jobs, njobs,
+                                                                      (T*)s->d_start);
+    HIPCHECK(hipGetLastError());
+  }
   int chunk = njobs;
   if (const char* e = std::getenv("PHX_CHUNK"))
     if (std::atoi(e) > 0) chunk = std::atoi(e);

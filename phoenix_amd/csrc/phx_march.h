@@ -38,6 +38,58 @@ template <> struct Lim<double> {  // precision.hpp:53-58
 PHX_DEV void scs(float x, float* s, float* c) { sincosf(x, s, c); }
 PHX_DEV void scs(double x, double* s, double* c) { sincos(x, s, c); }
 
+// sin and cos of a float |x| < ~100: Cody-Waite reduction by pi/2 (two-part constant,
+// fma), Taylor polynomials on [-pi/4, pi/4] (sin to x^9, cos to x^10: truncation below
+// 2e-9), quadrant by the low bits of the quotient.
+PHX_DEV void sincos_small(float x, float* s, float* c) {
+  const float q = rintf(x * 0.63661977236758134f);
+  const int qi = (int)q;
+  float r = fmaf(-q, 1.57079637050628662109375f, x);
+  r = fmaf(-q, -4.37113900018624283e-8f, r);
+  const float z = r * r;
+  float ps = fmaf(z, 2.7557319223985893e-6f, -1.9841269841269841e-4f);
+  ps = fmaf(ps, z, 8.3333333333333333e-3f);
+  ps = fmaf(ps, z, -1.6666666666666667e-1f);
+  ps = fmaf(ps * z, r, r);
+  float pc = fmaf(z, -2.7557319223985888e-7f, 2.4801587301587302e-5f);
+  pc = fmaf(pc, z, -1.3888888888888889e-3f);
+  pc = fmaf(pc, z, 4.1666666666666667e-2f);
+  pc = fmaf(pc, z, -0.5f);
+  pc = fmaf(pc, z, 1.0f);
+  float sn = (qi & 1) ? pc : ps, cs = (qi & 1) ? ps : pc;
+  if (qi & 2) sn = -sn;
+  if ((qi + 1) & 2) cs = -cs;
+  *s = sn;
+  *c = cs;
+}
+
+// sin/cos of 2 x2 and of theta = x2 + (h/2) sin 2x2 (the MKS-BHAC stretching).  FP32: one
+// sincos of x2, the double-angle identities, and theta = x2 + delta by angle addition with
+// |delta| <= h/2 <= 0.5 expanded in Taylor series (to delta^9 / delta^10, error < 1e-9);
+// FP64: the library sincos of both angles.
+PHX_DEV void theta_trig(float x2, float hs, float* s2x, float* c2x, float* s, float* c) {
+  float sx, cx;
+  sincos_small(x2, &sx, &cx);
+  *s2x = 2.0f * sx * cx;
+  *c2x = (cx - sx) * (cx + sx);
+  const float d = 0.5f * hs * *s2x, z = d * d;
+  float sd = fmaf(z, 2.7557319223985893e-6f, -1.9841269841269841e-4f);
+  sd = fmaf(sd, z, 8.3333333333333333e-3f);
+  sd = fmaf(sd, z, -1.6666666666666667e-1f);
+  sd = fmaf(sd * z, d, d);
+  float cd = fmaf(z, -2.7557319223985888e-7f, 2.4801587301587302e-5f);
+  cd = fmaf(cd, z, -1.3888888888888889e-3f);
+  cd = fmaf(cd, z, 4.1666666666666667e-2f);
+  cd = fmaf(cd, z, -0.5f);
+  cd = fmaf(cd, z, 1.0f);
+  *s = sx * cd + cx * sd;
+  *c = cx * cd - sx * sd;
+}
+PHX_DEV void theta_trig(double x2, double hs, double* s2x, double* c2x, double* s, double* c) {
+  sincos(2. * x2, s2x, c2x);
+  sincos(x2 + 0.5 * hs * *s2x, s, c);
+}
+
 // ------------------------------------------------------------------ metric
 // Kerr-Schild in MKS-BHAC coordinates (t, ln r, x2, phi), theta = x2 + h/2 sin 2x2
 // (src/core/geometry/metric.hpp:26-142).  Only the 7 non-zero covariant and 6
@@ -52,8 +104,7 @@ PHX_DEV Met<T> metric(T a, T hs, T x1, T x2) {
   Met<T> g;
   const T r = exp(x1);
   T s2x, c2x, s, c;
-  scs(T(2) * x2, &s2x, &c2x);
-  scs(x2 + T(0.5) * hs * s2x, &s, &c);
+  theta_trig(x2, hs, &s2x, &c2x, &s, &c);
   const T s2 = s * s, rho2 = r * r + a * a * c * c, hf = T(1) + hs * c2x;
   const T z = T(2) * r / rho2;
   g.d00 = -T(1) + z;
@@ -112,11 +163,10 @@ PHX_DEV Geo<T> geo_at(T a, T hs, T x1, T x2) {
   G.r = exp(x1);
   G.r2 = G.r * G.r;
   T s2x, c2x;
-  scs(T(2) * x2, &s2x, &c2x);
+  theta_trig(x2, hs, &s2x, &c2x, &G.s, &G.c);
   G.tp = T(1) + hs * c2x;
   G.tp2 = G.tp * G.tp;
   G.tpp = -T(2) * hs * s2x;
-  scs(x2 + T(0.5) * hs * s2x, &G.s, &G.c);
   G.s2 = G.s * G.s;
   G.s2th = T(2) * G.s * G.c;
   G.cot = G.c / G.s;
