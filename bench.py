@@ -1,28 +1,33 @@
 #!/usr/bin/env python3
 """Throughput of the polarized GRRT raymarch on one MI355X, in rays x frequencies per second.
 
-Workload (one "step" = one full render of one image on a snapshot resident in HBM):
-the 2-level AMR torus snapshot (96x48x48 base cells in 12^3 blocks, refined where
-rho > 0.1 and rho > 0.5 of the maximum: 464 leaves, 0.80M cells, spin 0.9375),
-a 1024 x 1024 image at 230 GHz, inclination 10 deg, m_unit 1.2e18, R_high 3,
-camera at r = 1e4, fov 40 x 40 Rg.  The snapshot is generated (seeded) on first
-use, decoded on the device once and kept resident; the timed region renders
+Headline (one "step" = one full render of one image on a snapshot resident in HBM): the
+2-level AMR torus snapshot (96x48x48 base cells in 12^3 blocks, refined where rho > 0.1 and
+rho > 0.5 of the maximum: 464 leaves, 0.80M cells, spin 0.9375), a 1024 x 1024 image at
+230 GHz, inclination 10 deg, m_unit 1.2e18, R_high 3, camera at r = 1e4, fov 40 x 40 Rg,
+marched by the mixed-precision path (FP32 bulk; FP64 ray state, FP64 steps near the horizon
+and the poles, FP64 cell location, Kahan Stokes sums).  The same image by the plain FP32 path
+(the reference's FP32 method) and the FP64 path is timed beside it.  The snapshot is generated
+(seeded) on first use, decoded on the device once and kept resident; the timed region renders
 into device memory (no host copies).
 
-Prints one JSON line with the roofline of the march kernel (algorithmic flops
-per launch from the counted operation mix in tests/golden/op_counts.json times
-the launch's own step / segment counts, over its HIP-event duration) and a CPU
-baseline: the reference's CPU build (oracle/_ref, when present) or the oracle,
-timed on a strided sample of the same image on the host's cores.
+Also measured (skip with --quick): the 64-image batch of uneven size in one persistent launch
+(autotuned) against rendering the same images one after another with the plain FP32 path, and
+the serving loop on two resident 1.9M-cell snapshots under Poisson arrivals at 80% of its
+measured saturation (p50 / p99 latency against the same kind of requests served one at a time).
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--precision fp32|fp64]
+Prints one JSON line with the roofline of the march kernel (algorithmic flops per launch from
+the counted operation mix in tests/golden/op_counts.json times the launch's own step / segment
+counts, over its HIP-event duration) and a CPU baseline: the reference's CPU build (oracle/_ref,
+when present) or the oracle, timed on a strided sample of the same image on the host's cores.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--precision mixed|fp32|fp64] [--quick]
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
-import subprocess
 import sys
 import tempfile
 import time
@@ -38,21 +43,56 @@ W = H = 1024
 FREQ = 230e9
 
 
-def workload_config(tmpdir: str):
+def log(msg: str) -> None:
+    print("# " + msg, file=sys.stderr, flush=True)
+
+
+def workload_config(tmpdir: str, preset: str = "c2", name: str = "bench"):
     from phoenix_amd import configs, snapshot as S
 
-    dump, grid, info = S.ensure_snapshot("c2", S.default_dir())
-    cfg = configs.write_config(os.path.join(tmpdir, "bench.toml"), dump, grid, camera__img_width=str(W),
+    dump, grid, info = S.ensure_snapshot(preset, S.default_dir())
+    cfg = configs.write_config(os.path.join(tmpdir, f"{name}.toml"), dump, grid, camera__img_width=str(W),
                                camera__img_height=str(H), camera__frequency="230e9",
                                camera__inclination="10", plasma__m_unit="1.2e18", plasma__r_high="3")
     return cfg, dump, grid, info
 
 
-def algorithmic_flops(info: dict, nf: int, precision: str) -> float:
+def algorithmic_flops(info: dict, nf: int, precision: str) -> tuple:
+    """(FP32 flops, FP64 flops) of one launch: counted ops per step and per emitting segment x
+    frequency (the oracle's operation count of this workload) times the launch's own counts.
+    The mixed path's FP64 steps are counted as FP64 work."""
     with open(os.path.join(ROOT, "tests", "golden", "op_counts.json")) as f:
-        oc = json.load(f)["c2_1024_230GHz"][precision]
+        oc = json.load(f)["c2_1024_230GHz"]
     useful_segf = info["nplasma"] * nf - info["bad"]
-    return info["steps"] * oc["F_step"] + useful_segf * oc["F_segf"]
+    if precision == "fp64":
+        o = oc["fp64"]
+        return 0.0, info["steps"] * o["F_step"] + useful_segf * o["F_segf"]
+    o32, o64 = oc["fp32"], oc["fp64"]
+    s64 = info.get("fp64_steps", 0) if precision == "mixed" else 0
+    return (info["steps"] - s64) * o32["F_step"] + useful_segf * o32["F_segf"], s64 * o64["F_step"]
+
+
+def measured_traffic(precision: str):
+    """HBM bytes per launch of the march kernel on this workload, from the copy-calibrated
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary kept in profiles/, or None."""
+    path = os.path.join(ROOT, "profiles", f"k_march_{precision}_c2_1024_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("hbm_bytes_per_launch"):
+                return float(line.split("=", 1)[1].split()[0])
+    return None
+
+
+def roofline(info: dict, nf: int, precision: str, kernel_s: float) -> dict:
+    w32, w64 = algorithmic_flops(info, nf, precision)
+    achieved = (w32 + w64) / kernel_s / 1e12
+    # the peak of this instruction mix: FP32 work at the FP32 peak, FP64 work at the FP64 peak
+    t_peak = w32 / (PEAK_TFLOPS["fp32"] * 1e12) + w64 / (PEAK_TFLOPS["fp64"] * 1e12)
+    peak = (w32 + w64) / t_peak / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": measured_traffic(precision)}
 
 
 def cpu_baseline(cfg: str, dump: str, grid: str, threads: int, stride: int) -> dict:
@@ -76,14 +116,148 @@ def cpu_baseline(cfg: str, dump: str, grid: str, threads: int, stride: int) -> d
             "sample": sample + f", {secs:.1f} s, FP32 build"}
 
 
+def timed_renders(snap, prec, steps, warmup, sync, label):
+    for i in range(warmup):
+        _, info = snap.render(None, prec, to_host=False)
+        log(f"{label} warmup {i}: kernel {info['kernel_s'] * 1e3:.1f} ms")
+    sync()
+    t0 = time.perf_counter()
+    kern, infos = [], []
+    for _ in range(steps):
+        _, info = snap.render(None, prec, to_host=False)
+        kern.append(info["kernel_s"])
+        infos.append(info)
+    sync()
+    return time.perf_counter() - t0, kern, infos[-1]
+
+
+def c4_requests(run, seed: int = 4):
+    """64 images: 8 x 64^2, 16 x 128^2, 16 x 256^2, 16 x 512^2, 8 x 1024^2; inclination U(5, 175);
+    frequency from {86, 230, 345} GHz; m_unit log-uniform in [5e17, 5e18]; R_high U(1, 40)."""
+    rng = np.random.default_rng(seed)
+    sizes = [64] * 8 + [128] * 16 + [256] * 16 + [512] * 16 + [1024] * 8
+    return [run.request(img_width=n, img_height=n, inclination=float(rng.uniform(5, 175)),
+                        freqs=(float(rng.choice([86e9, 230e9, 345e9])),),
+                        m_unit=float(10 ** rng.uniform(np.log10(5e17), np.log10(5e18))),
+                        r_high=float(rng.uniform(1, 40))) for n in sizes]
+
+
+def c5_request(run, rng):
+    """resolution from {128^2 .. 1024^2}, 1-4 frequencies log-uniform in 86-690 GHz, inclination
+    U(5, 175), m_unit and R_high as in the 64-image batch"""
+    n = int(rng.choice([128, 256, 512, 1024]))
+    nf = int(rng.integers(1, 5))
+    freqs = tuple(float(f) for f in np.sort(10 ** rng.uniform(np.log10(86e9), np.log10(690e9), nf)))
+    return run.request(img_width=n, img_height=n, inclination=float(rng.uniform(5, 175)), freqs=freqs,
+                       m_unit=float(10 ** rng.uniform(np.log10(5e17), np.log10(5e18))),
+                       r_high=float(rng.uniform(1, 40)))
+
+
+def bench_batch(run, snap, capi, prec, sync) -> dict:
+    reqs = c4_requests(run)
+    rays = sum(r.img_width * r.img_height * r.nfreqs for r in reqs)
+    t0 = time.perf_counter()
+    tune, sampled = snap.autotune(reqs, prec, sample_stride=32)
+    t_tune = time.perf_counter() - t0
+    log(f"c4 autotune {tune} ({t_tune:.1f} s)")
+    sync()
+    t0 = time.perf_counter()
+    _, info = snap.render_batch(reqs, prec, to_host=False)
+    sync()
+    wall = time.perf_counter() - t0
+    snap.set_tune()
+    t0 = time.perf_counter()
+    _, info32 = snap.render_batch(reqs, capi.FP32, to_host=False)
+    sync()
+    wall32 = time.perf_counter() - t0
+    # the same images one after another by the plain FP32 path (samples processed in place, as the
+    # reference's kernel does)
+    seq_kernel = 0.0
+    t0 = time.perf_counter()
+    for r in reqs:
+        _, i1 = snap.render(r, capi.FP32, to_host=False, plain=True)
+        seq_kernel += i1["kernel_s"]
+    sync()
+    wall_seq = time.perf_counter() - t0
+    log(f"c4 batch: {rays / wall:.4g} rays*freq/s batched, {rays / wall32:.4g} batched fp32, "
+        f"{rays / wall_seq:.4g} one at a time plain fp32")
+    return {"images": len(reqs), "rays": rays, "tune": tune, "autotune_s": t_tune,
+            "batched_rays_per_s": rays / wall, "batched_kernel_s": info["kernel_s"],
+            "batched_fp32_rays_per_s": rays / wall32,
+            "sequential_plain_fp32_rays_per_s": rays / wall_seq, "sequential_kernel_s": seq_kernel,
+            "batched_vs_sequential_plain_fp32": wall_seq / wall}
+
+
+def bench_serve(capi, prec, tmpdir) -> dict:
+    runs, snaps = [], []
+    for name in ("c5", "c5_a05"):
+        cfg, _, _, si = workload_config(tmpdir, name, name)
+        r = capi.Run(cfg)
+        runs.append(r)
+        snaps.append(capi.Snapshot(r, capi.TABLE_FP32 | (capi.TABLE_FP64 if prec == capi.FP64 else 0)))
+        log(f"serving snapshot {name}: {si['nleafs']} leaves, {si['cells']} cells")
+    rng = np.random.default_rng(5)
+
+    def draw(n):
+        out = []
+        for _ in range(n):
+            k = int(rng.integers(0, 2))
+            out.append((k, c5_request(runs[k], rng)))
+        return out
+
+    srv = capi.Server(snaps, prec)
+    try:
+        # saturation: a closed burst of requests
+        burst = draw(24)
+        t0 = time.perf_counter()
+        for t in [srv.submit(k, r) for k, r in burst]:
+            srv.wait(t)
+        sat_wall = time.perf_counter() - t0
+        sat_rays = sum(r.img_width * r.img_height * r.nfreqs for _, r in burst)
+        sat_req_rate = len(burst) / sat_wall
+        log(f"serving saturation: {sat_rays / sat_wall:.4g} rays*freq/s, {sat_req_rate:.3g} requests/s")
+        # single-request latency: the same kind of requests, one at a time
+        single = [srv.wait(srv.submit(k, r))[1] for k, r in draw(12)]
+        # Poisson arrivals at 80% of saturation
+        load = 0.8
+        lam = load * sat_req_rate
+        srv.reset_stats()
+        arrivals = draw(40)
+        t_next = time.perf_counter()
+        tickets = []
+        for k, r in arrivals:
+            t_next += rng.exponential(1.0 / lam)
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            tickets.append(srv.submit(k, r))
+        lat = [srv.wait(t)[1] for t in tickets]
+        st = srv.stats()
+    finally:
+        srv.stop()
+        for s in snaps:
+            s.close()
+        for r in runs:
+            r.close()
+    p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+    s50, s99 = float(np.percentile(single, 50)), float(np.percentile(single, 99))
+    log(f"serving at {load:.0%} load: p50 {p50 * 1e3:.0f} ms, p99 {p99 * 1e3:.0f} ms; one at a time p50 "
+        f"{s50 * 1e3:.0f} ms, p99 {s99 * 1e3:.0f} ms")
+    return {"snapshots": ["c5 spin 0.9375", "c5 spin 0.5"], "load": load,
+            "saturation_rays_per_s": sat_rays / sat_wall, "saturation_requests_per_s": sat_req_rate,
+            "requests": len(arrivals), "throughput_rays_per_s": st["rays"] / max(st["wall_s"], 1e-9),
+            "p50_ms": p50 * 1e3, "p99_ms": p99 * 1e3, "single_p50_ms": s50 * 1e3, "single_p99_ms": s99 * 1e3,
+            "p99_vs_single_p99": p99 / s99, "launches": st["launches"], "peak_result_bytes": st["peak_bytes"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--precision", choices=["mixed", "fp32", "fp64"], default="mixed")
+    ap.add_argument("--quick", action="store_true", help="the headline image only (no batch or serving runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 side measurement")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-stride", type=int, default=16)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -113,60 +287,82 @@ def main() -> int:
     except Exception:  # torch is plumbing only; the library synchronises its own stream
         sync = lambda: None  # noqa: E731
 
-    prec = capi.FP32 if args.precision == "fp32" else capi.FP64
+    precs = {"fp32": capi.FP32, "fp64": capi.FP64, "mixed": capi.MIXED}
+    prec = precs[args.precision]
+    extra = {}
     with tempfile.TemporaryDirectory() as td:
         cfg, dump, grid, sinfo = workload_config(td)
-        print(f"# snapshot c2: {sinfo['nleafs']} leaves, {sinfo['cells']} cells", file=sys.stderr, flush=True)
+        log(f"snapshot c2: {sinfo['nleafs']} leaves, {sinfo['cells']} cells")
         run = capi.Run(cfg)
         snap = capi.Snapshot(run, capi.TABLE_FP32 | capi.TABLE_FP64)
         nf = run.model.nfreqs
-        print(f"# resident snapshot {snap.bytes / 2**20:.1f} MiB", file=sys.stderr, flush=True)
-        for i in range(args.warmup):
-            _, info = snap.render(None, prec, to_host=False)
-            print(f"# warmup {i}: kernel {info['kernel_s'] * 1e3:.1f} ms", file=sys.stderr, flush=True)
-        sync()
-        t0 = time.perf_counter()
-        kern, infos = [], []
-        for i in range(args.steps):
-            _, info = snap.render(None, prec, to_host=False)
-            kern.append(info["kernel_s"])
-            infos.append(info)
-        sync()
-        wall = time.perf_counter() - t0
         rays = W * H * nf
+        log(f"resident snapshot {snap.bytes / 2**20:.1f} MiB")
+        wall, kern, info = timed_renders(snap, prec, args.steps, args.warmup, sync, args.precision)
         value = args.steps * rays / wall
-        info = infos[-1]
-        flops = algorithmic_flops(info, nf, args.precision)
-        achieved = flops / float(np.mean(kern)) / 1e12
-        peak = PEAK_TFLOPS[args.precision]
-        extra = {"kernel_ms_mean": float(np.mean(kern) * 1e3), "kernel_ms_min": float(np.min(kern) * 1e3),
-                 "steps_per_ray": info["steps"] / (W * H), "segments_per_ray": info["nplasma"] / (W * H),
-                 "dropped_segments": info["bad"], "vgprs": info["vgprs"], "scratch_bytes": info["scratch_bytes"],
-                 "waves_per_cu": info["waves_per_cu"], "grid": [info["blocks"], info["block"]],
-                 "device": info["device"], "decode_s": info["decode_s"],
-                 "algorithmic_tflop_per_launch": flops / 1e12,
-                 "frac_of_unpacked_fp32_peak": achieved / 78.6 if args.precision == "fp32" else None}
-        if not args.no_fp64 and args.precision == "fp32":
-            _, i64 = snap.render(None, capi.FP64, to_host=False)
-            extra["fp64_rays_freq_per_s"] = rays / i64["kernel_s"]
-            extra["fp64_kernel_ms"] = i64["kernel_s"] * 1e3
-        print(f"# timed {args.steps} renders: {wall:.3f} s, {value:.4g} rays*freq/s", file=sys.stderr, flush=True)
+        kernel_s = float(np.mean(kern))
+        roof = roofline(info, nf, args.precision, kernel_s)
+        extra.update({"kernel_ms_mean": kernel_s * 1e3, "kernel_ms_min": float(np.min(kern) * 1e3),
+                      "steps_per_ray": info["steps"] / (W * H), "segments_per_ray": info["nplasma"] / (W * H),
+                      "dropped_segments": info["bad"],
+                      "fp64_step_fraction": info["fp64_steps"] / max(1, info["steps"]),
+                      "vgprs": info["vgprs"], "scratch_bytes": info["scratch_bytes"], "lds_bytes": info["lds_bytes"],
+                      "waves_per_cu": info["waves_per_cu"], "grid": [info["blocks"], info["block"]],
+                      "device": info["device"], "decode_s": info["decode_s"],
+                      "algorithmic_tflop_per_launch": sum(algorithmic_flops(info, nf, args.precision)) / 1e12,
+                      "frac_of_unpacked_fp32_peak": roof["achieved"] / 78.6})
+        log(f"timed {args.steps} renders ({args.precision}): {wall:.3f} s, {value:.4g} rays*freq/s")
+        # the same image by the other paths; "plain_fp32" is the reference's method as the
+        # reference's kernel runs it (samples processed in place)
+        others = [("fp32", capi.FP32, False), ("plain_fp32", capi.FP32, True), ("fp64", capi.FP64, False),
+                  ("mixed", capi.MIXED, False)]
+        for name, p2, plain in others:
+            if name == args.precision:
+                continue
+            steps = 1 if name == "fp64" else args.steps
+            for i in range(1):
+                snap.render(None, p2, to_host=False, plain=plain)
+            sync()
+            t0 = time.perf_counter()
+            k2, i2 = [], None
+            for _ in range(steps):
+                _, i2 = snap.render(None, p2, to_host=False, plain=plain)
+                k2.append(i2["kernel_s"])
+            sync()
+            w2 = time.perf_counter() - t0
+            extra[f"{name}_rays_freq_per_s"] = steps * rays / w2
+            extra[f"{name}_kernel_ms"] = float(np.mean(k2)) * 1e3
+            extra[f"{name}_roofline_frac"] = roofline(i2, nf, "fp32" if name == "plain_fp32" else name,
+                                                      float(np.mean(k2)))["frac"]
+        if "plain_fp32_rays_freq_per_s" in extra:
+            extra["value_vs_plain_fp32"] = value / extra["plain_fp32_rays_freq_per_s"]
+        if not args.quick:
+            try:
+                extra["batch_c4"] = bench_batch(run, snap, capi, prec, sync)
+            except Exception as e:  # reported, not fatal: the headline is already measured
+                extra["batch_c4"] = {"error": str(e)}
         snap.close()
         run.close()
+        if not args.quick:
+            try:
+                extra["serving_c5"] = bench_serve(capi, prec, td)
+            except Exception as e:
+                extra["serving_c5"] = {"error": str(e)}
         cpu = None
         if not args.no_cpu_baseline:
-            print("# cpu baseline ...", file=sys.stderr, flush=True)
+            log("cpu baseline ...")
             cpu = cpu_baseline(cfg, dump, grid, args.cpu_threads, args.cpu_stride)
+    label = {"mixed": "mixed precision (FP32 bulk, FP64 near horizon/poles and cell location, Kahan Stokes)",
+             "fp32": "FP32 (the reference's FP32 method)", "fp64": "FP64"}[args.precision]
     out = {
         "metric": METRIC, "value": value, "unit": "rays·freq/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.precision == "fp32" else "f64",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic (seeded Fishbone-Moncrief torus snapshot, BHAC format)",
         "config": {"workload": "c2 snapshot (96x48x48 in 12^3 blocks, 2 AMR levels, 0.80M cells), 1024x1024, "
-                               "230 GHz, i=10, m_unit 1.2e18, R_high 3, plain " + args.precision.upper(),
-                   "image": [W, H], "frequencies_hz": [FREQ], "rays_per_step": W * H * nf},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None},
+                               "230 GHz, i=10, m_unit 1.2e18, R_high 3, " + label,
+                   "image": [W, H], "frequencies_hz": [FREQ], "rays_per_step": rays},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "extra": extra,
     }

@@ -51,7 +51,8 @@ typedef struct phx_info {
   int32_t lds_bytes, waves_per_cu;     /* static LDS per workgroup; occupancy the launch assumed */
   double upload_s, kernel_s, decode_s; /* table upload, march kernels (HIP events), device decode */
   int64_t steps, nplasma, bad;         /* sums over all rays (src/core/render/pixel.hpp:34-37) */
-  int64_t rays;                        /* rays marched (x frequency chunks) */
+  int64_t rays;                        /* rays x frequencies rendered */
+  int64_t fp64_steps;                  /* PHX_MIXED: geodesic steps taken in FP64 (near the horizon / poles) */
 } phx_info;
 
 /* device::Progress (src/device/kernel.cuh:32): called on the caller's thread after each chunk */
@@ -158,7 +159,8 @@ int phx_dev_render(const phx_model* model, const char* dump_file, int32_t precis
  * across renders; requests then differ only in camera / plasma / frequency parameters. */
 typedef struct phx_snapshot phx_snapshot;
 
-/* precisions: bit set of PHX_FP32 -> 1, PHX_FP64 -> 2 (PHX_MIXED uses both tables) */
+/* cell tables to keep: bit set (PHX_FP32 and PHX_MIXED renders use the FP32 cell table, PHX_FP64
+ * the FP64 one; the block tables of both precisions are always resident) */
 #define PHX_TABLE_FP32 1
 #define PHX_TABLE_FP64 2
 int phx_snapshot_open(const phx_model* model, const char* dump_file, int32_t tables, phx_snapshot** out);
@@ -174,6 +176,66 @@ int phx_snapshot_render(phx_snapshot* snap, const phx_model* request, int32_t pr
                         int32_t njobs, phx_freqout* res, phx_info* info);
 /* copies the device result of the last phx_snapshot_render into res */
 int phx_snapshot_result(phx_snapshot* snap, phx_freqout* res, int64_t nres);
+
+/* Batch render: the n requests (complete models, e.g. phx_model_derive'd copies of one run) are
+ * marched by one persistent launch over a single job queue holding every pixel of every request
+ * and frequency chunk, so images of uneven size, inclination, frequency, m_unit and R_high share
+ * the GPU.  Requests may differ only in camera, image size, frequencies, m_unit and R_high.
+ * res[i] (res or res[i] may be NULL) receives request i's img_width*img_height*nfreqs results.
+ * The batched form of device::render (src/device/kernel.cu:284-360), which marches one image. */
+int phx_snapshot_render_batch(phx_snapshot* snap, const phx_model* const* requests, int32_t n, int32_t precision,
+                              phx_freqout* const* res, phx_info* info);
+
+/* launch configuration of batch renders */
+typedef struct phx_tune {
+  int32_t queue;         /* -1 default, 0 samples processed in place, 1 per-wave LDS sample queue */
+  int32_t blocks_per_cu; /* 0 = occupancy limit */
+  int32_t order;         /* 0 request after request, 1 requests interleaved one wave (64 jobs) at a time */
+  int32_t fp64_wave;     /* PHX_MIXED: 1 = a lane's FP64 step takes its whole wave to FP64 */
+} phx_tune;
+/* Autotuner: times every configuration on every sample_stride-th wave of the batch's jobs,
+ * keeps the fastest for the snapshot's batch renders, returns it and its sampled rays*freq/s.
+ * Replaces the reference's compile-time GPU_BLOCK / GPU_MINBLOCKS (kernel.cu:37-42). */
+int phx_autotune(phx_snapshot* snap, const phx_model* const* requests, int32_t n, int32_t precision,
+                 int32_t sample_stride, phx_tune* best, double* rate);
+int phx_snapshot_set_tune(phx_snapshot* snap, const phx_tune* tune);
+
+/* ---------------------------------------------------------------- serving loop
+ * No counterpart in the reference (one image per run, src/main.cpp:143-230).  A worker thread
+ * serves render requests on resident snapshots with continuous batching: requests are admitted
+ * in arrival order while their result buffers fit hbm_budget; every launch marches the next
+ * quantum_jobs jobs (pixels x frequency chunks) across all admitted requests of a snapshot, so
+ * requests arriving while others are on the GPU join the next launch; snapshots with work run
+ * on their own streams.  Latency = submit -> result copied to the caller's buffer.
+ * The requests of one snapshot must share the launch-wide parameters of a batch render. */
+typedef struct phx_server phx_server;
+typedef struct phx_server_opts {
+  int32_t precision;    /* PHX_FP32 (default), PHX_FP64 or PHX_MIXED */
+  int32_t quantum_jobs; /* jobs per launch; 0 = 262144 */
+  int64_t hbm_budget;   /* bytes of admitted result buffers; 0 = 8 GiB */
+  int32_t max_active;   /* requests in flight at most; 0 = no limit but the budget */
+  int32_t pad;
+} phx_server_opts;
+typedef struct phx_serve_stats {
+  int64_t submitted, completed, failed, active, waiting;
+  int64_t rays;     /* rays x frequencies of completed requests */
+  int64_t launches; /* march launches */
+  int64_t peak_bytes;
+  double busy_s;    /* summed launch time (HIP events; overlapping snapshots counted once per tick) */
+  double wall_s;    /* since start or the last reset */
+  double p50_s, p99_s, mean_s, max_s; /* latency of completed requests */
+} phx_serve_stats;
+int phx_server_start(phx_snapshot* const* snaps, int32_t nsnaps, const phx_server_opts* opts, phx_server** out);
+/* queues a request on snapshot `snap`; res (caller-owned, img_width*img_height*nfreqs entries,
+ * may be NULL) receives the result when the request completes */
+int phx_server_submit(phx_server* srv, int32_t snap, const phx_model* request, phx_freqout* res, int64_t* ticket);
+/* waits up to timeout_s (< 0: forever) for a ticket; *done = 1 once its result is in res (the
+ * ticket is then retired) */
+int phx_server_wait(phx_server* srv, int64_t ticket, double timeout_s, int32_t* done, double* latency_s);
+int phx_server_stats(phx_server* srv, phx_serve_stats* stats);
+int phx_server_reset_stats(phx_server* srv);
+/* stops the worker after the launch in progress; requests not completed are dropped */
+int phx_server_stop(phx_server* srv);
 
 /* decoded primitives of a snapshot, [nleafs][cells][8] as double (for checks) */
 int phx_snapshot_prims(phx_snapshot* snap, int32_t table, double* out, int64_t n);

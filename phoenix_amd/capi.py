@@ -32,7 +32,7 @@ class Info(ctypes.Structure):
                 ("vgprs", c_int), ("sgprs", c_int), ("scratch_bytes", c_int), ("lds_bytes", c_int),
                 ("waves_per_cu", c_int), ("upload_s", c_double), ("kernel_s", c_double),
                 ("decode_s", c_double), ("steps", c_int64), ("nplasma", c_int64), ("bad", c_int64),
-                ("rays", c_int64)]
+                ("rays", c_int64), ("fp64_steps", c_int64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -74,6 +74,28 @@ class Config(ctypes.Structure):
                 ("show_header", c_int)]
 
 
+class Tune(ctypes.Structure):
+    _fields_ = [("queue", c_int), ("blocks_per_cu", c_int), ("order", c_int), ("fp64_wave", c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("precision", c_int), ("quantum_jobs", c_int), ("hbm_budget", c_int64), ("max_active", c_int),
+                ("pad", c_int)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("submitted", c_int64), ("completed", c_int64), ("failed", c_int64), ("active", c_int64),
+                ("waiting", c_int64), ("rays", c_int64), ("launches", c_int64), ("peak_bytes", c_int64),
+                ("busy_s", c_double), ("wall_s", c_double), ("p50_s", c_double), ("p99_s", c_double),
+                ("mean_s", c_double), ("max_s", c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class PhxError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"phx error {code}: {msg}")
@@ -103,6 +125,19 @@ SIGNATURES = {
     "phx_snapshot_bytes": (c_int64, [ctypes.c_void_p]),
     "phx_snapshot_render": (c_int, [ctypes.c_void_p, ctypes.POINTER(Model), c_int, P_int, c_int,
                                     ctypes.POINTER(FreqOut), ctypes.POINTER(Info)]),
+    "phx_snapshot_render_batch": (c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.POINTER(Model)), c_int, c_int,
+                                          ctypes.POINTER(ctypes.POINTER(FreqOut)), ctypes.POINTER(Info)]),
+    "phx_autotune": (c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.POINTER(Model)), c_int, c_int, c_int,
+                             ctypes.POINTER(Tune), ctypes.POINTER(c_double)]),
+    "phx_snapshot_set_tune": (c_int, [ctypes.c_void_p, ctypes.POINTER(Tune)]),
+    "phx_server_start": (c_int, [ctypes.POINTER(ctypes.c_void_p), c_int, ctypes.POINTER(ServerOpts),
+                                 ctypes.POINTER(ctypes.c_void_p)]),
+    "phx_server_submit": (c_int, [ctypes.c_void_p, c_int, ctypes.POINTER(Model), ctypes.POINTER(FreqOut),
+                                  ctypes.POINTER(c_int64)]),
+    "phx_server_wait": (c_int, [ctypes.c_void_p, c_int64, c_double, P_int, ctypes.POINTER(c_double)]),
+    "phx_server_stats": (c_int, [ctypes.c_void_p, ctypes.POINTER(ServeStats)]),
+    "phx_server_reset_stats": (c_int, [ctypes.c_void_p]),
+    "phx_server_stop": (c_int, [ctypes.c_void_p]),
     "phx_snapshot_result": (c_int, [ctypes.c_void_p, ctypes.POINTER(FreqOut), c_int64]),
     "phx_snapshot_prims": (c_int, [ctypes.c_void_p, c_int, P_dbl, c_int64]),
 }
@@ -216,6 +251,7 @@ class Snapshot:
         self.run = run
         self.h = ctypes.c_void_p()
         check(lib().phx_snapshot_open(ctypes.byref(run.model), run.config.dump_file, tables, ctypes.byref(self.h)))
+        self._tune = {"queue": -1, "blocks_per_cu": 0, "order": 0, "fp64_wave": 1}
 
     def close(self):
         if self.h:
@@ -232,7 +268,17 @@ class Snapshot:
     def bytes(self) -> int:
         return int(lib().phx_snapshot_bytes(self.h))
 
-    def render(self, request: Model | None = None, precision: int = FP32, jobs=None, to_host: bool = True):
+    def render(self, request: Model | None = None, precision: int = FP32, jobs=None, to_host: bool = True,
+               plain: bool = False):
+        """One render on the resident snapshot.  plain: samples processed in place by each lane (the
+        reference kernel's way) instead of through the per-wave sample queue."""
+        if plain:
+            saved = dict(self._tune)
+            self.set_tune(**dict(saved, queue=0))
+            try:
+                return self.render(request, precision, jobs, to_host)
+            finally:
+                self.set_tune(**saved)
         m = request or self.run.model
         n = m.img_width * m.img_height * m.nfreqs
         jp, nj = None, 0
@@ -249,9 +295,94 @@ class Snapshot:
             res = res.reshape(m.img_width * m.img_height, m.nfreqs, 6)
         return res, info.as_dict()
 
+    def render_batch(self, requests, precision: int = FP32, to_host: bool = True):
+        """phx_snapshot_render_batch: one launch over every pixel of every request.
+        Returns ([res_i [W*H, nf, 6] or None], info)."""
+        n = len(requests)
+        rq = (ctypes.POINTER(Model) * n)(*[ctypes.pointer(m) for m in requests])
+        outs, ptrs = [], (ctypes.POINTER(FreqOut) * n)()
+        for i, m in enumerate(requests):
+            if to_host:
+                arr, rp = _res_array(m.img_width * m.img_height * m.nfreqs)
+                outs.append(arr)
+                ptrs[i] = rp
+            else:
+                outs.append(None)
+        info = Info()
+        check(lib().phx_snapshot_render_batch(self.h, rq, n, precision, ptrs if to_host else None,
+                                              ctypes.byref(info)))
+        res = [None if a is None else a.reshape(m.img_width * m.img_height, m.nfreqs, 6)
+               for a, m in zip(outs, requests)]
+        return res, info.as_dict()
+
+    def autotune(self, requests, precision: int = FP32, sample_stride: int = 8):
+        """phx_autotune: returns (tune dict, sampled rays*freq/s); the choice stays with the snapshot."""
+        n = len(requests)
+        rq = (ctypes.POINTER(Model) * n)(*[ctypes.pointer(m) for m in requests])
+        t, rate = Tune(), c_double()
+        check(lib().phx_autotune(self.h, rq, n, precision, sample_stride, ctypes.byref(t), ctypes.byref(rate)))
+        self._tune = t.as_dict()
+        return t.as_dict(), rate.value
+
+    def set_tune(self, queue: int = -1, blocks_per_cu: int = 0, order: int = 0, fp64_wave: int = 1):
+        t = Tune(queue, blocks_per_cu, order, fp64_wave)
+        check(lib().phx_snapshot_set_tune(self.h, ctypes.byref(t)))
+        self._tune = t.as_dict()
+
     def prims(self, table: int) -> np.ndarray:
         m = self.run.model
         n = m.nleafs * m.cells * 8
         out = np.zeros(n)
         check(lib().phx_snapshot_prims(self.h, table, out.ctypes.data_as(P_dbl), n))
         return out.reshape(m.nleafs, m.cells, 8)
+
+
+class Server:
+    """phx_server_*: continuous-batching render service over resident snapshots."""
+
+    def __init__(self, snapshots, precision: int = FP32, quantum_jobs: int = 0, hbm_budget: int = 0,
+                 max_active: int = 0):
+        arr = (ctypes.c_void_p * len(snapshots))(*[sn.h.value for sn in snapshots])
+        opts = ServerOpts(precision, quantum_jobs, hbm_budget, max_active, 0)
+        self.h = ctypes.c_void_p()
+        check(lib().phx_server_start(arr, len(snapshots), ctypes.byref(opts), ctypes.byref(self.h)))
+        self._keep = {}
+
+    def submit(self, snap: int, request: Model) -> int:
+        """Queues a request; returns its ticket.  The result array stays alive until wait()."""
+        n = request.img_width * request.img_height * request.nfreqs
+        arr, rp = _res_array(n)
+        req = Model()
+        ctypes.pointer(req)[0] = request
+        t = c_int64()
+        check(lib().phx_server_submit(self.h, snap, ctypes.byref(req), rp, ctypes.byref(t)))
+        self._keep[t.value] = (arr, req)
+        return t.value
+
+    def wait(self, ticket: int, timeout: float = -1.0):
+        """(result [W*H, nf, 6], latency_s) once done, or (None, 0.0) on timeout."""
+        done, lat = c_int(), c_double()
+        check(lib().phx_server_wait(self.h, ticket, timeout, ctypes.byref(done), ctypes.byref(lat)))
+        if not done.value:
+            return None, 0.0
+        arr, req = self._keep.pop(ticket)
+        return arr.reshape(req.img_width * req.img_height, req.nfreqs, 6), lat.value
+
+    def stats(self) -> dict:
+        st = ServeStats()
+        check(lib().phx_server_stats(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def reset_stats(self):
+        check(lib().phx_server_reset_stats(self.h))
+
+    def stop(self):
+        if self.h:
+            check(lib().phx_server_stop(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass

@@ -9,7 +9,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <condition_variable>
+#include <deque>
+#include <unordered_map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -32,24 +36,151 @@ namespace phx {
 constexpr int BLOCK = 256;  // 4 wave64 per workgroup
 
 // ============================================================================ kernels
-// per-ray start record: hi[12] (x, k, F1), lo[8] (FP32 compensation of x, k), sign of beta, padding
-constexpr int RAY_REC = 24;
+// Precision of a march (the reference's PHX_ACCURACY, CMakeLists.txt:16-22, as a template):
+//   M32  float cells and arithmetic, compensated (hi, lo) FP32 ray state (the FP32 build);
+//   M64  double throughout (the FP64 build);
+//   MIX  float cells and bulk arithmetic, FP64 ray state, FP64 RK4 steps near the horizon and
+//        the poles, FP64 location of a sample's cell, Kahan-compensated Stokes sums.
+constexpr int M32 = 0, M64 = 1, MIX = 2;
+template <int M> struct ModeOf;
+template <> struct ModeOf<M32> { using T = float; using L = float; };
+template <> struct ModeOf<M64> { using T = double; using L = double; };
+template <> struct ModeOf<MIX> { using T = float; using L = double; };
 
-template <class T, int NF>
+constexpr int CHUNK = 4;  // frequencies per task (per lane)
+
+// A task is one image request times one chunk of at most CHUNK of its frequencies; a job is
+// (task, pixel).  One launch marches the jobs of any number of tasks of one snapshot.
+template <class T>
+struct TaskDev {
+  T Ne, Bu, rhigh, pad;                          // plasma units of the request (bhac_io.hpp:230-237)
+  T freq[CHUNK], hconst[CHUNK], jscale[CHUNK];   // per frequency (bhac_io.hpp:244-249)
+  phx_freqout* out;                              // result of (pixel 0, first frequency of the chunk)
+  int nf, nf_total, stats, pad1;                 // frequencies in the chunk / in the request; count steps
+};
+
+template <int M>
 struct MarchArgs {
-  Grid<T> G;
+  using T = typename ModeOf<M>::T;
+  using L = typename ModeOf<M>::L;
+  Grid<T> G;    // cell table and the per-step lookup table
+  Grid<L> GL;   // the block table a sample's cell is located in (FP64 for MIX)
   Plasma<T> P;
-  const T* start;  // [njobs][RAY_REC], in job order
+  const void* start;  // per-job start records, in job order
   T stepsize, r_out, cut_in;
+  double a64, hs64, lnr_fp64, x2_pole;  // MIX: FP64 steps where ln r < lnr_fp64 or x2 within x2_pole of a pole
+  int fp64_wave;                        // MIX: a lane's FP64 step takes its whole wave to FP64
   int max_steps, max_order;
-  int nf_total, f0, nf, count_stats;
-  T freq[NF], hconst[NF], jscale[NF];
-  const int* jobs;
+  const TaskDev<T>* tasks;
+  const int2* jobs;
   int j0, j1;
   int* counter;
-  phx_freqout* res;
   unsigned long long* tot;
 };
+
+// ---- ray state of a lane.  StRef: the reference's state (compensated FP32 pair, or FP64);
+// StMix: FP64 state stepped by rk4_mixed, or by the FP64 RK4 near the horizon and the poles.
+template <class T>
+struct StRef {
+  static constexpr int REC = 24;  // start record: hi[12], lo[8] (FP32 compensation of x, k), sign of beta
+  Ray<T> g;
+  PHX_DEV void load(const T* st, T* bsign) {
+#pragma unroll
+    for (int q = 0; q < 12; q++) g.hi[q] = st[q];
+#pragma unroll
+    for (int q = 0; q < 8; q++) g.lo[q] = Lim<T>::fp32 ? st[12 + q] : T(0);
+#pragma unroll
+    for (int q = 8; q < 12; q++) g.lo[q] = T(0);
+    *bsign = st[20];
+  }
+  PHX_DEV void view(T cx[4], T ck[4], T cf[4]) const {
+    constexpr bool F32 = Lim<T>::fp32;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      cx[q] = F32 ? g.hi[q] + g.lo[q] : g.hi[q];
+      ck[q] = F32 ? g.hi[4 + q] + g.lo[4 + q] : g.hi[4 + q];
+      cf[q] = F32 ? g.hi[8 + q] + g.lo[8 + q] : g.hi[8 + q];
+    }
+  }
+  PHX_DEV void lpos(const T cx[4], T pl[4]) const {
+#pragma unroll
+    for (int q = 0; q < 4; q++) pl[q] = cx[q];
+  }
+  PHX_DEV T x2p() const { return pole_x2(g); }
+  template <class A>
+  PHX_DEV bool step(const A& args, T dl) {
+    rk4(args.P.a, args.P.hs, g, dl);
+    return false;
+  }
+};
+// The FP64 RK4 step of the mixed march: classical RK4 on the FP64 state (the same arithmetic
+// as the FP64 march's rk4), written so that only the state, the stage sum, the stage point
+// and the stage derivative are live (the next stage point is formed as soon as a stage's
+// derivative is known), with one library sincos per stage.  Out of line: it runs on a minority
+// of steps, and inlined its FP64 temporaries push the FP32 bulk of the kernel into scratch.
+__device__ __noinline__ void rk4_fp64_lean(double a, double hs, double X[12], double dt) {
+  double y[12], sum[12];
+#pragma unroll
+  for (int i = 0; i < 12; i++) y[i] = X[i];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    double d[12];
+    const Geo<double> G = geo_at<double, true>(a, hs, y[1], y[2]);
+#pragma unroll
+    for (int i = 0; i < 4; i++) d[i] = y[4 + i];
+    chris(G, y + 4, y + 4, d + 4);
+    chris(G, y + 4, y + 8, d + 8);
+    const double wsum = (q == 0 || q == 3) ? 1. : 2.;
+#pragma unroll
+    for (int i = 0; i < 12; i++) sum[i] = q == 0 ? d[i] : sum[i] + wsum * d[i];
+    if (q == 3) break;
+    const double wdt = (q == 2 ? 1.0 : 0.5) * dt;
+#pragma unroll
+    for (int i = 0; i < 12; i++) y[i] = X[i] + wdt * d[i];
+  }
+  const double dt6 = dt / 6.;
+#pragma unroll
+  for (int i = 0; i < 12; i++) X[i] += sum[i] * dt6;
+}
+struct StMix {
+  static constexpr int REC = 16;  // start record (doubles): x, k, F1, sign of beta
+  double X[12];
+  PHX_DEV void load(const double* st, float* bsign) {
+#pragma unroll
+    for (int q = 0; q < 12; q++) X[q] = st[q];
+    *bsign = float(st[12]);
+  }
+  PHX_DEV void view(float cx[4], float ck[4], float cf[4]) const {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      cx[q] = float(X[q]);
+      ck[q] = float(X[4 + q]);
+      cf[q] = float(X[8 + q]);
+    }
+  }
+  PHX_DEV void lpos(const float*, double pl[4]) const {
+#pragma unroll
+    for (int q = 0; q < 4; q++) pl[q] = X[q];
+  }
+  PHX_DEV float x2p() const { return float(X[2] > 0.5 * PI_D ? PI_D - X[2] : X[2]); }
+  // returns whether the step ran in FP64; WAVE: one lane's FP64 step takes its whole wave to
+  // FP64 (the wave runs one step code instead of both)
+  template <class A>
+  PHX_DEV bool step(const A& args, float dl) {
+    bool fp64 = X[1] < args.lnr_fp64 || X[2] < args.x2_pole || X[2] > PI_D - args.x2_pole;
+    if (args.fp64_wave) fp64 = __ballot(fp64) != 0ull;
+    if (fp64) {
+      rk4_fp64_lean(args.a64, args.hs64, X, double(dl));
+    } else {
+      rk4_mixed(args.P.a, args.P.hs, X, dl);
+    }
+    return fp64;
+  }
+};
+template <int M>
+using StateOf = typename std::conditional<M == MIX, StMix, StRef<typename ModeOf<M>::T>>::type;
+template <int M>
+using RecOf = typename std::conditional<M == M32, float, double>::type;  // start record scalar
 
 // ---- one plasma sample, split into its frequency-independent part and the per-frequency
 // segment operator (src/core/render/pixel.hpp:52-123)
@@ -59,9 +190,12 @@ struct Prep {
 };
 // 0: no emission; 1: emits; 2: emits, but the rotation or the frequency ratio is NaN so every
 // frequency's operator will be non-finite and dropped
-template <class T, int NF>
-PHX_DEV int sample_prepare(const MarchArgs<T, NF>& A, const T px[4], const T pk[4], const T pf[4], T dl, int leaf,
-                           Prep<T>& sp) {
+template <int M>
+PHX_DEV int sample_prepare(const MarchArgs<M>& A, const TaskDev<typename ModeOf<M>::T>& tk,
+                           const typename ModeOf<M>::T px[4], const typename ModeOf<M>::L pl[4],
+                           const typename ModeOf<M>::T pk[4], const typename ModeOf<M>::T pf[4],
+                           typename ModeOf<M>::T dl, int leaf, Prep<typename ModeOf<M>::T>& sp) {
+  using T = typename ModeOf<M>::T;
   T k[4] = {pk[0], pk[1], pk[2], pk[3]};
   const T rs = exp(px[1]);
   const Met<T> gs = metric(A.P.a, A.P.hs, px[1], px[2]);
@@ -73,7 +207,7 @@ PHX_DEV int sample_prepare(const MarchArgs<T, NF>& A, const T px[4], const T pk[
     k[0] = bc + sqrt(bc * bc + gz);
   }
   Fluid<T> fl;
-  if (!sample_fluid(A.G, A.P, px, leaf, fl)) return 0;
+  if (!sample_fluid(A.G, A.GL, A.P, tk.Ne, tk.Bu, tk.rhigh, pl, leaf, fl)) return 0;
   const T pa = pitch_angle(fl.g, k, fl.b, fl.U);
   if (pa == T(0) || pa == T(PI_D)) return 0;
   T perp[4], para[4];
@@ -93,12 +227,12 @@ PHX_DEV int sample_prepare(const MarchArgs<T, NF>& A, const T px[4], const T pk[
   return 1;
 }
 // segment operator of frequency f; false when it is not finite (the reference drops it)
-template <class T, int NF>
-PHX_DEV bool sample_op(const MarchArgs<T, NF>& A, const Prep<T>& sp, int f, T OA[16], T OS[4], T* dtau, T* dtauF) {
-  const T nup = A.freq[f] * sp.ratio;
+template <class T>
+PHX_DEV bool sample_op(const TaskDev<T>& tk, const Prep<T>& sp, int f, T OA[16], T OS[4], T* dtau, T* dtauF) {
+  const T nup = tk.freq[f] * sp.ratio;
   const Coef<T> seg = invariant_rotated(thermal(sp.te, sp.ne, nup, sp.B, sp.sth, sp.cth, sp.kr1, sp.kr0), nup,
-                                        A.jscale[f], sp.c2, sp.s2);
-  const T h = sp.dl * A.hconst[f];
+                                        tk.jscale[f], sp.c2, sp.s2);
+  const T h = sp.dl * tk.hconst[f];
   segment_op(seg, h, OA, OS);
   bool fin = true;
 #pragma unroll
@@ -109,9 +243,10 @@ PHX_DEV bool sample_op(const MarchArgs<T, NF>& A, const Prep<T>& sp, int f, T OA
   *dtauF = fabs(seg.rV * h);
   return fin;
 }
-// S += A O_S; A <- A O_A (operators.hpp:74-86), one row at a time in place
-template <class T>
-PHX_DEV void accumulate(T AA[16], T SS[4], const T OA[16], const T OS[4]) {
+// S += A O_S; A <- A O_A (operators.hpp:74-86), one row at a time in place.  KAHAN: the
+// Stokes sum carries a compensation term per component (the mixed-precision march).
+template <class T, bool KAHAN>
+PHX_DEV void accumulate(T AA[16], T SS[4], T SC[4], const T OA[16], const T OS[4]) {
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     T row[4];
@@ -124,226 +259,38 @@ PHX_DEV void accumulate(T AA[16], T SS[4], const T OA[16], const T OS[4]) {
       for (int kq = 0; kq < 4; kq++) t += AA[4 * i + kq] * OA[4 * kq + j];
       row[j] = t;
     }
-    SS[i] += s;
+    if (KAHAN) {
+      const T y = s - SC[i];
+      const T t = SS[i] + y;
+      SC[i] = (t - SS[i]) - y;
+      SS[i] = t;
+    } else {
+      SS[i] += s;
+    }
 #pragma unroll
     for (int j = 0; j < 4; j++) AA[4 * i + j] = row[j];
   }
 }
 
-// Persistent march: each lane owns one ray at a time; a lane whose ray ends writes it
-// out and takes the next job (wave-aggregated atomic on the job counter), so rays that
-// hit the horizon or escape free their lane at once instead of idling until the wave's
+// ---------------------------------------------------------------------------- the march
+// Persistent lane-refill march: each lane owns one ray at a time; a lane whose ray ends
+// writes it out and takes the next job (wave-aggregated atomic on the job counter), so rays
+// that hit the horizon or escape free their lane at once instead of idling until the wave's
 // longest ray ends.  One pass of the loop is one pass of render_pixel's loop
-// (src/core/render/pixel.hpp:179-211): view, process the pending sample with the
-// current position's leaf as its cache, turning points, step size, RK4.
-// FP32: two waves per SIMD (a 256-VGPR budget); FP64: one wave, its state does not fit in 256.
-template <class T, int NF>
-__global__ __launch_bounds__(BLOCK, Lim<T>::fp32 ? 2 : 1) void k_march(const MarchArgs<T, NF> A) {
-  constexpr bool F32 = Lim<T>::fp32;
-  const int lane = threadIdx.x & 63;
-  Ray<T> g;
-  T px[4] = {T(0), T(0), T(0), T(0)}, pk[4], pf[4], pdl = T(0);
-  T AA[NF][16], SS[NF][4], tau[NF], tauF[NF];
-  int steps = 0, turns = 0, stop = 0, np = 0, bad = 0, idx = 0;
-  T thdot = T(0), dlp = T(0);
-  bool bneg = false, bpos = false, have = false;
-  unsigned s_steps = 0, s_np = 0, s_bad = 0;  // per-lane sums (a lane marches well under 2^32 steps)
-
-  for (;;) {
-    if (!have) {
-      const unsigned long long m = __ballot(1);
-      const int leader = __ffsll((long long)m) - 1;
-      int base = 0;
-      if (lane == leader) base = atomicAdd(A.counter, (int)__popcll(m));
-      base = __shfl(base, leader);
-      const int n = A.j0 + base + (int)__popcll(m & ((1ull << lane) - 1ull));
-      if (n >= A.j1) break;
-      idx = A.jobs[n];
-      // the ray's starting state, formed by k_rays (camera photon + polarization basis)
-      const T* st = A.start + (size_t)n * RAY_REC;
-#pragma unroll
-      for (int q = 0; q < 12; q++) g.hi[q] = st[q];
-#pragma unroll
-      for (int q = 0; q < 8; q++) g.lo[q] = F32 ? st[12 + q] : T(0);
-#pragma unroll
-      for (int q = 8; q < 12; q++) g.lo[q] = T(0);
-      const T bsign = st[20];
-#pragma unroll
-      for (int f = 0; f < NF; f++) {
-#pragma unroll
-        for (int q = 0; q < 16; q++) AA[f][q] = (q % 5 == 0) ? T(1) : T(0);
-#pragma unroll
-        for (int q = 0; q < 4; q++) SS[f][q] = T(0);
-        tau[f] = tauF[f] = T(0);
-      }
-      steps = turns = stop = np = bad = 0;
-      thdot = dlp = T(0);
-      bneg = bsign < T(0);
-      bpos = bsign > T(0);
-      have = true;
-    }
-
-    // ---- view of the current state
-    T cx[4], ck[4], cf[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      cx[q] = F32 ? g.hi[q] + g.lo[q] : g.hi[q];
-      ck[q] = F32 ? g.hi[4 + q] + g.lo[4 + q] : g.hi[4 + q];
-      cf[q] = F32 ? g.hi[8 + q] + g.lo[8 + q] : g.hi[8 + q];
-    }
-    const T r = exp(cx[1]);
-    const bool alive = r < A.r_out && r > A.cut_in && steps < A.max_steps && !stop;
-    const T rs = exp(px[1]);  // radius of the pending sample
-
-    // ---- one AMR lookup serves the pending sample's cache (leaf_of) and the step's cell cap (grid_dxc)
-    int leaf_next = -1;
-    bool have_dxc = false;
-    T dxc[3] = {T(0), T(0), T(0)};
-    if (alive) {
-      T wx[4] = {cx[0], cx[1], cx[2], cx[3]};
-      wrap(wx);
-      const bool need_dxc = r < A.P.rt_outer;
-      // the cache leaf only matters to a pending sample that can emit (r < rt_outer)
-      const bool need_leaf = steps >= 2 && rs < A.P.rt_outer && !off_domain(A.G, wx);
-      if (need_dxc || need_leaf) {
-        const int L = find_leaf(A.G, wx);
-        if (need_leaf) leaf_next = L;
-        if (need_dxc && L >= 0) {
-          have_dxc = true;
-#pragma unroll
-          for (int d = 0; d < 3; d++) dxc[d] = A.G.blk[12 * L + 6 + d];
-        }
-      }
-    }
-
-    // ---- the pending sample (src/core/render/pixel.hpp:52-123).  A sample at r >= rt_outer
-    // cannot emit (pixel.hpp:80) and changes no ray state, so nothing of it is computed.
-    // the pending sample is the state before the last step, i.e. sample number steps - 1; the
-    // camera sample (number 0) is never processed (pixel.hpp:186)
-    if (steps >= 2 && rs < A.P.rt_outer) {
-      Prep<T> sp;
-      const int status = sample_prepare(A, px, pk, pf, pdl, leaf_next, sp);
-      if (status) np++;
-      if (status == 2) {
-        // a NaN rotation or frequency ratio makes every frequency's operator non-finite: the
-        // reference drops each of them (pixel.hpp:107-116); count them without computing them
-        bad += A.nf;
-      } else if (status == 1) {
-#pragma unroll
-        for (int f = 0; f < NF; f++) {
-          if (f >= A.nf) break;
-          T OA[16], OS[4], dtau, dtauF;
-          if (!sample_op(A, sp, f, OA, OS, &dtau, &dtauF)) {
-            bad++;
-            continue;
-          }
-          accumulate(AA[f], SS[f], OA, OS);
-          tau[f] += dtau;
-          tauF[f] += dtauF;
-        }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      px[q] = cx[q];
-      pk[q] = ck[q];
-      pf[q] = cf[q];
-    }
-    pdl = dlp;
-
-    if (!alive) {
-      // ---- ray done: write it out and free the lane (pixel.hpp:213-223)
-#pragma unroll
-      for (int f = 0; f < NF; f++) {
-        if (f >= A.nf) break;
-        double2* o = reinterpret_cast<double2*>(A.res + ((size_t)idx * A.nf_total + A.f0 + f));
-        if (np > 0) {
-          o[0] = make_double2((double)SS[f][0], (double)SS[f][1]);
-          o[1] = make_double2((double)SS[f][2], (double)SS[f][3]);
-          o[2] = make_double2((double)tau[f], (double)tauF[f]);
-        } else {
-          o[0] = o[1] = o[2] = make_double2(0., 0.);
-        }
-      }
-      s_steps += steps;
-      s_np += np;
-      s_bad += bad;
-      have = false;
-      continue;
-    }
-
-    // ---- turning points of theta (pixel.hpp:193-197)
-    if (thdot * ck[2] < T(0) && steps > 2) turns++;
-    thdot = ck[2];
-    if ((bneg && turns > A.max_order) || (bpos && turns > A.max_order + 1)) stop = 1;
-
-    // ---- grid-aware step and RK4 (pixel.hpp:200-210)
-    const T dl = step_len(A.stepsize, pole_x2(g), ck, have_dxc, dxc);
-    rk4(A.P.a, A.P.hs, g, dl);
-    dlp = fabs(dl);
-    steps++;
-  }
-
-  // wave reduction, then one atomic per wave; steps and segments are counted by the first
-  // frequency chunk only, dropped segments (per frequency) by every chunk
-  unsigned long long w_steps = s_steps, w_np = s_np, w_bad = s_bad;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    w_steps += __shfl_xor(w_steps, off);
-    w_np += __shfl_xor(w_np, off);
-    w_bad += __shfl_xor(w_bad, off);
-  }
-  if (lane == 0) {
-    if (A.count_stats) {
-      atomicAdd(&A.tot[0], w_steps);
-      atomicAdd(&A.tot[1], w_np);
-    }
-    atomicAdd(&A.tot[2], w_bad);
-  }
-}
-
-// Starting state of every job's ray: pixel -> impact parameters with the position-angle
-// rotation (camera.hpp:11-23, jobs.hpp:64-76), the RAPTOR camera photon in FP64
-// (camera.hpp:71-135), the camera polarization basis in the march's precision
-// (tetrad.hpp:111-133), and the compensated (hi, lo) split of the FP64 photon (geodesic.hpp:48-60).
-template <class T>
-__global__ __launch_bounds__(BLOCK) void k_rays(const Cam C, T a, T hs, const int* __restrict__ jobs, int njobs,
-                                                T* __restrict__ out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= njobs) return;
-  const int idx = jobs[n];
-  const int i = idx / C.H, j = idx % C.H;
-  const double a0 = -0.5 * C.fov_x + (i + 0.5) * C.fov_x / C.W;
-  const double b0 = -0.5 * C.fov_y + (j + 0.5) * C.fov_y / C.H;
-  const double alpha = a0 * C.cos_pa + b0 * C.sin_pa;
-  const double beta = -a0 * C.sin_pa + b0 * C.cos_pa;
-  double ph[8];
-  camera_photon(C, alpha, beta, ph);
-  T F1[4];
-  camera_F1<T>(a, hs, ph, F1);
-  T* o = out + (size_t)n * RAY_REC;
-  for (int q = 0; q < 8; q++) {
-    const T hi = T(ph[q]);
-    o[q] = hi;
-    o[12 + q] = Lim<T>::fp32 ? T(ph[q] - (double)hi) : T(0);
-  }
-  for (int q = 0; q < 4; q++) o[8 + q] = F1[q];
-  o[20] = beta < 0. ? T(-1) : beta > 0. ? T(1) : T(0);
-  o[21] = o[22] = o[23] = T(0);
-}
-
-// ---------------------------------------------------------------------------- queued march
-// FP32 march with a per-wave sample queue in LDS.  Geodesic steps run on every lane each
-// pass; a lane whose pending sample can emit pushes it (position, wavevector, polarization
-// vector, step length, cache leaf) into its wave's queue instead of processing it at once.
-// When the queue holds 64 samples, when a lane holds KMAX of them, or when a ray with queued
-// samples ends, all active lanes of the wave compute the queued samples' segment operators
-// together, one sample per lane, and each owning lane then folds its own operators into its
-// accumulator in the order it queued them, which is the reference's order
-// (src/core/render/pixel.hpp:186-188).  The divergent, expensive part of a sample (fluid,
-// tetrad, coefficients, 4x4 inverse) thus runs on full waves.
+// (src/core/render/pixel.hpp:179-211): view, the pending sample with the current position's
+// leaf as its cache, turning points, step size, RK4.
+//
+// QUEUE: a per-wave sample queue in LDS.  A lane whose pending sample can emit pushes it
+// (position, wavevector, polarization vector, step length, cache leaf, task) into its wave's
+// queue instead of processing it at once.  When the queue holds 64 samples, when a lane holds
+// KMAX of them, or when a ray with queued samples ends, all active lanes of the wave compute
+// the queued samples' segment operators together, one sample per lane, and each owning lane
+// then folds its own operators into its accumulator in the order it queued them, which is
+// the reference's order (pixel.hpp:186-188).  The divergent, expensive part of a sample
+// (fluid, tetrad, coefficients, 4x4 inverse) thus runs on full waves.  Without QUEUE the
+// lane processes its sample in place.
 constexpr int QCAP = 128;  // slots per wave (at most 63 + 64 are in use at a flush)
-constexpr int QW = 24;     // scalars per slot: 14 in (x, k, F1, dl, leaf), 23 out (O_A, O_S, dtau, dtauF, flag, emits)
+constexpr int QW = 24;     // scalars per slot: 21 in (x, k, F1, dl, leaf, FP64 position, task), 24 out
 constexpr int KMAX = 4;    // queued samples a lane may hold
 
 PHX_DEV void wave_sync() {
@@ -352,22 +299,32 @@ PHX_DEV void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <class T, int NF>
-__global__ __launch_bounds__(BLOCK, 2) void k_march_q(const MarchArgs<T, NF> A) {
-  __shared__ T qbuf[BLOCK / 64][QCAP * QW];
-  T* const Q = qbuf[threadIdx.x >> 6];
-  constexpr bool F32 = Lim<T>::fp32;
+#ifndef PHX_MIX_WAVES_PER_SIMD
+#define PHX_MIX_WAVES_PER_SIMD 2  // occupancy the mixed march is compiled for (1: 512 VGPRs, 2: 256)
+#endif
+template <int M, int NF, bool QUEUE>
+__global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_SIMD : 2) void k_march(
+    const MarchArgs<M> A) {
+  using T = typename ModeOf<M>::T;
+  using L = typename ModeOf<M>::L;
+  using St = StateOf<M>;
+  using R = RecOf<M>;
+  constexpr bool KAHAN = M == MIX;
+  __shared__ __align__(16) T qbuf[QUEUE ? BLOCK / 64 : 1][QUEUE ? QCAP * QW : 1];
+  T* const Q = qbuf[QUEUE ? (threadIdx.x >> 6) : 0];
   const int lane = threadIdx.x & 63;
   const unsigned long long lt = (1ull << lane) - 1ull;
-  Ray<T> g;
+  St st;
   T px[4] = {T(0), T(0), T(0), T(0)}, pk[4] = {T(0), T(0), T(0), T(0)}, pf[4] = {T(0), T(0), T(0), T(0)};
+  L pl[4] = {L(0), L(0), L(0), L(0)};
   T pdl = T(0);
-  T AA[NF][16], SS[NF][4], tau[NF], tauF[NF];
-  int steps = 0, turns = 0, stop = 0, np = 0, bad = 0, idx = 0;
+  T AA[NF][16], SS[NF][4], SC[NF][4], tau[NF], tauF[NF];
+  int steps = 0, turns = 0, stop = 0, np = 0, bad = 0, pix = 0, task = 0, nf = 0;
   T thdot = T(0), dlp = T(0);
-  bool bneg = false, bpos = false, have = false, waiting = false;
-  unsigned s_steps = 0, s_np = 0, s_bad = 0;
-  int qcount = 0, nmine = 0;  // qcount is the same in every active lane of the wave
+  bool bneg = false, bpos = false, have = false, stats = false;
+  unsigned s_steps = 0, s_np = 0, s_bad = 0;  // per-lane sums (a lane marches well under 2^32 steps)
+  unsigned s_fp64 = 0;                        // MIX: steps taken in FP64
+  int qcount = 0, nmine = 0;                  // qcount is the same in every active lane of the wave
   int myslot[KMAX];
 #pragma unroll
   for (int j = 0; j < KMAX; j++) myslot[j] = 0;
@@ -381,21 +338,19 @@ __global__ __launch_bounds__(BLOCK, 2) void k_march_q(const MarchArgs<T, NF> A) 
       base = __shfl(base, leader);
       const int n = A.j0 + base + (int)__popcll(m & lt);
       if (n >= A.j1) break;
-      idx = A.jobs[n];
-      const T* st = A.start + (size_t)n * RAY_REC;
-#pragma unroll
-      for (int q = 0; q < 12; q++) g.hi[q] = st[q];
-#pragma unroll
-      for (int q = 0; q < 8; q++) g.lo[q] = F32 ? st[12 + q] : T(0);
-#pragma unroll
-      for (int q = 8; q < 12; q++) g.lo[q] = T(0);
-      const T bsign = st[20];
+      const int2 jb = A.jobs[n];
+      task = jb.x;
+      pix = jb.y;
+      nf = A.tasks[task].nf;
+      stats = A.tasks[task].stats != 0;
+      T bsign;
+      st.load(static_cast<const R*>(A.start) + (size_t)n * St::REC, &bsign);
 #pragma unroll
       for (int f = 0; f < NF; f++) {
 #pragma unroll
         for (int q = 0; q < 16; q++) AA[f][q] = (q % 5 == 0) ? T(1) : T(0);
 #pragma unroll
-        for (int q = 0; q < 4; q++) SS[f][q] = T(0);
+        for (int q = 0; q < 4; q++) SS[f][q] = SC[f][q] = T(0);
         tau[f] = tauF[f] = T(0);
       }
       steps = turns = stop = np = bad = 0;
@@ -405,159 +360,197 @@ __global__ __launch_bounds__(BLOCK, 2) void k_march_q(const MarchArgs<T, NF> A) 
       have = true;
     }
 
-    // ---- view, liveness and the one AMR lookup of this pass
+    // ---- view of the current state, liveness, and the one AMR lookup of this pass: it
+    // serves the pending sample's cache (leaf_of) and the step's cell cap (grid_dxc)
     T cx[4], ck[4], cf[4];
-    bool alive = false, push = false, have_dxc = false;
+    st.view(cx, ck, cf);
+    const T r = exp(cx[1]);
+    const bool alive = r < A.r_out && r > A.cut_in && steps < A.max_steps && !stop;
+    const T rs = exp(px[1]);  // radius of the pending sample
+    // the pending sample is the state before the last step, i.e. sample number steps - 1; the
+    // camera sample (number 0) is never processed (pixel.hpp:186).  A sample at r >= rt_outer
+    // cannot emit (pixel.hpp:80) and changes no ray state, so nothing of it is computed.
+    const bool emit = steps >= 2 && rs < A.P.rt_outer;
     int leaf_next = -1;
+    bool have_dxc = false;
     T dxc[3] = {T(0), T(0), T(0)};
-    if (!waiting) {
+    if (alive) {
+      T wx[4] = {cx[0], cx[1], cx[2], cx[3]};
+      wrap(wx);
+      const bool need_dxc = r < A.P.rt_outer;
+      const bool need_leaf = emit && !off_domain(A.G, wx);
+      if (need_dxc || need_leaf) {
+        const int Lf = find_leaf(A.G, wx);
+        if (need_leaf) leaf_next = Lf;
+        if (need_dxc && Lf >= 0) {
+          have_dxc = true;
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        cx[q] = F32 ? g.hi[q] + g.lo[q] : g.hi[q];
-        ck[q] = F32 ? g.hi[4 + q] + g.lo[4 + q] : g.hi[4 + q];
-        cf[q] = F32 ? g.hi[8 + q] + g.lo[8 + q] : g.hi[8 + q];
-      }
-      const T r = exp(cx[1]);
-      alive = r < A.r_out && r > A.cut_in && steps < A.max_steps && !stop;
-      const T rs = exp(px[1]);
-      push = steps >= 2 && rs < A.P.rt_outer;
-      if (alive) {
-        T wx[4] = {cx[0], cx[1], cx[2], cx[3]};
-        wrap(wx);
-        const bool need_dxc = r < A.P.rt_outer;
-        const bool need_leaf = push && !off_domain(A.G, wx);
-        if (need_dxc || need_leaf) {
-          const int L = find_leaf(A.G, wx);
-          if (need_leaf) leaf_next = L;
-          if (need_dxc && L >= 0) {
-            have_dxc = true;
-#pragma unroll
-            for (int d = 0; d < 3; d++) dxc[d] = A.G.blk[12 * L + 6 + d];
-          }
+          for (int d = 0; d < 3; d++) dxc[d] = A.G.blk[12 * Lf + 6 + d];
         }
       }
     }
 
-    // ---- queue the pending sample (every loop-resident lane takes part in the ballot)
-    const unsigned long long pm = __ballot(push);
-    if (push) {
-      const int slot = qcount + (int)__popcll(pm & lt);
-      T* e = Q + slot * QW;
+    if constexpr (QUEUE) {
+      // ---- queue the pending sample (every loop-resident lane takes part in the ballot)
+      const unsigned long long pm = __ballot(emit);
+      if (emit) {
+        const int slot = qcount + (int)__popcll(pm & lt);
+        T* e = Q + slot * QW;
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        e[q] = px[q];
-        e[4 + q] = pk[q];
-        e[8 + q] = pf[q];
-      }
-      e[12] = pdl;
-      e[13] = T(leaf_next);
-#pragma unroll
-      for (int j = 0; j < KMAX; j++)
-        if (j == nmine) myslot[j] = slot;
-      nmine++;
-    }
-    qcount += (int)__popcll(pm);
-
-    if (!waiting) {
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        px[q] = cx[q];
-        pk[q] = ck[q];
-        pf[q] = cf[q];
-      }
-      pdl = dlp;
-      if (!alive) {
-        waiting = true;  // written out once its queued samples are folded in
-      } else {
-        if (thdot * ck[2] < T(0) && steps > 2) turns++;
-        thdot = ck[2];
-        if ((bneg && turns > A.max_order) || (bpos && turns > A.max_order + 1)) stop = 1;
-        const T dl = step_len(A.stepsize, pole_x2(g), ck, have_dxc, dxc);
-        rk4(A.P.a, A.P.hs, g, dl);
-        dlp = fabs(dl);
-        steps++;
-      }
-    }
-
-    // ---- flush: all active lanes compute the queued samples, owners accumulate in order
-    const bool want = qcount >= QCAP - 64 || nmine >= KMAX || (waiting && nmine > 0);
-    if (__ballot(want) != 0ull) {
-      const unsigned long long act = __ballot(1);
-      const int nact = (int)__popcll(act), rank = (int)__popcll(act & lt);
-      for (int base = 0; base < qcount; base += nact) {
-        const int e = base + rank;
-        const bool mine = e < qcount;
-        Prep<T> sp;
-        int status = 0;
-        if (mine) {
-          const T* in = Q + e * QW;
-          T x[4], k[4], f1[4];
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            x[q] = in[q];
-            k[q] = in[4 + q];
-            f1[q] = in[8 + q];
-          }
-          status = sample_prepare(A, x, k, f1, in[12], (int)in[13], sp);
+        for (int q = 0; q < 4; q++) {
+          e[q] = px[q];
+          e[4 + q] = pk[q];
+          e[8 + q] = pf[q];
         }
+        e[12] = pdl;
+        e[13] = T(leaf_next);
+        if constexpr (!std::is_same<T, L>::value) {
+          double* e64 = reinterpret_cast<double*>(e + 14);
+#pragma unroll
+          for (int q = 0; q < 3; q++) e64[q] = pl[1 + q];
+        }
+        e[20] = T(task);
+#pragma unroll
+        for (int j = 0; j < KMAX; j++)
+          if (j == nmine) myslot[j] = slot;
+        nmine++;
+      }
+      qcount += (int)__popcll(pm);
+    } else if (emit) {
+      // ---- the pending sample, in place
+      const TaskDev<T>& tk = A.tasks[task];
+      Prep<T> sp;
+      const int status = sample_prepare<M>(A, tk, px, pl, pk, pf, pdl, leaf_next, sp);
+      if (status) np++;
+      if (status == 2) {
+        // a NaN rotation or frequency ratio makes every frequency's operator non-finite: the
+        // reference drops each of them (pixel.hpp:107-116); count them without computing them
+        bad += nf;
+      } else if (status == 1) {
 #pragma unroll
         for (int f = 0; f < NF; f++) {
-          if (f >= A.nf) break;
-          if (mine) {
-            T* out = Q + e * QW;
-            int flag = 0;  // 1: fold in, 2: dropped as non-finite
-            if (status == 1) {
-              T OA[16], OS[4], dtau, dtauF;
-              flag = sample_op(A, sp, f, OA, OS, &dtau, &dtauF) ? 1 : 2;
-              if (flag == 1) {
-#pragma unroll
-                for (int q = 0; q < 16; q++) out[q] = OA[q];
-#pragma unroll
-                for (int q = 0; q < 4; q++) out[16 + q] = OS[q];
-                out[20] = dtau;
-                out[21] = dtauF;
-              }
-            } else if (status == 2) {
-              flag = 2;
-            }
-            out[22] = T(flag);
-            out[23] = T(status != 0 ? 1 : 0);
+          if (f >= nf) break;
+          T OA[16], OS[4], dtau, dtauF;
+          if (!sample_op(tk, sp, f, OA, OS, &dtau, &dtauF)) {
+            bad++;
+            continue;
           }
-          wave_sync();
-#pragma unroll
-          for (int j = 0; j < KMAX; j++) {
-            const int sl = myslot[j];
-            if (j < nmine && sl >= base && sl < base + nact) {
-              const T* rr = Q + sl * QW;
-              const int flag = (int)rr[22];
-              if (f == 0 && rr[23] != T(0)) np++;
-              if (flag == 1) {
-                T OA[16], OS[4];
-#pragma unroll
-                for (int q = 0; q < 16; q++) OA[q] = rr[q];
-#pragma unroll
-                for (int q = 0; q < 4; q++) OS[q] = rr[16 + q];
-                accumulate(AA[f], SS[f], OA, OS);
-                tau[f] += rr[20];
-                tauF[f] += rr[21];
-              } else if (flag == 2) {
-                bad++;
-              }
-            }
-          }
-          wave_sync();
+          accumulate<T, KAHAN>(AA[f], SS[f], SC[f], OA, OS);
+          tau[f] += dtau;
+          tauF[f] += dtauF;
         }
       }
-      qcount = 0;
-      nmine = 0;
     }
 
-    if (waiting) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      px[q] = cx[q];
+      pk[q] = ck[q];
+      pf[q] = cf[q];
+    }
+    st.lpos(cx, pl);
+    pdl = dlp;
+    if (alive) {
+      // ---- turning points of theta (pixel.hpp:193-197), grid-aware step and RK4 (:200-210)
+      if (thdot * ck[2] < T(0) && steps > 2) turns++;
+      thdot = ck[2];
+      if ((bneg && turns > A.max_order) || (bpos && turns > A.max_order + 1)) stop = 1;
+      const T dl = step_len(A.stepsize, st.x2p(), ck, have_dxc, dxc);
+      if (st.step(A, dl) && stats) s_fp64++;
+      dlp = fabs(dl);
+      steps++;
+    }
+
+    if constexpr (QUEUE) {
+      // ---- flush: all active lanes compute the queued samples, owners accumulate in order
+      const bool want = qcount >= QCAP - 64 || nmine >= KMAX || (!alive && nmine > 0);
+      if (__ballot(want) != 0ull) {
+        const unsigned long long act = __ballot(1);
+        const int nact = (int)__popcll(act), rank = (int)__popcll(act & lt);
+        for (int base = 0; base < qcount; base += nact) {
+          const int e = base + rank;
+          const bool mine = e < qcount;
+          Prep<T> sp;
+          int status = 0, etask = 0;
+          if (mine) {
+            const T* in = Q + e * QW;
+            T x[4], k[4], f1[4];
+            L xl[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              x[q] = in[q];
+              k[q] = in[4 + q];
+              f1[q] = in[8 + q];
+              xl[q] = L(x[q]);
+            }
+            if constexpr (!std::is_same<T, L>::value) {
+              const double* e64 = reinterpret_cast<const double*>(in + 14);
+#pragma unroll
+              for (int q = 0; q < 3; q++) xl[1 + q] = e64[q];
+            }
+            etask = (int)in[20];
+            status = sample_prepare<M>(A, A.tasks[etask], x, xl, k, f1, in[12], (int)in[13], sp);
+          }
+          const int enf = mine ? A.tasks[etask].nf : 0;
+#pragma unroll
+          for (int f = 0; f < NF; f++) {
+            if (mine && f < enf) {
+              T* out = Q + e * QW;
+              int flag = 0;  // 1: fold in, 2: dropped as non-finite
+              if (status == 1) {
+                T OA[16], OS[4], dtau, dtauF;
+                flag = sample_op(A.tasks[etask], sp, f, OA, OS, &dtau, &dtauF) ? 1 : 2;
+                if (flag == 1) {
+#pragma unroll
+                  for (int q = 0; q < 16; q++) out[q] = OA[q];
+#pragma unroll
+                  for (int q = 0; q < 4; q++) out[16 + q] = OS[q];
+                  out[20] = dtau;
+                  out[21] = dtauF;
+                }
+              } else if (status == 2) {
+                flag = 2;
+              }
+              out[22] = T(flag);
+              out[23] = T(status != 0 ? 1 : 0);
+            }
+            wave_sync();
+#pragma unroll
+            for (int j = 0; j < KMAX; j++) {
+              const int sl = myslot[j];
+              if (j < nmine && sl >= base && sl < base + nact && f < nf) {
+                const T* rr = Q + sl * QW;
+                const int flag = (int)rr[22];
+                if (f == 0 && rr[23] != T(0)) np++;
+                if (flag == 1) {
+                  T OA[16], OS[4];
+#pragma unroll
+                  for (int q = 0; q < 16; q++) OA[q] = rr[q];
+#pragma unroll
+                  for (int q = 0; q < 4; q++) OS[q] = rr[16 + q];
+                  accumulate<T, KAHAN>(AA[f], SS[f], SC[f], OA, OS);
+                  tau[f] += rr[20];
+                  tauF[f] += rr[21];
+                } else if (flag == 2) {
+                  bad++;
+                }
+              }
+            }
+            wave_sync();
+          }
+        }
+        qcount = 0;
+        nmine = 0;
+      }
+    }
+
+    if (!alive) {
       // ---- ray done: write it out and free the lane (pixel.hpp:213-223)
+      phx_freqout* o0 = A.tasks[task].out + (size_t)pix * A.tasks[task].nf_total;
 #pragma unroll
       for (int f = 0; f < NF; f++) {
-        if (f >= A.nf) break;
-        double2* o = reinterpret_cast<double2*>(A.res + ((size_t)idx * A.nf_total + A.f0 + f));
+        if (f >= nf) break;
+        double2* o = reinterpret_cast<double2*>(o0 + f);
         if (np > 0) {
           o[0] = make_double2((double)SS[f][0], (double)SS[f][1]);
           o[1] = make_double2((double)SS[f][2], (double)SS[f][3]);
@@ -566,27 +559,81 @@ __global__ __launch_bounds__(BLOCK, 2) void k_march_q(const MarchArgs<T, NF> A) 
           o[0] = o[1] = o[2] = make_double2(0., 0.);
         }
       }
-      s_steps += steps;
-      s_np += np;
+      if (stats) {
+        s_steps += steps;
+        s_np += np;
+      }
       s_bad += bad;
       have = false;
-      waiting = false;
     }
   }
 
-  unsigned long long w_steps = s_steps, w_np = s_np, w_bad = s_bad;
+  // wave reduction, then one atomic per wave; steps and segments are counted by the first
+  // frequency chunk of each request only, dropped segments (per frequency) by every chunk
+  unsigned long long w_steps = s_steps, w_np = s_np, w_bad = s_bad, w_fp64 = s_fp64;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     w_steps += __shfl_xor(w_steps, off);
     w_np += __shfl_xor(w_np, off);
     w_bad += __shfl_xor(w_bad, off);
+    if (M == MIX) w_fp64 += __shfl_xor(w_fp64, off);
   }
   if (lane == 0) {
-    if (A.count_stats) {
-      atomicAdd(&A.tot[0], w_steps);
-      atomicAdd(&A.tot[1], w_np);
-    }
+    atomicAdd(&A.tot[0], w_steps);
+    atomicAdd(&A.tot[1], w_np);
     atomicAdd(&A.tot[2], w_bad);
+    if (M == MIX) atomicAdd(&A.tot[3], w_fp64);
+  }
+}
+
+// Starting state of every job's ray: pixel -> impact parameters with the position-angle
+// rotation (camera.hpp:11-23, jobs.hpp:64-76), the RAPTOR camera photon in FP64
+// (camera.hpp:71-135), the camera polarization basis (tetrad.hpp:111-133: in the march's
+// precision for M32 / M64, in FP64 for MIX), and for M32 the compensated (hi, lo) split of
+// the FP64 photon (geodesic.hpp:48-60).  cams[task] is the task's camera.
+template <int M>
+__global__ __launch_bounds__(BLOCK) void k_rays(const Cam* __restrict__ cams, const int2* __restrict__ jobs,
+                                                int njobs, void* __restrict__ out_) {
+  using T = typename ModeOf<M>::T;
+  using R = RecOf<M>;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= njobs) return;
+  R* const out = static_cast<R*>(out_);
+  const int2 jb = jobs[n];
+  Cam C = cams[jb.x];
+  if constexpr (M == M32) {  // the FP32 build's camera sees the spin and stretching rounded to float (camera.hpp:77)
+    C.a = double(float(C.a));
+    C.hs = double(float(C.hs));
+  }
+  const int idx = jb.y;
+  const int i = idx / C.H, j = idx % C.H;
+  const double a0 = -0.5 * C.fov_x + (i + 0.5) * C.fov_x / C.W;
+  const double b0 = -0.5 * C.fov_y + (j + 0.5) * C.fov_y / C.H;
+  const double alpha = a0 * C.cos_pa + b0 * C.sin_pa;
+  const double beta = -a0 * C.sin_pa + b0 * C.cos_pa;
+  double ph[8];
+  camera_photon(C, alpha, beta, ph);
+  const R bs = beta < 0. ? R(-1) : beta > 0. ? R(1) : R(0);
+  if constexpr (M == MIX) {
+    double F1[4];
+    camera_F1<double>(C.a, C.hs, ph, F1);
+    double* o = out + (size_t)n * StMix::REC;
+    for (int q = 0; q < 8; q++) o[q] = ph[q];
+    for (int q = 0; q < 4; q++) o[8 + q] = F1[q];
+    o[12] = bs;
+    o[13] = o[14] = o[15] = 0.;
+  } else {
+    T F1[4];
+    camera_F1<T>(T(C.a), T(C.hs), ph, F1);
+    R* o = out + (size_t)n * StRef<T>::REC;
+    for (int q = 0; q < 8; q++) {
+      const T hi = T(ph[q]);
+      o[q] = hi;
+      o[12 + q] = Lim<T>::fp32 ? T(ph[q] - (double)hi) : T(0);
+    }
+    for (int q = 0; q < 4; q++) o[8 + q] = F1[q];
+    o[20] = bs;
+    o[21] = o[22] = o[23] = R(0);
   }
 }
 
@@ -664,8 +711,21 @@ static double now_s() {
 
 template <class T>
 struct Tables {
-  T* blk = nullptr;
-  T* prim = nullptr;
+  T* blk = nullptr;   // [nleafs][12] block records
+  T* prim = nullptr;  // [nleafs * cells][8] primitives
+};
+
+// Launch knobs (picked by the autotuner or from the environment)
+struct Tune {
+  int queue = -1;         // -1: per precision default (queue for M32 / MIX), 0: in place, 1: LDS queue
+  int blocks_per_cu = 0;  // 0: the occupancy limit
+  int order = 0;          // batch job order: 0 request after request, 1 requests interleaved per wave
+  int fp64_wave = 1;      // mixed march: 1 = a lane's FP64 step takes its whole wave to FP64
+};
+// grow-only device buffer
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
 };
 
 }  // namespace phx
@@ -677,31 +737,30 @@ struct phx_snapshot {
   phx::Tables<double> t64;
   int* map = nullptr;
   int fd[3] = {0, 0, 0};
-  int64_t bytes = 0;
+  int64_t bytes = 0;  // resident tables
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  phx_freqout* d_res = nullptr;
-  size_t res_cap = 0;
-  int* d_jobs = nullptr;
-  size_t jobs_cap = 0;
-  void* d_start = nullptr;  // per-job ray starting states
-  size_t start_cap = 0;
+  phx::DevBuf res, jobs, pix, start, tasks, cams;  // per-render work buffers (grow only)
   int* d_counter = nullptr;
   unsigned long long* d_tot = nullptr;
   double decode_s = 0., upload_s = 0.;
   int64_t last_nres = 0;
-  int last_chunk_jobs = 0;
-  int morton_w = 0, morton_h = 0;  // image size whose Morton job list d_jobs holds (0: none)
+  int last_jobs = 0;
+  int pix_w = 0, pix_h = 0;  // image whose Morton pixel list `pix` holds (0: none)
   int device = 0;
   std::vector<void*> allocs;
+  phx::Tune tune;  // launch configuration of batch renders (set by the autotuner)
+  std::mutex mu;   // one render at a time per snapshot
 };
 
 namespace phx {
 
 static int g_device = 0;
 static std::thread* g_warm = nullptr;
+static std::mutex g_warm_mu;
 
 static void warm_wait() {
+  std::lock_guard<std::mutex> lk(g_warm_mu);
   if (g_warm) {
     g_warm->join();
     delete g_warm;
@@ -718,13 +777,24 @@ static P* dalloc(phx_snapshot* s, size_t bytes) {
   return (P*)d;
 }
 
+static void* ensure(DevBuf& b, size_t bytes) {
+  bytes = std::max<size_t>(bytes, 256);
+  if (bytes > b.cap) {
+    if (b.p) HIPCHECK(hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+    HIPCHECK(hipMalloc(&b.p, bytes));
+    b.cap = bytes;
+  }
+  return b.p;
+}
+
 static void snapshot_free(phx_snapshot* s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (void* p : s->allocs) (void)hipFree(p);
-  if (s->d_res) (void)hipFree(s->d_res);
-  if (s->d_jobs) (void)hipFree(s->d_jobs);
-  if (s->d_start) (void)hipFree(s->d_start);
+  for (DevBuf* b : {&s->res, &s->jobs, &s->pix, &s->start, &s->tasks, &s->cams})
+    if (b->p) (void)hipFree(b->p);
   if (s->e0) (void)hipEventDestroy(s->e0);
   if (s->e1) (void)hipEventDestroy(s->e1);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -770,24 +840,22 @@ static std::vector<int> flat_map(const phx_model* m, int fd[3]) {
 }
 
 template <class T>
-static void upload_tables(phx_snapshot* s, const phx_model* m, const char* dump, Tables<T>& tb,
-                          const std::vector<double>* raw, int per, int nchunks) {
+static void upload_blocks(phx_snapshot* s, const phx_model* m, Tables<T>& tb) {
   const std::vector<T> rec = block_records<T>(m);
   tb.blk = dalloc<T>(s, rec.size() * sizeof(T));
   HIPCHECK(hipMemcpyAsync(tb.blk, rec.data(), rec.size() * sizeof(T), hipMemcpyHostToDevice, s->stream));
+  HIPCHECK(hipStreamSynchronize(s->stream));
+}
+template <class T>
+static void upload_prim(phx_snapshot* s, const phx_model* m, Tables<T>& tb) {
   const size_t nprim = (size_t)m->nleafs * m->cells * 8;
   tb.prim = dalloc<T>(s, nprim * sizeof(T));
-  if (m->prim) {
+  if (m->prim) {  // host primitives given: convert and upload; otherwise the dump is decoded on the device
     std::vector<T> p(nprim);
     for (size_t i = 0; i < nprim; i++) p[i] = T(m->prim[i]);
     HIPCHECK(hipMemcpyAsync(tb.prim, p.data(), nprim * sizeof(T), hipMemcpyHostToDevice, s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
-    return;
   }
-  (void)dump;
-  (void)raw;
-  (void)per;
-  (void)nchunks;
 }
 
 // Streams the dump's leaf blocks through two pinned buffers and decodes them on the device
@@ -886,8 +954,11 @@ static phx_snapshot* snapshot_open(const phx_model* m, const char* dump, int tab
   const std::vector<int> fm = flat_map(m, s->fd);
   s->map = dalloc<int>(s.get(), fm.size() * sizeof(int));
   HIPCHECK(hipMemcpyAsync(s->map, fm.data(), fm.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
-  if (tables & PHX_TABLE_FP32) upload_tables<float>(s.get(), m, dump, s->t32, nullptr, 0, 0);
-  if (tables & PHX_TABLE_FP64) upload_tables<double>(s.get(), m, dump, s->t64, nullptr, 0, 0);
+  // both block tables are always resident (a few KB): the mixed march locates cells in FP64
+  upload_blocks<float>(s.get(), m, s->t32);
+  upload_blocks<double>(s.get(), m, s->t64);
+  if (tables & PHX_TABLE_FP32) upload_prim<float>(s.get(), m, s->t32);
+  if (tables & PHX_TABLE_FP64) upload_prim<double>(s.get(), m, s->t64);
   HIPCHECK(hipStreamSynchronize(s->stream));
   s->upload_s = now_s() - t0;
   if (!m->prim) decode_dump(s.get(), m, dump);
@@ -921,60 +992,92 @@ static Grid<T> grid_of(const phx_snapshot* s, const Tables<T>& tb) {
   G.hi3 = T(m.x_hi[3]);
   return G;
 }
+template <class T>
+static const Tables<T>& tables_of(const phx_snapshot* s);
+template <>
+const Tables<float>& tables_of<float>(const phx_snapshot* s) {
+  return s->t32;
+}
+template <>
+const Tables<double>& tables_of<double>(const phx_snapshot* s) {
+  return s->t64;
+}
 
-template <class T, int NF>
-static void launch_march(phx_snapshot* s, const Tables<T>& tb, const phx_model* rq, int f0, int nf,
-                         const int* d_jobs, int j0, int j1, int count_stats, phx_info* info) {
-  MarchArgs<T, NF> A;
+// x2 at which |cos theta| = 0.99 on the northern side (theta = x2 + h/2 sin 2 x2), by Newton
+static double pole_x2_limit(double hs) {
+  const double th = std::acos(0.99);
+  double x = th;
+  for (int it = 0; it < 60; it++) {
+    const double f = x + 0.5 * hs * std::sin(2. * x) - th, df = 1. + hs * std::cos(2. * x);
+    const double nx = x - f / df;
+    if (std::fabs(nx - x) < 1e-15) {
+      x = nx;
+      break;
+    }
+    x = nx;
+  }
+  return x;
+}
+
+static Tune tune_from_env(Tune t) {
+  if (const char* e = std::getenv("PHX_MARCH")) {
+    if (!std::strcmp(e, "direct")) t.queue = 0;
+    if (!std::strcmp(e, "queued")) t.queue = 1;
+  }
+  if (const char* e = std::getenv("PHX_BLOCKS_PER_CU"))
+    if (std::atoi(e) > 0) t.blocks_per_cu = std::atoi(e);
+  if (const char* e = std::getenv("PHX_FP64_WAVE")) t.fp64_wave = std::atoi(e) != 0;
+  return t;
+}
+
+// what one march launch needs beyond the device buffers of the snapshot
+struct LaunchInfo {
+  int njobs = 0, ntasks = 0, nfmax = 1;
+  const phx_model* uni = nullptr;  // the request whose launch-wide parameters (stepping, cuts) apply
+};
+
+template <int M, int NF, bool Q>
+static void launch_march(phx_snapshot* s, const LaunchInfo& li, int j0, int j1, const Tune& tn, phx_info* info) {
+  using T = typename ModeOf<M>::T;
+  using Lt = typename ModeOf<M>::L;
+  MarchArgs<M> A;
   std::memset(&A, 0, sizeof A);
-  A.G = grid_of<T>(s, tb);
+  A.G = grid_of<T>(s, tables_of<T>(s));
+  A.GL = grid_of<Lt>(s, tables_of<Lt>(s));
   const phx_model& m = s->geo;
+  const phx_model* rq = li.uni;
   A.P.a = T(m.a);
   A.P.hs = T(m.hslope);
   A.P.gam = T(m.gam);
-  A.P.Ne = T(rq->Ne);
-  A.P.Bu = T(rq->Bu);
-  A.P.rhigh = T(rq->rhigh);
   A.P.rlow = T(rq->rlow);
   A.P.ibc = T(rq->ibc);
   A.P.sigma_cut = T(rq->sigma_cut);
   A.P.te_min = T(rq->thetae_min);
   A.P.te_max = T(rq->thetae_max);
   A.P.rt_outer = T(rq->rt_outer);
-  A.start = (const T*)s->d_start;
+  A.start = s->start.p;
   A.stepsize = T(rq->stepsize);
   A.r_out = T(rq->r_out);
   A.cut_in = T(rq->cut_in);
+  A.a64 = m.a;
+  A.hs64 = m.hslope;
+  A.lnr_fp64 = std::log(2. * (1. + std::sqrt(std::max(0., 1. - m.a * m.a))));  // r < 2 r_H
+  A.x2_pole = pole_x2_limit(m.hslope);                                          // |cos theta| > 0.99
+  A.fp64_wave = tn.fp64_wave;
   A.max_steps = rq->max_steps;
   A.max_order = rq->max_order;
-  A.nf_total = rq->nfreqs;
-  A.f0 = f0;
-  A.nf = nf;
-  A.count_stats = count_stats;
-  for (int f = 0; f < nf; f++) {
-    A.freq[f] = T(rq->freq[f0 + f]);
-    A.hconst[f] = T(rq->hconst[f0 + f]);
-    A.jscale[f] = T(rq->jscale[f0 + f]);
-  }
-  A.jobs = d_jobs;
+  A.tasks = (const TaskDev<T>*)s->tasks.p;
+  A.jobs = (const int2*)s->jobs.p;
   A.j0 = j0;
   A.j1 = j1;
   A.counter = s->d_counter;
-  A.res = s->d_res;
   A.tot = s->d_tot;
-  // FP32 with up to 4 frequencies per lane marches with the per-wave sample queue (PHX_MARCH=direct
-  // selects the direct kernel); FP64 and wider FP32 chunks march directly
-  bool queued = false;
-  const void* kfn = reinterpret_cast<const void*>(&k_march<T, NF>);
-  if constexpr (Lim<T>::fp32 && NF <= 4) {
-    const char* e = std::getenv("PHX_MARCH");
-    queued = !(e && !std::strcmp(e, "direct"));
-    if (queued) kfn = reinterpret_cast<const void*>(&k_march_q<T, NF>);
-  }
+  const void* kfn = reinterpret_cast<const void*>(&k_march<M, NF, Q>);
   int cus = 0, per_cu = 0;
   HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
   HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, BLOCK, 0));
   per_cu = std::max(per_cu, 1);
+  if (tn.blocks_per_cu > 0) per_cu = std::min(per_cu, tn.blocks_per_cu);
   int grid = per_cu * cus;
   const int njobs = j1 - j0;
   grid = std::max(1, std::min(grid, (njobs + BLOCK - 1) / BLOCK));
@@ -986,16 +1089,9 @@ static void launch_march(phx_snapshot* s, const Tables<T>& tb, const phx_model* 
   HIPCHECK(hipDeviceGetLimit(&limit, hipLimitStackSize));
   if (fa.localSizeBytes > limit) HIPCHECK(hipDeviceSetLimit(hipLimitStackSize, (fa.localSizeBytes + 255) & ~size_t(255)));
   HIPCHECK(hipMemsetAsync(s->d_counter, 0, sizeof(int), s->stream));
-  if constexpr (Lim<T>::fp32 && NF <= 4) {
-    if (queued)
-      k_march_q<T, NF><<<grid, BLOCK, 0, s->stream>>>(A);
-    else
-      k_march<T, NF><<<grid, BLOCK, 0, s->stream>>>(A);
-  } else {
-    k_march<T, NF><<<grid, BLOCK, 0, s->stream>>>(A);
-  }
+  k_march<M, NF, Q><<<grid, BLOCK, 0, s->stream>>>(A);
   HIPCHECK(hipGetLastError());
-  s->last_chunk_jobs = njobs;
+  s->last_jobs = njobs;
   if (info) {
     info->vgprs = fa.numRegs;
     info->scratch_bytes = (int32_t)fa.localSizeBytes;
@@ -1007,129 +1103,153 @@ static void launch_march(phx_snapshot* s, const Tables<T>& tb, const phx_model* 
   }
 }
 
-template <class T>
-static void march_all(phx_snapshot* s, const Tables<T>& tb, const phx_model* rq, const int* d_jobs, int njobs,
-                      phx_info* info, phx_progress_fn progress, void* user) {
-  // ray starting states for every job (shared by all frequency chunks)
-  const size_t need = (size_t)std::max(njobs, 1) * RAY_REC * sizeof(T);
-  if (need > s->start_cap) {
-    if (s->d_start) HIPCHECK(hipFree(s->d_start));
-    s->d_start = nullptr;
-    HIPCHECK(hipMalloc(&s->d_start, need));
-    s->start_cap = need;
-  }
-  {
-    const phx_model& m = s->geo;
-    Cam C;
-    C.a = (double)T(m.a);  // the camera uses the table-precision spin and stretching (camera.hpp:77)
-    C.hs = (double)T(m.hslope);
-    C.incl = rq->inclination;
-    C.r_cam = rq->r_cam;
-    C.t_init = rq->t_init;
-    C.phi_cam = rq->phi_cam;
-    C.fov_x = rq->fov_x;
-    C.fov_y = rq->fov_y;
-    C.cos_pa = rq->cos_pa;
-    C.sin_pa = rq->sin_pa;
-    C.W = rq->img_width;
-    C.H = rq->img_height;
-    if (njobs > 0)
-      k_rays<T><<<(njobs + BLOCK - 1) / BLOCK, BLOCK, 0, s->stream>>>(C, T(m.a), T(m.hslope), d_jobs, njobs,
-                                                                      (T*)s->d_start);
-    HIPCHECK(hipGetLastError());
-  }
-  int chunk = njobs;
+template <int M, int NF>
+static void launch_nf(phx_snapshot* s, const LaunchInfo& li, int j0, int j1, const Tune& tn, phx_info* info) {
+  const bool q = tn.queue < 0 ? (M != M64) : tn.queue != 0;
+  if (q)
+    launch_march<M, NF, true>(s, li, j0, j1, tn, info);
+  else
+    launch_march<M, NF, false>(s, li, j0, j1, tn, info);
+}
+
+// Ray start records for the jobs in s->jobs (cameras in s->cams), then the march over them,
+// in chunks of PHX_CHUNK jobs when that is set (the reference's chunking, kernel.cu:177-180).
+template <int M>
+static void march_jobs(phx_snapshot* s, const LaunchInfo& li, const Tune& tn, phx_info* info,
+                       phx_progress_fn progress, void* user) {
+  using R = RecOf<M>;
+  const int rec = M == MIX ? StMix::REC : StRef<typename ModeOf<M>::T>::REC;
+  ensure(s->start, (size_t)std::max(li.njobs, 1) * rec * sizeof(R));
+  if (li.njobs > 0)
+    k_rays<M><<<(li.njobs + BLOCK - 1) / BLOCK, BLOCK, 0, s->stream>>>((const Cam*)s->cams.p,
+                                                                      (const int2*)s->jobs.p, li.njobs,
+                                                                      s->start.p);
+  HIPCHECK(hipGetLastError());
+  int chunk = std::max(li.njobs, 1);
   if (const char* e = std::getenv("PHX_CHUNK"))
     if (std::atoi(e) > 0) chunk = std::atoi(e);
-  const int nfr = rq->nfreqs;
-  const int per = Lim<T>::fp32 ? 8 : 4;  // frequencies per march: keeps the FP64 state out of scratch
-  for (int f0 = 0; f0 < nfr; f0 += per) {
-    const int nf = std::min(per, nfr - f0);
-    for (int j0 = 0; j0 < njobs; j0 += chunk) {
-      const int j1 = std::min(njobs, j0 + chunk);
-      const int stats = f0 == 0;
-      if (nf == 1)
-        launch_march<T, 1>(s, tb, rq, f0, nf, d_jobs, j0, j1, stats, info);
-      else if (nf == 2)
-        launch_march<T, 2>(s, tb, rq, f0, nf, d_jobs, j0, j1, stats, info);
-      else if (nf <= 4)
-        launch_march<T, 4>(s, tb, rq, f0, nf, d_jobs, j0, j1, stats, info);
-      else if constexpr (Lim<T>::fp32)
-        launch_march<T, 8>(s, tb, rq, f0, nf, d_jobs, j0, j1, stats, info);
-      if (progress) {
-        HIPCHECK(hipStreamSynchronize(s->stream));
-        if (f0 + nf >= nfr) progress(j1 - j0, user);
-      }
+  for (int j0 = 0; j0 < li.njobs; j0 += chunk) {
+    const int j1 = std::min(li.njobs, j0 + chunk);
+    if (li.nfmax == 1)
+      launch_nf<M, 1>(s, li, j0, j1, tn, info);
+    else if (li.nfmax == 2)
+      launch_nf<M, 2>(s, li, j0, j1, tn, info);
+    else
+      launch_nf<M, 4>(s, li, j0, j1, tn, info);
+    if (progress) {
+      HIPCHECK(hipStreamSynchronize(s->stream));
+      progress(j1 - j0, user);
     }
   }
 }
 
-static void check_request(const phx_snapshot* s, const phx_model* rq, int precision) {
+static void check_precision(const phx_snapshot* s, int precision) {
+  if (precision != PHX_FP32 && precision != PHX_FP64 && precision != PHX_MIXED)
+    throw Error(PHX_E_ARG, "precision must be PHX_FP32, PHX_FP64 or PHX_MIXED");
+  if (precision != PHX_FP64 && !(s->tables & PHX_TABLE_FP32)) throw Error(PHX_E_ARG, "snapshot has no FP32 table");
+  if (precision == PHX_FP64 && !(s->tables & PHX_TABLE_FP64)) throw Error(PHX_E_ARG, "snapshot has no FP64 table");
+}
+
+static void check_request(const phx_model* rq) {
   if (!rq) throw Error(PHX_E_ARG, "null request");
   if (rq->img_width <= 0 || rq->img_height <= 0) throw Error(PHX_E_ARG, "image size must be positive");
   if ((int64_t)rq->img_width * rq->img_height > (int64_t)1 << 30) throw Error(PHX_E_RANGE, "image too large");
   if (rq->nfreqs < 1 || rq->nfreqs > PHX_MAX_FREQS) throw Error(PHX_E_ARG, "nfreqs must be in 1..32");
-  if (precision == PHX_FP32 && !(s->tables & PHX_TABLE_FP32)) throw Error(PHX_E_ARG, "snapshot has no FP32 table");
-  if (precision == PHX_FP64 && !(s->tables & PHX_TABLE_FP64)) throw Error(PHX_E_ARG, "snapshot has no FP64 table");
-  if (precision != PHX_FP32 && precision != PHX_FP64)
-    throw Error(PHX_E_ARG, "precision must be PHX_FP32 or PHX_FP64");
 }
 
-static void render(phx_snapshot* s, const phx_model* rq, int precision, const int32_t* jobs, int32_t njobs,
-                   phx_freqout* res, phx_info* info, phx_progress_fn progress, void* user) {
-  check_request(s, rq, precision);
-  HIPCHECK(hipSetDevice(s->device));
-  const int64_t npix = (int64_t)rq->img_width * rq->img_height;
-  std::vector<int32_t> order;
-  // the full-image Morton list is built once per image size and stays on the device
-  const bool reuse = !jobs && s->morton_w == rq->img_width && s->morton_h == rq->img_height;
-  if (!jobs) {
-    njobs = (int32_t)npix;
-    if (!reuse) {
-      order.resize(npix);
-      if (phx_schedule(rq, 1, order.data(), (int32_t)npix) != PHX_OK) throw Error(PHX_E_ARG, phx_last_error());
-      jobs = order.data();
+// the parameters one launch shares: a batch may differ only in camera, image, frequencies,
+// m_unit and R_high
+static void check_uniform(const phx_model* a, const phx_model* b) {
+  const bool same = a->stepsize == b->stepsize && a->r_out == b->r_out && a->cut_in == b->cut_in &&
+                    a->max_steps == b->max_steps && a->max_order == b->max_order && a->rlow == b->rlow &&
+                    a->ibc == b->ibc && a->sigma_cut == b->sigma_cut && a->thetae_min == b->thetae_min &&
+                    a->thetae_max == b->thetae_max && a->rt_outer == b->rt_outer;
+  if (!same)
+    throw Error(PHX_E_ARG, "requests of one batch must share stepsize, r_cam, horizon margin, max_steps, max_order, "
+                           "r_low, beta_c, sigma_cut, thetae bounds and rt_outer");
+}
+
+static Cam cam_of(const phx_snapshot* s, const phx_model* rq) {
+  Cam C;
+  C.a = s->geo.a;
+  C.hs = s->geo.hslope;
+  C.incl = rq->inclination;
+  C.r_cam = rq->r_cam;
+  C.t_init = rq->t_init;
+  C.phi_cam = rq->phi_cam;
+  C.fov_x = rq->fov_x;
+  C.fov_y = rq->fov_y;
+  C.cos_pa = rq->cos_pa;
+  C.sin_pa = rq->sin_pa;
+  C.W = rq->img_width;
+  C.H = rq->img_height;
+  return C;
+}
+
+// Tasks of one request: chunks of CHUNK frequencies; `out` is the request's result block.
+template <class T>
+static int add_tasks(const phx_model* rq, phx_freqout* out, std::vector<TaskDev<T>>& tasks) {
+  const int first = (int)tasks.size();
+  for (int f0 = 0; f0 < rq->nfreqs; f0 += CHUNK) {
+    TaskDev<T> t;
+    std::memset(&t, 0, sizeof t);
+    t.Ne = T(rq->Ne);
+    t.Bu = T(rq->Bu);
+    t.rhigh = T(rq->rhigh);
+    t.nf = std::min(CHUNK, rq->nfreqs - f0);
+    for (int f = 0; f < t.nf; f++) {
+      t.freq[f] = T(rq->freq[f0 + f]);
+      t.hconst[f] = T(rq->hconst[f0 + f]);
+      t.jscale[f] = T(rq->jscale[f0 + f]);
     }
-  } else {
-    if (njobs < 0 || njobs > npix) throw Error(PHX_E_ARG, "njobs out of range");
-    for (int32_t n = 0; n < njobs; n++)
-      if (jobs[n] < 0 || jobs[n] >= npix) throw Error(PHX_E_ARG, "job index outside the image");
+    t.out = out + f0;
+    t.nf_total = rq->nfreqs;
+    t.stats = f0 == 0;
+    tasks.push_back(t);
   }
-  const size_t nres = (size_t)npix * rq->nfreqs;
-  if (nres > s->res_cap) {
-    if (s->d_res) HIPCHECK(hipFree(s->d_res));
-    s->d_res = nullptr;
-    HIPCHECK(hipMalloc((void**)&s->d_res, nres * sizeof(phx_freqout)));
-    s->res_cap = nres;
+  return (int)tasks.size() - first;
+}
+
+__global__ void k_expand(const int* __restrict__ pix, int npix, int task0, int ntasks, int2* __restrict__ jobs) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= npix * ntasks) return;
+  const int t = n / npix, i = n % npix;
+  jobs[n] = make_int2(task0 + t, pix[i]);
+}
+
+// pixel order of a full image: Morton order of (i, j) (the reference's GPU schedule,
+// src/host/render/jobs.hpp:18-46); a square power-of-two image is enumerated directly
+static void morton_pixels(const phx_model* rq, std::vector<int32_t>& out) {
+  const int W = rq->img_width, H = rq->img_height;
+  const int64_t n = (int64_t)W * H;
+  out.resize(n);
+  if (W == H && (W & (W - 1)) == 0 && W <= 65536) {
+    for (int64_t k = 0; k < n; k++) {
+      uint32_t i = 0, j = 0;
+      for (int b = 0; b < 16; b++) {
+        j |= (uint32_t)((k >> (2 * b)) & 1) << b;
+        i |= (uint32_t)((k >> (2 * b + 1)) & 1) << b;
+      }
+      out[k] = (int32_t)((int64_t)i * H + j);
+    }
+    return;
   }
-  if ((size_t)njobs > s->jobs_cap) {
-    if (s->d_jobs) HIPCHECK(hipFree(s->d_jobs));
-    s->d_jobs = nullptr;
-    s->morton_w = s->morton_h = 0;
-    HIPCHECK(hipMalloc((void**)&s->d_jobs, std::max<size_t>(njobs, 1) * sizeof(int32_t)));
-    s->jobs_cap = std::max<size_t>(njobs, 1);
-  }
-  phx_info local;
-  if (!info) info = &local;
-  std::memset(info, 0, sizeof *info);
+  if (phx_schedule(rq, 1, out.data(), (int32_t)n) != PHX_OK) throw Error(PHX_E_ARG, phx_last_error());
+}
+
+static void fill_info_device(const phx_snapshot* s, phx_info* info) {
   hipDeviceProp_t prop;
   HIPCHECK(hipGetDeviceProperties(&prop, s->device));
   std::snprintf(info->device, sizeof info->device, "%s (%s)", prop.name, prop.gcnArchName);
-  if (jobs) {
-    HIPCHECK(hipMemcpyAsync(s->d_jobs, jobs, (size_t)njobs * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
-    s->morton_w = s->morton_h = 0;
-    if (!order.empty()) {
-      s->morton_w = rq->img_width;
-      s->morton_h = rq->img_height;
-    }
-  }
-  if ((int64_t)njobs < npix) HIPCHECK(hipMemsetAsync(s->d_res, 0, nres * sizeof(phx_freqout), s->stream));
+}
+
+// the march of the jobs already in s->jobs; fills timing and statistics
+template <int M>
+static void run_march(phx_snapshot* s, const LaunchInfo& li, const Tune& tn, phx_info* info, phx_progress_fn progress,
+                      void* user) {
   HIPCHECK(hipMemsetAsync(s->d_tot, 0, 4 * sizeof(unsigned long long), s->stream));
   HIPCHECK(hipEventRecord(s->e0, s->stream));
-  if (precision == PHX_FP32)
-    march_all<float>(s, s->t32, rq, s->d_jobs, njobs, info, progress, user);
-  else
-    march_all<double>(s, s->t64, rq, s->d_jobs, njobs, info, progress, user);
+  march_jobs<M>(s, li, tn, info, progress, user);
   HIPCHECK(hipEventRecord(s->e1, s->stream));
   HIPCHECK(hipEventSynchronize(s->e1));
   float ms = 0.f;
@@ -1138,18 +1258,475 @@ static void render(phx_snapshot* s, const phx_model* rq, int precision, const in
   HIPCHECK(hipMemcpy(tot, s->d_tot, sizeof tot, hipMemcpyDeviceToHost));
   int taken = 0;
   HIPCHECK(hipMemcpy(&taken, s->d_counter, sizeof taken, hipMemcpyDeviceToHost));
-  if (taken < s->last_chunk_jobs)  // every lane takes jobs until the list is exhausted
+  if (li.njobs > 0 && taken < s->last_jobs)  // every lane takes jobs until the list is exhausted
     throw Error(PHX_E_DEVICE, "march kernel did not run to completion (" + std::to_string(taken) + " of " +
-                                  std::to_string(s->last_chunk_jobs) + " jobs taken)");
+                                  std::to_string(s->last_jobs) + " jobs taken)");
   info->kernel_s = ms * 1e-3;
   info->steps = (int64_t)tot[0];
   info->nplasma = (int64_t)tot[1];
   info->bad = (int64_t)tot[2];
-  info->rays = (int64_t)njobs * rq->nfreqs;
+  info->fp64_steps = (int64_t)tot[3];
   info->decode_s = s->decode_s;
   info->upload_s = s->upload_s;
+}
+
+static void run_precision(phx_snapshot* s, int precision, const LaunchInfo& li, const Tune& tn, phx_info* info,
+                          phx_progress_fn progress, void* user) {
+  if (precision == PHX_FP32)
+    run_march<M32>(s, li, tn, info, progress, user);
+  else if (precision == PHX_FP64)
+    run_march<M64>(s, li, tn, info, progress, user);
+  else
+    run_march<MIX>(s, li, tn, info, progress, user);
+}
+
+template <class T>
+static void upload_tasks(phx_snapshot* s, const std::vector<TaskDev<T>>& tasks, const std::vector<Cam>& cams) {
+  ensure(s->tasks, tasks.size() * sizeof(TaskDev<T>));
+  ensure(s->cams, cams.size() * sizeof(Cam));
+  HIPCHECK(hipMemcpyAsync(s->tasks.p, tasks.data(), tasks.size() * sizeof(TaskDev<T>), hipMemcpyHostToDevice,
+                          s->stream));
+  HIPCHECK(hipMemcpyAsync(s->cams.p, cams.data(), cams.size() * sizeof(Cam), hipMemcpyHostToDevice, s->stream));
+}
+
+// One request: its frequency chunks become tasks over the same pixel list (the full image in
+// Morton order, cached on the device per image size, or the caller's jobs).
+static void render(phx_snapshot* s, const phx_model* rq, int precision, const int32_t* jobs, int32_t njobs,
+                   phx_freqout* res, phx_info* info, phx_progress_fn progress, void* user, const Tune& tn0) {
+  check_precision(s, precision);
+  check_request(rq);
+  std::lock_guard<std::mutex> lk(s->mu);
+  HIPCHECK(hipSetDevice(s->device));
+  const Tune tn = tune_from_env(tn0);
+  const int64_t npix = (int64_t)rq->img_width * rq->img_height;
+  if (jobs) {
+    if (njobs < 0 || njobs > npix) throw Error(PHX_E_ARG, "njobs out of range");
+    for (int32_t n = 0; n < njobs; n++)
+      if (jobs[n] < 0 || jobs[n] >= npix) throw Error(PHX_E_ARG, "job index outside the image");
+  } else {
+    njobs = (int32_t)npix;
+  }
+  const size_t nres = (size_t)npix * rq->nfreqs;
+  phx_freqout* d_res = (phx_freqout*)ensure(s->res, nres * sizeof(phx_freqout));
+  // pixel list
+  if (jobs) {
+    ensure(s->pix, (size_t)njobs * sizeof(int32_t));
+    HIPCHECK(hipMemcpyAsync(s->pix.p, jobs, (size_t)njobs * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+    s->pix_w = s->pix_h = 0;
+  } else if (!(s->pix_w == rq->img_width && s->pix_h == rq->img_height)) {
+    std::vector<int32_t> order;
+    morton_pixels(rq, order);
+    ensure(s->pix, (size_t)npix * sizeof(int32_t));
+    HIPCHECK(hipMemcpyAsync(s->pix.p, order.data(), (size_t)npix * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    s->pix_w = rq->img_width;
+    s->pix_h = rq->img_height;
+  }
+  // tasks, cameras, jobs
+  const Cam cam = cam_of(s, rq);
+  int ntasks = 0;
+  if (precision == PHX_FP64) {
+    std::vector<TaskDev<double>> tk;
+    ntasks = add_tasks<double>(rq, d_res, tk);
+    upload_tasks(s, tk, std::vector<Cam>(ntasks, cam));
+  } else {
+    std::vector<TaskDev<float>> tk;
+    ntasks = add_tasks<float>(rq, d_res, tk);
+    upload_tasks(s, tk, std::vector<Cam>(ntasks, cam));
+  }
+  const int64_t ntot = (int64_t)njobs * ntasks;
+  if (ntot > INT32_MAX) throw Error(PHX_E_RANGE, "too many jobs in one render");
+  ensure(s->jobs, (size_t)std::max<int64_t>(ntot, 1) * sizeof(int2));
+  if (ntot > 0)
+    k_expand<<<(unsigned)((ntot + BLOCK - 1) / BLOCK), BLOCK, 0, s->stream>>>((const int*)s->pix.p, njobs, 0, ntasks,
+                                                                              (int2*)s->jobs.p);
+  HIPCHECK(hipGetLastError());
+  if ((int64_t)njobs < npix) HIPCHECK(hipMemsetAsync(d_res, 0, nres * sizeof(phx_freqout), s->stream));
+  phx_info local;
+  if (!info) info = &local;
+  std::memset(info, 0, sizeof *info);
+  fill_info_device(s, info);
+  LaunchInfo li;
+  li.njobs = (int)ntot;
+  li.ntasks = ntasks;
+  li.nfmax = std::min(CHUNK, rq->nfreqs) <= 1 ? 1 : std::min(CHUNK, rq->nfreqs) <= 2 ? 2 : 4;
+  li.uni = rq;
+  run_precision(s, precision, li, tn, info, progress, user);
+  info->rays = (int64_t)njobs * rq->nfreqs;
   s->last_nres = (int64_t)nres;
-  if (res) HIPCHECK(hipMemcpy(res, s->d_res, nres * sizeof(phx_freqout), hipMemcpyDeviceToHost));
+  if (res) HIPCHECK(hipMemcpy(res, d_res, nres * sizeof(phx_freqout), hipMemcpyDeviceToHost));
+}
+
+
+// Many requests in one launch: every request's frequency chunks become tasks, and the job list
+// holds all their pixels (each request's in Morton order, requests one after another, or
+// interleaved 64 jobs at a time).  Results go to one device block per request.
+struct BatchPlan {
+  std::vector<int64_t> off;  // result offset of each request in s->res (entries)
+  int64_t nres = 0, rays = 0;
+  int njobs = 0, ntasks = 0, nfmax = 1;
+};
+template <class T>
+static BatchPlan plan_batch(phx_snapshot* s, const phx_model* const* rqs, int n, int order, int sample_stride) {
+  BatchPlan bp;
+  bp.off.resize(n);
+  for (int r = 0; r < n; r++) {
+    check_request(rqs[r]);
+    check_uniform(rqs[0], rqs[r]);
+    bp.off[r] = bp.nres;
+    bp.nres += (int64_t)rqs[r]->img_width * rqs[r]->img_height * rqs[r]->nfreqs;
+  }
+  phx_freqout* d_res = (phx_freqout*)ensure(s->res, (size_t)bp.nres * sizeof(phx_freqout));
+  std::vector<TaskDev<T>> tasks;
+  std::vector<Cam> cams;
+  std::vector<std::vector<int2>> per(n);
+  std::vector<int32_t> pix;
+  for (int r = 0; r < n; r++) {
+    const int t0 = (int)tasks.size();
+    const int nt = add_tasks<T>(rqs[r], d_res + bp.off[r], tasks);
+    const Cam c = cam_of(s, rqs[r]);
+    for (int t = 0; t < nt; t++) cams.push_back(c);
+    bp.nfmax = std::max(bp.nfmax, std::min(CHUNK, rqs[r]->nfreqs));
+    morton_pixels(rqs[r], pix);
+    std::vector<int2>& jl = per[r];
+    for (int t = 0; t < nt; t++)
+      for (size_t i = 0; i < pix.size(); i += 64 * (size_t)sample_stride)
+        for (size_t k = i; k < std::min(pix.size(), i + 64); k++) jl.push_back(make_int2(t0 + t, pix[k]));
+    bp.rays += (int64_t)(jl.size() / nt) * rqs[r]->nfreqs;
+  }
+  std::vector<int2> jobs;
+  if (order == 1) {  // round-robin over requests, 64 jobs (one wave) at a time
+    std::vector<size_t> pos(n, 0);
+    for (bool any = true; any;) {
+      any = false;
+      for (int r = 0; r < n; r++)
+        if (pos[r] < per[r].size()) {
+          const size_t e = std::min(per[r].size(), pos[r] + 64);
+          jobs.insert(jobs.end(), per[r].begin() + pos[r], per[r].begin() + e);
+          pos[r] = e;
+          any = true;
+        }
+    }
+  } else {
+    for (int r = 0; r < n; r++) jobs.insert(jobs.end(), per[r].begin(), per[r].end());
+  }
+  if (jobs.size() > (size_t)INT32_MAX) throw Error(PHX_E_RANGE, "too many jobs in one batch");
+  bp.njobs = (int)jobs.size();
+  bp.ntasks = (int)tasks.size();
+  bp.nfmax = bp.nfmax <= 1 ? 1 : bp.nfmax <= 2 ? 2 : 4;
+  upload_tasks(s, tasks, cams);
+  ensure(s->jobs, jobs.size() * sizeof(int2));
+  HIPCHECK(hipMemcpyAsync(s->jobs.p, jobs.data(), jobs.size() * sizeof(int2), hipMemcpyHostToDevice, s->stream));
+  if (sample_stride > 1) HIPCHECK(hipMemsetAsync(d_res, 0, (size_t)bp.nres * sizeof(phx_freqout), s->stream));
+  HIPCHECK(hipStreamSynchronize(s->stream));
+  return bp;
+}
+
+static BatchPlan run_batch(phx_snapshot* s, const phx_model* const* rqs, int n, int precision, const Tune& tn,
+                           int sample_stride, phx_info* info) {
+  check_precision(s, precision);
+  if (n < 1) throw Error(PHX_E_ARG, "empty batch");
+  HIPCHECK(hipSetDevice(s->device));
+  const BatchPlan bp = precision == PHX_FP64 ? plan_batch<double>(s, rqs, n, tn.order, sample_stride)
+                                             : plan_batch<float>(s, rqs, n, tn.order, sample_stride);
+  std::memset(info, 0, sizeof *info);
+  fill_info_device(s, info);
+  LaunchInfo li;
+  li.njobs = bp.njobs;
+  li.ntasks = bp.ntasks;
+  li.nfmax = bp.nfmax;
+  li.uni = rqs[0];
+  run_precision(s, precision, li, tn, info, nullptr, nullptr);
+  info->rays = bp.rays;
+  return bp;
+}
+
+static void render_batch(phx_snapshot* s, const phx_model* const* rqs, int n, int precision, phx_freqout* const* res,
+                         phx_info* info) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  phx_info local;
+  if (!info) info = &local;
+  const BatchPlan bp = run_batch(s, rqs, n, precision, tune_from_env(s->tune), 1, info);
+  s->last_nres = bp.nres;
+  for (int r = 0; r < n; r++)
+    if (res && res[r])
+      HIPCHECK(hipMemcpy(res[r], (phx_freqout*)s->res.p + bp.off[r],
+                         (size_t)rqs[r]->img_width * rqs[r]->img_height * rqs[r]->nfreqs * sizeof(phx_freqout),
+                         hipMemcpyDeviceToHost));
+}
+
+// Autotuner: times every launch configuration (LDS queue or in place, occupancy, job order) on
+// a strided sample of the batch (every sample_stride-th wave of 64 jobs), keeps the fastest in
+// the snapshot and returns it with its sampled rate.
+static void autotune(phx_snapshot* s, const phx_model* const* rqs, int n, int precision, int sample_stride,
+                     phx_tune* best, double* best_rate) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  sample_stride = std::max(1, sample_stride);
+  int max_bpc = 0;
+  {
+    int cus = 0;
+    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
+    (void)cus;
+    max_bpc = precision == PHX_FP64 ? 1 : 2;
+  }
+  double rate_best = -1.;
+  Tune tb;
+  for (int q = 0; q <= 1; q++)
+    for (int bpc = 1; bpc <= max_bpc; bpc++)
+      for (int order = 0; order <= (n > 1 ? 1 : 0); order++)
+        for (int fw = 0; fw <= (precision == PHX_MIXED ? 1 : 0); fw++) {
+        Tune t;
+        t.queue = q;
+        t.blocks_per_cu = bpc;
+        t.order = order;
+        t.fp64_wave = fw;
+        phx_info inf;
+        const BatchPlan bp = run_batch(s, rqs, n, precision, t, sample_stride, &inf);
+        const double rate = (double)bp.rays / std::max(inf.kernel_s, 1e-9);
+        if (rate > rate_best) {
+          rate_best = rate;
+          tb = t;
+        }
+      }
+  s->tune = tb;
+  if (best) {
+    best->queue = tb.queue;
+    best->blocks_per_cu = tb.blocks_per_cu;
+    best->order = tb.order;
+    best->fp64_wave = tb.fp64_wave;
+  }
+  if (best_rate) *best_rate = rate_best;
+}
+
+// ============================================================================ serving loop
+// A worker thread keeps the snapshots resident and serves render requests with continuous
+// batching: a request is admitted when its result buffer fits the HBM budget, its jobs (pixels
+// x frequency chunks) join a FIFO of active work, and every launch takes the next `quantum`
+// jobs across all active requests of a snapshot, so requests that arrive while others are on
+// the GPU share the following launches with them.  Snapshots with pending work are launched on
+// their own streams and overlap on the device.  A request is done when its last job has been
+// marched; its result is copied to the caller's buffer and its latency (submit -> copied)
+// recorded.  No counterpart in the reference (one image per process run, main.cpp:143-230).
+struct Request {
+  int64_t ticket = 0;
+  int snap = 0;
+  phx_model rq{};
+  phx_freqout* host_res = nullptr;
+  phx_freqout* d_res = nullptr;
+  int64_t bytes = 0;
+  std::vector<int2> jobs;  // (chunk, pixel), chunk-major
+  size_t next = 0, done = 0;
+  double t_submit = 0., t_done = 0.;
+  int state = 0;  // 0 waiting, 1 active, 2 done, 3 failed
+  std::string error;
+};
+
+struct Server {
+  std::vector<phx_snapshot*> snaps;
+  phx_server_opts opts{};
+  std::thread worker;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  bool stop = false;
+  int64_t next_ticket = 1;
+  std::deque<std::shared_ptr<Request>> waiting;
+  std::vector<std::shared_ptr<Request>> active;
+  std::unordered_map<int64_t, std::shared_ptr<Request>> all;
+  std::vector<double> latencies;
+  std::vector<phx_model> uni;  // per snapshot: the launch-wide parameters of its first request
+  std::vector<char> have_uni;
+  int64_t submitted = 0, completed = 0, failed = 0, rays = 0, launches = 0, admitted_bytes = 0, peak_bytes = 0;
+  double busy_s = 0., t_start = 0.;
+  std::string fatal;
+};
+
+template <class T>
+static void serve_tick(Server* sv, std::vector<std::shared_ptr<Request>>& act) {
+  const int M = sv->opts.precision == PHX_FP32 ? M32 : sv->opts.precision == PHX_FP64 ? M64 : MIX;
+  const int nsn = (int)sv->snaps.size();
+  struct Slice {
+    Request* r;
+    size_t a, b;
+  };
+  std::vector<std::vector<Slice>> slices(nsn);
+  std::vector<int> njobs(nsn, 0);
+  const size_t quantum = (size_t)std::max(1, sv->opts.quantum_jobs);
+  for (auto& r : act) {  // FIFO: the oldest request's remaining jobs first
+    const int k = r->snap;
+    const size_t left = r->jobs.size() - r->next;
+    const size_t room = quantum - (size_t)njobs[k];
+    if (!left || !room) continue;
+    const size_t take = std::min(left, room);
+    slices[k].push_back({r.get(), r->next, r->next + take});
+    njobs[k] += (int)take;
+    r->next += take;
+  }
+  std::vector<int> launched;
+  std::vector<std::unique_lock<std::mutex>> held;  // the snapshots' own renders wait for this tick
+  for (int k = 0; k < nsn; k++) {
+    if (!njobs[k]) continue;
+    phx_snapshot* s = sv->snaps[k];
+    held.emplace_back(s->mu);
+    std::vector<TaskDev<T>> tasks;
+    std::vector<Cam> cams;
+    std::vector<int2> jobs;
+    jobs.reserve(njobs[k]);
+    int nfmax = 1;
+    for (const Slice& sl : slices[k]) {
+      const int t0 = (int)tasks.size();
+      const int nt = add_tasks<T>(&sl.r->rq, sl.r->d_res, tasks);
+      const Cam c = cam_of(s, &sl.r->rq);
+      for (int t = 0; t < nt; t++) cams.push_back(c);
+      nfmax = std::max(nfmax, std::min(CHUNK, sl.r->rq.nfreqs));
+      for (size_t j = sl.a; j < sl.b; j++) jobs.push_back(make_int2(t0 + sl.r->jobs[j].x, sl.r->jobs[j].y));
+    }
+    HIPCHECK(hipSetDevice(s->device));
+    upload_tasks(s, tasks, cams);
+    ensure(s->jobs, jobs.size() * sizeof(int2));
+    HIPCHECK(hipMemcpyAsync(s->jobs.p, jobs.data(), jobs.size() * sizeof(int2), hipMemcpyHostToDevice, s->stream));
+    LaunchInfo li;
+    li.njobs = njobs[k];
+    li.ntasks = (int)tasks.size();
+    li.nfmax = nfmax <= 1 ? 1 : nfmax <= 2 ? 2 : 4;
+    li.uni = &slices[k].front().r->rq;
+    HIPCHECK(hipMemsetAsync(s->d_tot, 0, 4 * sizeof(unsigned long long), s->stream));
+    HIPCHECK(hipEventRecord(s->e0, s->stream));
+    const Tune tn = tune_from_env(s->tune);
+    if (M == M32)
+      march_jobs<M32>(s, li, tn, nullptr, nullptr, nullptr);
+    else if (M == M64)
+      march_jobs<M64>(s, li, tn, nullptr, nullptr, nullptr);
+    else
+      march_jobs<MIX>(s, li, tn, nullptr, nullptr, nullptr);
+    HIPCHECK(hipEventRecord(s->e1, s->stream));
+    launched.push_back(k);
+  }
+  double busy = 0.;
+  for (int k : launched) {
+    phx_snapshot* s = sv->snaps[k];
+    HIPCHECK(hipEventSynchronize(s->e1));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, s->e0, s->e1));
+    busy = std::max(busy, ms * 1e-3);
+    int taken = 0;
+    HIPCHECK(hipMemcpy(&taken, s->d_counter, sizeof taken, hipMemcpyDeviceToHost));
+    if (taken < s->last_jobs) throw Error(PHX_E_DEVICE, "serving launch did not run to completion");
+    for (const Slice& sl : slices[k]) sl.r->done += sl.b - sl.a;
+  }
+  held.clear();
+  std::lock_guard<std::mutex> lk(sv->mu);
+  sv->busy_s += busy;
+  sv->launches += (int64_t)launched.size();
+}
+
+static void serve_loop(Server* sv) {
+  std::vector<std::shared_ptr<Request>> act;
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->cv_work.wait(lk, [&] { return sv->stop || !sv->waiting.empty() || !sv->active.empty(); });
+      if (sv->stop) break;
+      // admission by HBM budget, in arrival order (one request is always admitted when none is active)
+      while (!sv->waiting.empty()) {
+        auto& r = sv->waiting.front();
+        if (!sv->active.empty() && sv->admitted_bytes + r->bytes > sv->opts.hbm_budget) break;
+        if (sv->opts.max_active > 0 && (int)sv->active.size() >= sv->opts.max_active) break;
+        sv->admitted_bytes += r->bytes;
+        sv->peak_bytes = std::max(sv->peak_bytes, sv->admitted_bytes);
+        r->state = 1;
+        sv->active.push_back(r);
+        sv->waiting.pop_front();
+      }
+      act = sv->active;
+    }
+    // device buffers and job lists of newly admitted requests (outside the lock)
+    for (auto& r : act)
+      if (!r->d_res && r->state == 1) {
+        try {
+          phx_snapshot* s = sv->snaps[r->snap];
+          HIPCHECK(hipSetDevice(s->device));
+          HIPCHECK(hipMalloc((void**)&r->d_res, (size_t)r->bytes));
+          std::vector<int32_t> pix;
+          morton_pixels(&r->rq, pix);
+          const int nch = (r->rq.nfreqs + CHUNK - 1) / CHUNK;
+          r->jobs.reserve(pix.size() * nch);
+          for (int c = 0; c < nch; c++)
+            for (int32_t p : pix) r->jobs.push_back(make_int2(c, p));
+        } catch (const std::exception& e) {
+          std::lock_guard<std::mutex> lk(sv->mu);
+          r->state = 3;
+          r->error = e.what();
+        }
+      }
+    try {
+      std::vector<std::shared_ptr<Request>> run;
+      for (auto& r : act)
+        if (r->state == 1) run.push_back(r);
+      if (!run.empty()) {
+        if (sv->opts.precision == PHX_FP64)
+          serve_tick<double>(sv, run);
+        else
+          serve_tick<float>(sv, run);
+      }
+    } catch (const std::exception& e) {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      sv->fatal = e.what();
+      for (auto& r : act) {
+        r->state = 3;
+        r->error = e.what();
+      }
+    }
+    // completed requests: copy out, free, record
+    for (auto& r : act) {
+      const bool finished = r->state == 1 && r->done == r->jobs.size() && !r->jobs.empty();
+      std::string copy_error;
+      if (finished) {
+        try {
+          const size_t n = (size_t)r->rq.img_width * r->rq.img_height * r->rq.nfreqs;
+          if (r->host_res) HIPCHECK(hipMemcpy(r->host_res, r->d_res, n * sizeof(phx_freqout), hipMemcpyDeviceToHost));
+        } catch (const std::exception& e) {
+          copy_error = e.what();
+        }
+      }
+      bool failed_now = false;
+      {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (!copy_error.empty()) {
+          r->state = 3;
+          r->error = copy_error;
+        }
+        failed_now = r->state == 3;
+      }
+      if (finished || failed_now) {
+        if (r->d_res) (void)hipFree(r->d_res);
+        r->d_res = nullptr;
+        r->jobs.clear();
+        r->jobs.shrink_to_fit();
+        std::lock_guard<std::mutex> lk(sv->mu);
+        r->t_done = now_s();
+        sv->admitted_bytes -= r->bytes;
+        if (r->state == 3) {
+          sv->failed++;
+        } else {
+          r->state = 2;
+          sv->completed++;
+          sv->rays += (int64_t)r->rq.img_width * r->rq.img_height * r->rq.nfreqs;
+          sv->latencies.push_back(r->t_done - r->t_submit);
+        }
+        sv->active.erase(std::remove(sv->active.begin(), sv->active.end(), r), sv->active.end());
+      }
+    }
+    sv->cv_done.notify_all();
+  }
+}
+
+static double percentile(std::vector<double> v, double q) {
+  if (v.empty()) return 0.;
+  std::sort(v.begin(), v.end());
+  const double pos = q * (double)(v.size() - 1);
+  const size_t i = (size_t)pos;
+  const double f = pos - (double)i;
+  return i + 1 < v.size() ? v[i] * (1. - f) + v[i + 1] * f : v[i];
 }
 
 }  // namespace phx
@@ -1169,6 +1746,7 @@ int phx_set_device(int32_t device) {
 
 int phx_dev_warmup(void) {
   return guarded([&] {
+    std::lock_guard<std::mutex> lk(g_warm_mu);
     if (!g_warm) g_warm = new std::thread([] { (void)hipFree(nullptr); });
   });
 }
@@ -1179,7 +1757,7 @@ int phx_dev_render(const phx_model* model, const char* dump, int32_t precision, 
     if (!res) throw Error(PHX_E_ARG, "res must not be NULL");
     const int tables = precision == PHX_FP64 ? PHX_TABLE_FP64 : PHX_TABLE_FP32;
     std::unique_ptr<phx_snapshot, void (*)(phx_snapshot*)> s(snapshot_open(model, dump, tables), snapshot_free);
-    render(s.get(), model, precision, jobs, njobs, res, info, progress, user);
+    render(s.get(), model, precision, jobs, njobs, res, info, progress, user, Tune());
   });
 }
 
@@ -1200,14 +1778,176 @@ int phx_snapshot_render(phx_snapshot* s, const phx_model* rq, int32_t precision,
                         phx_freqout* res, phx_info* info) {
   return guarded([&] {
     if (!s) throw Error(PHX_E_ARG, "null snapshot");
-    render(s, rq, precision, jobs, njobs, res, info, nullptr, nullptr);
+    render(s, rq, precision, jobs, njobs, res, info, nullptr, nullptr, s->tune);
+  });
+}
+
+int phx_snapshot_render_batch(phx_snapshot* s, const phx_model* const* requests, int32_t n, int32_t precision,
+                              phx_freqout* const* res, phx_info* info) {
+  return guarded([&] {
+    if (!s || !requests) throw Error(PHX_E_ARG, "null snapshot or request list");
+    render_batch(s, requests, n, precision, res, info);
+  });
+}
+
+int phx_autotune(phx_snapshot* s, const phx_model* const* requests, int32_t n, int32_t precision,
+                 int32_t sample_stride, phx_tune* best, double* rate) {
+  return guarded([&] {
+    if (!s || !requests) throw Error(PHX_E_ARG, "null snapshot or request list");
+    autotune(s, requests, n, precision, sample_stride, best, rate);
+  });
+}
+
+int phx_snapshot_set_tune(phx_snapshot* s, const phx_tune* t) {
+  return guarded([&] {
+    if (!s || !t) throw Error(PHX_E_ARG, "null snapshot or tune");
+    if (t->queue < -1 || t->queue > 1 || t->blocks_per_cu < 0 || t->order < 0 || t->order > 1 || t->fp64_wave < 0 ||
+        t->fp64_wave > 1)
+      throw Error(PHX_E_ARG, "tune value out of range");
+    s->tune.queue = t->queue;
+    s->tune.blocks_per_cu = t->blocks_per_cu;
+    s->tune.order = t->order;
+    s->tune.fp64_wave = t->fp64_wave;
+  });
+}
+
+}  // extern "C"
+
+struct phx_server {
+  phx::Server sv;
+};
+
+extern "C" {
+
+int phx_server_start(phx_snapshot* const* snaps, int32_t nsnaps, const phx_server_opts* opts, phx_server** out) {
+  return guarded([&] {
+    if (!snaps || nsnaps < 1 || !out) throw Error(PHX_E_ARG, "server needs at least one snapshot");
+    for (int i = 0; i < nsnaps; i++)
+      if (!snaps[i]) throw Error(PHX_E_ARG, "null snapshot");
+    auto srv = std::make_unique<phx_server>();
+    Server& sv = srv->sv;
+    sv.snaps.assign(snaps, snaps + nsnaps);
+    sv.uni.assign(nsnaps, phx_model{});
+    sv.have_uni.assign(nsnaps, 0);
+    if (opts) sv.opts = *opts;
+    if (sv.opts.precision == 0) sv.opts.precision = PHX_FP32;
+    for (phx_snapshot* s : sv.snaps) check_precision(s, sv.opts.precision);
+    if (sv.opts.hbm_budget <= 0) sv.opts.hbm_budget = (int64_t)8 << 30;
+    if (sv.opts.quantum_jobs <= 0) sv.opts.quantum_jobs = 1 << 18;
+    sv.t_start = now_s();
+    sv.worker = std::thread(serve_loop, &sv);
+    *out = srv.release();
+  });
+}
+
+int phx_server_submit(phx_server* srv, int32_t snap, const phx_model* request, phx_freqout* res, int64_t* ticket) {
+  return guarded([&] {
+    if (!srv || !request) throw Error(PHX_E_ARG, "null server or request");
+    Server& sv = srv->sv;
+    if (snap < 0 || snap >= (int)sv.snaps.size()) throw Error(PHX_E_ARG, "no such snapshot");
+    check_request(request);
+    auto r = std::make_shared<Request>();
+    r->snap = snap;
+    r->rq = *request;
+    r->host_res = res;
+    r->bytes = (int64_t)request->img_width * request->img_height * request->nfreqs * (int64_t)sizeof(phx_freqout);
+    if (r->bytes > sv.opts.hbm_budget) throw Error(PHX_E_RANGE, "request larger than the server's HBM budget");
+    {
+      std::lock_guard<std::mutex> lk(sv.mu);
+      if (sv.stop) throw Error(PHX_E_ARG, "server stopped");
+      // a launch shares stepping and cuts: the requests of one snapshot must agree with its first
+      if (sv.have_uni[snap])
+        check_uniform(&sv.uni[snap], request);
+      else
+        sv.uni[snap] = *request;
+      sv.have_uni[snap] = true;
+      r->ticket = sv.next_ticket++;
+      r->t_submit = now_s();
+      sv.waiting.push_back(r);
+      sv.all[r->ticket] = r;
+      sv.submitted++;
+      if (ticket) *ticket = r->ticket;
+    }
+    sv.cv_work.notify_one();
+  });
+}
+
+int phx_server_wait(phx_server* srv, int64_t ticket, double timeout_s, int32_t* done, double* latency_s) {
+  return guarded([&] {
+    if (!srv) throw Error(PHX_E_ARG, "null server");
+    Server& sv = srv->sv;
+    std::unique_lock<std::mutex> lk(sv.mu);
+    auto it = sv.all.find(ticket);
+    if (it == sv.all.end()) throw Error(PHX_E_ARG, "unknown ticket");
+    std::shared_ptr<Request> r = it->second;
+    auto ready = [&] { return r->state >= 2; };
+    if (timeout_s < 0)
+      sv.cv_done.wait(lk, ready);
+    else
+      sv.cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready);
+    if (done) *done = r->state >= 2;
+    if (latency_s) *latency_s = r->state >= 2 ? r->t_done - r->t_submit : 0.;
+    if (r->state == 3) throw Error(PHX_E_DEVICE, "request failed: " + r->error);
+    if (r->state >= 2) sv.all.erase(it);
+  });
+}
+
+int phx_server_stats(phx_server* srv, phx_serve_stats* st) {
+  return guarded([&] {
+    if (!srv || !st) throw Error(PHX_E_ARG, "null server or stats");
+    Server& sv = srv->sv;
+    std::lock_guard<std::mutex> lk(sv.mu);
+    std::memset(st, 0, sizeof *st);
+    st->submitted = sv.submitted;
+    st->completed = sv.completed;
+    st->failed = sv.failed;
+    st->active = (int64_t)sv.active.size();
+    st->waiting = (int64_t)sv.waiting.size();
+    st->rays = sv.rays;
+    st->launches = sv.launches;
+    st->busy_s = sv.busy_s;
+    st->wall_s = now_s() - sv.t_start;
+    st->peak_bytes = sv.peak_bytes;
+    st->p50_s = percentile(sv.latencies, 0.50);
+    st->p99_s = percentile(sv.latencies, 0.99);
+    st->max_s = sv.latencies.empty() ? 0. : *std::max_element(sv.latencies.begin(), sv.latencies.end());
+    double sum = 0.;
+    for (double v : sv.latencies) sum += v;
+    st->mean_s = sv.latencies.empty() ? 0. : sum / (double)sv.latencies.size();
+  });
+}
+
+int phx_server_reset_stats(phx_server* srv) {
+  return guarded([&] {
+    if (!srv) throw Error(PHX_E_ARG, "null server");
+    Server& sv = srv->sv;
+    std::lock_guard<std::mutex> lk(sv.mu);
+    sv.latencies.clear();
+    sv.completed = sv.failed = sv.rays = sv.launches = sv.submitted = 0;
+    sv.busy_s = 0.;
+    sv.t_start = now_s();
+  });
+}
+
+int phx_server_stop(phx_server* srv) {
+  return guarded([&] {
+    if (!srv) return;
+    {
+      std::lock_guard<std::mutex> lk(srv->sv.mu);
+      srv->sv.stop = true;
+    }
+    srv->sv.cv_work.notify_all();
+    if (srv->sv.worker.joinable()) srv->sv.worker.join();
+    for (auto& r : srv->sv.active)
+      if (r->d_res) (void)hipFree(r->d_res);
+    delete srv;
   });
 }
 
 int phx_snapshot_result(phx_snapshot* s, phx_freqout* res, int64_t nres) {
   return guarded([&] {
     if (!s || !res || nres != s->last_nres) throw Error(PHX_E_ARG, "result size mismatch");
-    HIPCHECK(hipMemcpy(res, s->d_res, (size_t)nres * sizeof(phx_freqout), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(res, s->res.p, (size_t)nres * sizeof(phx_freqout), hipMemcpyDeviceToHost));
   });
 }
 

@@ -8,6 +8,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "phx_bessel_table.h"
 
 namespace phxd {
@@ -89,6 +91,51 @@ PHX_DEV void theta_trig(double x2, double hs, double* s2x, double* c2x, double* 
   sincos(2. * x2, s2x, c2x);
   sincos(x2 + 0.5 * hs * *s2x, s, c);
 }
+// FP64 with one library sincos: sin/cos of x2, the double-angle identities, and theta = x2 +
+// delta by angle addition with sin/cos of |delta| <= h/2 <= 0.5 in Taylor series to delta^19 /
+// delta^20 (truncation below 2e-23).  Used by the FP64 steps of the mixed march.
+PHX_DEV void theta_trig_1(double x2, double hs, double* s2x, double* c2x, double* s, double* c) {
+  double sx, cx;
+  sincos(x2, &sx, &cx);
+  *s2x = 2. * sx * cx;
+  *c2x = (cx - sx) * (cx + sx);
+  const double d = 0.5 * hs * *s2x, z = d * d;
+  // sin d = d (1 - z/3! + z^2/5! - ...), cos d = 1 - z/2! + z^2/4! - ... (Horner in z)
+  double ps = -1. / 121645100408832000.;  // -1/19!
+  ps = fma(ps, z, 1. / 355687428096000.);
+  ps = fma(ps, z, -1. / 1307674368000.);
+  ps = fma(ps, z, 1. / 6227020800.);
+  ps = fma(ps, z, -1. / 39916800.);
+  ps = fma(ps, z, 1. / 362880.);
+  ps = fma(ps, z, -1. / 5040.);
+  ps = fma(ps, z, 1. / 120.);
+  ps = fma(ps, z, -1. / 6.);
+  const double sd = fma(ps * z, d, d);
+  double pc = 1. / 2432902008176640000.;  // 1/20!
+  pc = fma(pc, z, -1. / 6402373705728000.);
+  pc = fma(pc, z, 1. / 20922789888000.);
+  pc = fma(pc, z, -1. / 87178291200.);
+  pc = fma(pc, z, 1. / 479001600.);
+  pc = fma(pc, z, -1. / 3628800.);
+  pc = fma(pc, z, 1. / 40320.);
+  pc = fma(pc, z, -1. / 720.);
+  pc = fma(pc, z, 1. / 24.);
+  pc = fma(pc, z, -0.5);
+  const double cd = fma(pc, z, 1.);
+  *s = sx * cd + cx * sd;
+  *c = cx * cd - sx * sd;
+}
+template <bool ONE_SINCOS>
+PHX_DEV void theta_trig_sel(double x2, double hs, double* s2x, double* c2x, double* s, double* c) {
+  if (ONE_SINCOS)
+    theta_trig_1(x2, hs, s2x, c2x, s, c);
+  else
+    theta_trig(x2, hs, s2x, c2x, s, c);
+}
+template <bool ONE_SINCOS>
+PHX_DEV void theta_trig_sel(float x2, float hs, float* s2x, float* c2x, float* s, float* c) {
+  theta_trig(x2, hs, s2x, c2x, s, c);
+}
 
 // ------------------------------------------------------------------ metric
 // Kerr-Schild in MKS-BHAC coordinates (t, ln r, x2, phi), theta = x2 + h/2 sin 2x2
@@ -155,7 +202,7 @@ template <class T>
 struct Geo {
   T a, a2, r, r2, tp, tp2, tpp, s, c, s2, s2th, cot, Sig, Sb, Del, A, B, C;
 };
-template <class T>
+template <class T, bool ONE_SINCOS = false>
 PHX_DEV Geo<T> geo_at(T a, T hs, T x1, T x2) {
   Geo<T> G;
   G.a = a;
@@ -163,7 +210,7 @@ PHX_DEV Geo<T> geo_at(T a, T hs, T x1, T x2) {
   G.r = exp(x1);
   G.r2 = G.r * G.r;
   T s2x, c2x;
-  theta_trig(x2, hs, &s2x, &c2x, &G.s, &G.c);
+  theta_trig_sel<ONE_SINCOS>(x2, hs, &s2x, &c2x, &G.s, &G.c);
   G.tp = T(1) + hs * c2x;
   G.tp2 = G.tp * G.tp;
   G.tpp = -T(2) * hs * s2x;
@@ -232,7 +279,7 @@ PHX_DEV void two_sum_add(T& hi, T& lo, T d) {  // TwoSum + Fast2Sum (precision.h
 // RK4 of x, k (geodesic) and F1 (parallel transport) with the FP32 south-hemisphere
 // fold (geodesic.hpp:222-256).  The weighted stage sum k1 + 2 k2 + 2 k3 + k4 is
 // accumulated as the stages are formed, so only it and the next stage's offset are kept.
-template <class T>
+template <class T, bool ONE_SINCOS = false>
 PHX_DEV void rk4(T a, T hs, Ray<T>& g, T dt) {
   constexpr bool F32 = Lim<T>::fp32;
   const T pi_hi = T(PI_D), pi_lo = T(PI_D - (double)T(PI_D));
@@ -251,7 +298,7 @@ PHX_DEV void rk4(T a, T hs, Ray<T>& g, T dt) {
       y[10] = -y[10];
     }
     T d[12];
-    const Geo<T> G = geo_at(a, hs, y[1], y[2]);
+    const Geo<T> G = geo_at<T, ONE_SINCOS>(a, hs, y[1], y[2]);
 #pragma unroll
     for (int i = 0; i < 4; i++) d[i] = y[4 + i];
     chris(G, y + 4, y + 4, d + 4);
@@ -278,6 +325,52 @@ PHX_DEV void rk4(T a, T hs, Ray<T>& g, T dt) {
     else
       g.hi[i] += incr;
   }
+}
+// RK4 of the mixed-precision march: the state is held in FP64 and every stage is evaluated
+// in FP32 from the FP32 rounding of the state plus the stage offset; the weighted stage sum
+// is added to the FP64 state.  The south-hemisphere fold mirrors x2 about pi in FP64 first,
+// as the FP32 march does with its compensated pair.
+PHX_DEV void rk4_mixed(float a, float hs, double X[12], float dt) {
+  const bool south = X[2] > 0.5 * PI_D;
+  float base[12];
+#pragma unroll
+  for (int i = 0; i < 12; i++) base[i] = float(X[i]);
+  if (south) base[2] = float(PI_D - X[2]);
+  float sum[12], inc[12];
+#pragma unroll
+  for (int i = 0; i < 12; i++) inc[i] = 0.f;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    float y[12];
+#pragma unroll
+    for (int i = 0; i < 12; i++) y[i] = base[i] + inc[i];
+    if (south) {
+      y[2] = base[2] - inc[2];
+      y[6] = -y[6];
+      y[10] = -y[10];
+    }
+    float d[12];
+    const Geo<float> G = geo_at(a, hs, y[1], y[2]);
+#pragma unroll
+    for (int i = 0; i < 4; i++) d[i] = y[4 + i];
+    chris(G, y + 4, y + 4, d + 4);
+    chris(G, y + 4, y + 8, d + 8);
+    if (south) {
+      d[2] = -d[2];
+      d[6] = -d[6];
+      d[10] = -d[10];
+    }
+    const float wsum = (q == 0 || q == 3) ? 1.f : 2.f;
+#pragma unroll
+    for (int i = 0; i < 12; i++) sum[i] = q == 0 ? d[i] : sum[i] + wsum * d[i];
+    if (q == 3) break;
+    const float wdt = (q == 2 ? 1.f : 0.5f) * dt;
+#pragma unroll
+    for (int i = 0; i < 12; i++) inc[i] = wdt * d[i];
+  }
+  const float dt6 = dt / 6.f;
+#pragma unroll
+  for (int i = 0; i < 12; i++) X[i] += double(sum[i] * dt6);
 }
 template <class T>
 PHX_DEV T pole_x2(const Ray<T>& g) {  // geodesic.hpp:200-205
@@ -377,8 +470,8 @@ struct Fluid {
   Met<T> g;
 };
 template <class T>
-struct Plasma {  // per-request plasma parameters (src/core/model/bhac.hpp:150-151)
-  T a, hs, gam, Ne, Bu, rhigh, rlow, ibc, sigma_cut, te_min, te_max, rt_outer;
+struct Plasma {  // plasma parameters shared by every image of a launch (src/core/model/bhac.hpp:150-151)
+  T a, hs, gam, rlow, ibc, sigma_cut, te_min, te_max, rt_outer;
 };
 PHX_DEV void load8(const float* p, float v[8]) {
   const float4 a = *reinterpret_cast<const float4*>(p), b = *reinterpret_cast<const float4*>(p + 4);
@@ -393,32 +486,38 @@ PHX_DEV void load8(const double* p, double v[8]) {
 // get_fluid_params + derive_fluid (src/core/model/sampler.hpp:305-427): wrap, cached-leaf test,
 // re-wrap and lookup on a miss, cell by truncation, weights from the cell's lower corner,
 // block-local trilinear gather of the 8 primitives, then U^mu, b^mu, |B|, Theta_e and the cuts.
-template <class T>
-PHX_DEV int sample_fluid(const Grid<T>& G, const Plasma<T>& P, const T X[4], int cache, Fluid<T>& fl) {
-  constexpr bool F32 = Lim<T>::fp32;
-  T x[4] = {X[0], X[1], X[2], X[3]};
+// The cell is located in precision L (the block table GL) and the gather and the fluid
+// derivation run in T (the cell table G); L == T except in the mixed-precision march, which
+// locates the cell in FP64.  Ne, Bu, rhigh are the request's (per image) plasma units.
+template <class T, class L>
+PHX_DEV int sample_fluid(const Grid<T>& G, const Grid<L>& GL, const Plasma<T>& P, T Ne, T Bu, T rhigh, const L X[4],
+                         int cache, Fluid<T>& fl) {
+  constexpr bool F32 = Lim<L>::fp32;
+  L x[4] = {X[0], X[1], X[2], X[3]};
   wrap(x);
-  if (exp(x[1]) < T(1) || off_domain(G, x)) return 0;
+  // r < 1: exp(x1) < 1 (the FP64 locate of the mixed march tests x1 < 0, the same cut)
+  const bool inside_one = std::is_same<T, L>::value ? (exp(x[1]) < L(1)) : (x[1] < L(0));
+  if (inside_one || off_domain(GL, x)) return 0;
   int q = cache;
-  if (q == -1 || !in_leaf_closed(G, q, x)) {
+  if (q == -1 || !in_leaf_closed(GL, q, x)) {
     wrap(x);
-    q = find_leaf(G, x);
+    q = find_leaf(GL, x);
   }
 #pragma unroll
-  for (int i = 0; i < 4; i++) fl.x[i] = x[i];
+  for (int i = 0; i < 4; i++) fl.x[i] = T(x[i]);
   if (q == -1) return 0;
-  const T* b = G.blk + 12 * q;
+  const L* b = GL.blk + 12 * q;
   const int nxs[3] = {G.nx0, G.nx1, G.nx2};
   int c[3];
   T del[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    const T lb = b[d], dx = b[6 + d];
-    const T u = (x[1 + d] - lb) / dx;
+    const L lb = b[d], dx = b[6 + d];
+    const L u = (x[1 + d] - lb) / dx;
     c[d] = (int)u;
     if (c[d] >= nxs[d]) c[d] = nxs[d] - 1;
-    const int ix = F32 ? (int)floor(u) : (int)(u + T(1000)) - 1000;
-    del[d] = (ix < 0) ? T(0) : (ix > nxs[d] - 2) ? T(1) : (x[1 + d] - (T(ix) * dx + lb)) / dx;
+    const int ix = F32 ? (int)floor(u) : (int)(u + L(1000)) - 1000;
+    del[d] = (ix < 0) ? T(0) : (ix > nxs[d] - 2) ? T(1) : T((x[1 + d] - (L(ix) * dx + lb)) / dx);
   }
   const int ip = c[0] + 1 < G.nx0 ? c[0] + 1 : c[0];
   const int jp = c[1] + 1 < G.nx1 ? c[1] + 1 : c[1];
@@ -445,13 +544,13 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Plasma<T>& P, const T X[4], int
   }
   // derive_fluid
   const T tiny = T(1.e-6);
-  const Met<T> g = metric(P.a, P.hs, x[1], x[2]);
+  const Met<T> g = metric(P.a, P.hs, fl.x[1], fl.x[2]);
   const MetU<T> u = metric_up(P.a, g);
   fl.g = g;
   const T rho = qv[0], uu = qv[1];
   const T gV[4] = {T(0), qv[2], qv[3], qv[4]};
   const T Bp[4] = {T(0), qv[5], qv[6], qv[7]};
-  fl.ne = rho * P.Ne + tiny;
+  fl.ne = rho * Ne + tiny;
   const T shift1 = u.u01 / (-u.u00);  // shift^2 = shift^3 = 0 in Kerr-Schild
   const T alpha = T(1) / sqrt(-u.u00);
   const T vv = g.d11 * gV[1] * gV[1] + T(2) * g.d13 * gV[1] * gV[3] + g.d22 * gV[2] * gV[2] + g.d33 * gV[3] * gV[3];
@@ -468,11 +567,11 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Plasma<T>& P, const T X[4], int
   T bd[4];
   lower(g, fl.b, bd);
   const T Bsq = fabs(fl.b[0] * bd[0] + fl.b[1] * bd[1] + fl.b[2] * bd[2] + fl.b[3] * bd[3]) + tiny;
-  fl.B = sqrt(Bsq) * P.Bu;
+  fl.B = sqrt(Bsq) * Bu;
   const T beta = uu * (P.gam - T(1)) / (T(0.5) * (Bsq + tiny));
   const T bb = beta * P.ibc;
   const T b2 = bb * bb;
-  const T trat = P.rhigh * b2 / (T(1) + b2) + P.rlow / (T(1) + b2);
+  const T trat = rhigh * b2 / (T(1) + b2) + P.rlow / (T(1) + b2);
   const T te_unit = T(1. / 3. * (MP / ME)) / (trat + T(1));
   fl.te = (uu / rho) * te_unit;
   if ((Bsq / (rho + T(1e-20)) > P.sigma_cut) || g.r > P.rt_outer || fl.te > P.te_max || fl.te < P.te_min) {

@@ -1,0 +1,97 @@
+"""Batched renders (one persistent launch over the pixels of many uneven images) against
+each image rendered alone, the autotuner, and the mixed-precision march on a batch.
+
+Each ray is marched by the same per-ray code whether its image is alone or in a batch (the
+job queue only decides which lane takes it), so images whose frequency chunks select the
+same kernel instantiation must come out bit-identical; across instantiations (a batch whose
+widest request has more frequencies per lane) they are compared with the parity gates.
+"""
+import numpy as np
+import pytest
+
+from phoenix_amd import capi, configs, snapshot as S
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [  # (W, H, frequencies, inclination, m_unit, R_high)
+    (16, 16, (230e9,), 10.0, 1.2e18, 3.0),
+    (32, 24, (86e9,), 60.0, 5e17, 20.0),
+    (8, 8, (345e9,), 163.0, 3e18, 1.5),
+    (24, 24, (150e9,), 120.0, 2e18, 40.0),
+]
+SPECS_NF = [
+    (16, 16, (230e9, 345e9), 30.0, 1.2e18, 3.0),
+    (8, 12, (86e9, 150e9, 230e9, 345e9, 690e9), 80.0, 8e17, 10.0),
+    (12, 8, (230e9,), 140.0, 4e18, 25.0),
+]
+
+
+@pytest.fixture(scope="module")
+def c1(snapdir, tmp_path_factory):
+    dump, grid, _ = S.ensure_snapshot("c1", snapdir)
+    cfg = configs.write_config(str(tmp_path_factory.mktemp("b") / "c1.toml"), dump, grid)
+    run = capi.Run(cfg)
+    snap = capi.Snapshot(run, capi.TABLE_FP32 | capi.TABLE_FP64)
+    yield run, snap
+    snap.close()
+    run.close()
+
+
+def _requests(run, specs):
+    return [run.request(img_width=w, img_height=h, freqs=f, inclination=i, m_unit=m, r_high=rh)
+            for (w, h, f, i, m, rh) in specs]
+
+
+@pytest.mark.parametrize("precision", [capi.FP32, capi.MIXED, capi.FP64])
+def test_batch_is_bitwise_the_images_alone(c1, precision):
+    run, snap = c1
+    reqs = _requests(run, SPECS)
+    batch, info = snap.render_batch(reqs, precision)
+    assert info["rays"] == sum(w * h * len(f) for (w, h, f, *_ ) in SPECS)
+    steps = 0
+    for rq, rb in zip(reqs, batch):
+        alone, ia = snap.render(rq, precision)
+        steps += ia["steps"]
+        assert np.array_equal(rb, alone), f"{rq.img_width}x{rq.img_height} i={rq.inclination}: batch != alone"
+    assert info["steps"] == steps
+
+
+@pytest.mark.parametrize("precision", [capi.FP32, capi.MIXED])
+def test_batch_mixed_frequency_counts(c1, precision):
+    run, snap = c1
+    reqs = _requests(run, SPECS_NF)
+    batch, _ = snap.render_batch(reqs, precision)
+    for rq, rb in zip(reqs, batch):
+        alone, _ = snap.render(rq, precision)
+        figs = parity.compare(rb, alone)
+        exact = np.array_equal(rb, alone)
+        print(f"\n{rq.img_width}x{rq.img_height} nf={rq.nfreqs} fp{precision}: exact={exact}; {parity.summary(figs)}")
+        assert parity.passes(figs), parity.summary(figs)
+
+
+def test_batch_vs_fp64_judge(c1):
+    """Every image of a mixed-precision batch against the same images rendered in FP64."""
+    run, snap = c1
+    reqs = _requests(run, SPECS + SPECS_NF)
+    mixed, _ = snap.render_batch(reqs, capi.MIXED)
+    judge, _ = snap.render_batch(reqs, capi.FP64)
+    for rq, a, b in zip(reqs, mixed, judge):
+        figs = parity.compare(a, b)
+        print(f"\n{rq.img_width}x{rq.img_height} i={rq.inclination} nf={rq.nfreqs}: {parity.summary(figs)}")
+        assert parity.passes(figs), parity.summary(figs)
+
+
+def test_autotune_picks_a_valid_configuration(c1):
+    run, snap = c1
+    reqs = _requests(run, SPECS + SPECS_NF)
+    ref, _ = snap.render_batch(reqs, capi.FP32)
+    tune, rate = snap.autotune(reqs, capi.FP32, sample_stride=2)
+    print(f"\nautotune: {tune} at {rate:.3g} rays*freq/s (sampled)")
+    assert tune["queue"] in (0, 1) and tune["blocks_per_cu"] >= 1 and tune["order"] in (0, 1)
+    assert rate > 0
+    again, _ = snap.render_batch(reqs, capi.FP32)
+    for rq, a, b in zip(reqs, again, ref):
+        figs = parity.compare(a, b)
+        assert parity.passes(figs), parity.summary(figs)
+    snap.set_tune()  # back to the defaults for later tests
