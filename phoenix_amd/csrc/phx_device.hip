@@ -116,13 +116,16 @@ struct StRef {
 // The FP64 RK4 step of the mixed march: classical RK4 on the FP64 state (the same arithmetic
 // as the FP64 march's rk4), written so that only the state, the stage sum, the stage point
 // and the stage derivative are live (the next stage point is formed as soon as a stage's
-// derivative is known), with one library sincos per stage.  Out of line: it runs on a minority
-// of steps, and inlined its FP64 temporaries push the FP32 bulk of the kernel into scratch.
-__device__ __noinline__ void rk4_fp64_lean(double a, double hs, double X[12], double dt) {
+// derivative is known), with one library sincos per stage and the four stages kept as a loop:
+// it runs inline in the FP32 kernel, where an unrolled FP64 step or a call would spill.
+PHX_DEV void rk4_fp64_lean(double a, double hs, double X[12], double dt) {
   double y[12], sum[12];
 #pragma unroll
-  for (int i = 0; i < 12; i++) y[i] = X[i];
-#pragma unroll
+  for (int i = 0; i < 12; i++) {
+    y[i] = X[i];
+    sum[i] = 0.;
+  }
+#pragma unroll 1
   for (int q = 0; q < 4; q++) {
     double d[12];
     const Geo<double> G = geo_at<double, true>(a, hs, y[1], y[2]);
@@ -131,12 +134,12 @@ __device__ __noinline__ void rk4_fp64_lean(double a, double hs, double X[12], do
     chris(G, y + 4, y + 4, d + 4);
     chris(G, y + 4, y + 8, d + 8);
     const double wsum = (q == 0 || q == 3) ? 1. : 2.;
-#pragma unroll
-    for (int i = 0; i < 12; i++) sum[i] = q == 0 ? d[i] : sum[i] + wsum * d[i];
-    if (q == 3) break;
     const double wdt = (q == 2 ? 1.0 : 0.5) * dt;
 #pragma unroll
-    for (int i = 0; i < 12; i++) y[i] = X[i] + wdt * d[i];
+    for (int i = 0; i < 12; i++) {
+      sum[i] = q == 0 ? d[i] : sum[i] + wsum * d[i];
+      y[i] = X[i] + wdt * d[i];
+    }
   }
   const double dt6 = dt / 6.;
 #pragma unroll
@@ -144,36 +147,46 @@ __device__ __noinline__ void rk4_fp64_lean(double a, double hs, double X[12], do
 }
 struct StMix {
   static constexpr int REC = 16;  // start record (doubles): x, k, F1, sign of beta
-  double X[12];
+  // the FP64 state lives in LDS, component-major ([12][BLOCK], conflict-free), so that it holds
+  // no VGPRs outside the step that updates it
+  double* Xs = nullptr;
+  PHX_DEV double x(int q) const { return Xs[q * BLOCK]; }
   PHX_DEV void load(const double* st, float* bsign) {
 #pragma unroll
-    for (int q = 0; q < 12; q++) X[q] = st[q];
+    for (int q = 0; q < 12; q++) Xs[q * BLOCK] = st[q];
     *bsign = float(st[12]);
   }
   PHX_DEV void view(float cx[4], float ck[4], float cf[4]) const {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      cx[q] = float(X[q]);
-      ck[q] = float(X[4 + q]);
-      cf[q] = float(X[8 + q]);
+      cx[q] = float(x(q));
+      ck[q] = float(x(4 + q));
+      cf[q] = float(x(8 + q));
     }
   }
   PHX_DEV void lpos(const float*, double pl[4]) const {
 #pragma unroll
-    for (int q = 0; q < 4; q++) pl[q] = X[q];
+    for (int q = 0; q < 4; q++) pl[q] = x(q);
   }
-  PHX_DEV float x2p() const { return float(X[2] > 0.5 * PI_D ? PI_D - X[2] : X[2]); }
+  PHX_DEV float x2p() const {
+    const double x2 = x(2);
+    return float(x2 > 0.5 * PI_D ? PI_D - x2 : x2);
+  }
   // returns whether the step ran in FP64; WAVE: one lane's FP64 step takes its whole wave to
   // FP64 (the wave runs one step code instead of both)
   template <class A>
   PHX_DEV bool step(const A& args, float dl) {
+    double X[12];
+#pragma unroll
+    for (int q = 0; q < 12; q++) X[q] = x(q);
     bool fp64 = X[1] < args.lnr_fp64 || X[2] < args.x2_pole || X[2] > PI_D - args.x2_pole;
     if (args.fp64_wave) fp64 = __ballot(fp64) != 0ull;
-    if (fp64) {
+    if (fp64)
       rk4_fp64_lean(args.a64, args.hs64, X, double(dl));
-    } else {
+    else
       rk4_mixed(args.P.a, args.P.hs, X, dl);
-    }
+#pragma unroll
+    for (int q = 0; q < 12; q++) Xs[q * BLOCK] = X[q];
     return fp64;
   }
 };
@@ -311,10 +324,12 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
   using R = RecOf<M>;
   constexpr bool KAHAN = M == MIX;
   __shared__ __align__(16) T qbuf[QUEUE ? BLOCK / 64 : 1][QUEUE ? QCAP * QW : 1];
+  __shared__ double xstate[M == MIX ? 12 * BLOCK : 1];
   T* const Q = qbuf[QUEUE ? (threadIdx.x >> 6) : 0];
   const int lane = threadIdx.x & 63;
   const unsigned long long lt = (1ull << lane) - 1ull;
   St st;
+  if constexpr (M == MIX) st.Xs = xstate + threadIdx.x;
   T px[4] = {T(0), T(0), T(0), T(0)}, pk[4] = {T(0), T(0), T(0), T(0)}, pf[4] = {T(0), T(0), T(0), T(0)};
   L pl[4] = {L(0), L(0), L(0), L(0)};
   T pdl = T(0);

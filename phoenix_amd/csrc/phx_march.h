@@ -91,12 +91,50 @@ PHX_DEV void theta_trig(double x2, double hs, double* s2x, double* c2x, double* 
   sincos(2. * x2, s2x, c2x);
   sincos(x2 + 0.5 * hs * *s2x, s, c);
 }
-// FP64 with one library sincos: sin/cos of x2, the double-angle identities, and theta = x2 +
+// FP64 sin and cos of a small |x| (< ~1e3): Cody-Waite reduction by pi/2 with a two-part
+// constant (fma), Taylor polynomials on [-pi/4, pi/4] (sin to r^19, cos to r^20: truncation
+// below 1e-19), quadrant from the low bits of the quotient.  Compact, so that the FP64 steps of
+// the mixed march keep few registers (the library's sincos carries a large-argument path).
+PHX_DEV void sincos_small(double x, double* s, double* c) {
+  const double q = rint(x * 0.63661977236758134308);
+  const int qi = (int)q;
+  double r = fma(-q, 1.5707963267948966192, x);
+  r = fma(-q, 6.123233995736766036e-17, r);
+  const double z = r * r;
+  double ps = -1. / 121645100408832000.;
+  ps = fma(ps, z, 1. / 355687428096000.);
+  ps = fma(ps, z, -1. / 1307674368000.);
+  ps = fma(ps, z, 1. / 6227020800.);
+  ps = fma(ps, z, -1. / 39916800.);
+  ps = fma(ps, z, 1. / 362880.);
+  ps = fma(ps, z, -1. / 5040.);
+  ps = fma(ps, z, 1. / 120.);
+  ps = fma(ps, z, -1. / 6.);
+  const double sr = fma(ps * z, r, r);
+  double pc = 1. / 2432902008176640000.;
+  pc = fma(pc, z, -1. / 6402373705728000.);
+  pc = fma(pc, z, 1. / 20922789888000.);
+  pc = fma(pc, z, -1. / 87178291200.);
+  pc = fma(pc, z, 1. / 479001600.);
+  pc = fma(pc, z, -1. / 3628800.);
+  pc = fma(pc, z, 1. / 40320.);
+  pc = fma(pc, z, -1. / 720.);
+  pc = fma(pc, z, 1. / 24.);
+  pc = fma(pc, z, -0.5);
+  const double cr = fma(pc, z, 1.);
+  double sn = (qi & 1) ? cr : sr, cs = (qi & 1) ? sr : cr;
+  if (qi & 2) sn = -sn;
+  if ((qi + 1) & 2) cs = -cs;
+  *s = sn;
+  *c = cs;
+}
+
+// FP64 with one compact sincos: sin/cos of x2, the double-angle identities, and theta = x2 +
 // delta by angle addition with sin/cos of |delta| <= h/2 <= 0.5 in Taylor series to delta^19 /
 // delta^20 (truncation below 2e-23).  Used by the FP64 steps of the mixed march.
 PHX_DEV void theta_trig_1(double x2, double hs, double* s2x, double* c2x, double* s, double* c) {
   double sx, cx;
-  sincos(x2, &sx, &cx);
+  sincos_small(x2, &sx, &cx);
   *s2x = 2. * sx * cx;
   *c2x = (cx - sx) * (cx + sx);
   const double d = 0.5 * hs * *s2x, z = d * d;
