@@ -139,6 +139,22 @@ int phx_schedule(const phx_model* model, int32_t morton, int32_t* jobs, int32_t 
 /* host::output::compute_spec (src/host/io/output.hpp:43-53): flux [Jy] of frequency fi, non-finite pixels skipped */
 int phx_compute_spec(const phx_model* model, const phx_freqout* res, int32_t fi, double S[4]);
 
+/* host::output::compute_tau (src/host/io/output.hpp:58-71): mean tau, tauF over the finite pixels */
+int phx_compute_tau(const phx_model* model, const phx_freqout* res, int32_t fi, double T[2]);
+/* host::output::write_image (output.hpp:77-94): text image of frequency fi, one row per pixel
+ * (i outer): x, y [arcsec], I, Q, U, V [Jy / pixel], tau, tauF */
+int phx_write_image_text(const phx_model* model, const phx_freqout* res, int32_t fi, const char* path);
+/* host::output::write_spectrum (output.hpp:99-111): nu, I, Q, U, V [Jy], mean tau, mean tauF per frequency */
+int phx_write_spectrum(const phx_model* model, const phx_freqout* res, const char* path);
+/* host::output::write_image_hdf5 (output.hpp:144-167): RAPTOR-layout HDF5 image (datasets I%e, tau%e,
+ * Q%e, U%e, V%e, tauF%e per frequency, alpha, beta; [nblocks][block_size^2] doubles, Jy/sr).  The HDF5
+ * library (>= 1.10) is loaded at run time (PHX_HDF5_LIB, libhdf5.so, /opt/conda/lib/libhdf5.so). */
+int phx_write_image_hdf5(const phx_model* model, const phx_freqout* res, int32_t block_size, const char* path);
+/* write_output of src/main.cpp:52-86: outdir/img_data.h5 and/or outdir/image[_f].dat per output.image
+ * (text when HDF5 fails), then outdir/spectrum.dat; the messages the reference prints go to log */
+int phx_write_output(const phx_model* model, const phx_config* cfg, const phx_freqout* res, char* log,
+                     int32_t log_len);
+
 /* ---------------------------------------------------------------- device path */
 
 int phx_set_device(int32_t device);

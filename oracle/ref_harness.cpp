@@ -12,6 +12,7 @@
 // needs hdf5.h, which is not part of the path).
 //
 // Usage: phx_ref_fpXX config.toml out.bin [section.key=value ...]
+//        phx_ref_fpXX config.toml --config-only [section.key=value ...]   (parsed keys or "ERROR msg")
 //        phx_ref_fpXX config.toml out.bin --pixels first,count[,stride] ...
 //        (pixel subset first + k*stride, k < count)
 // out.bin: int32 {W, H, nf, 0} int64 {steps, nplasma, bad} f64 {raymarch_s}
@@ -46,6 +47,30 @@ int main(int argc, char** argv) {
   }
   core::bhac::Model model = {};
   host::config::Params params;
+  if (outp == "--config-only") {  // host::config::load_config alone: the parsed keys, or the error
+    try {
+      host::config::load_config(cfg, ov, params, &model.uparams);
+    } catch (const std::exception& e) {
+      std::printf("ERROR %s\n", e.what());
+      return 1;
+    }
+    const core::bhac::UserParams& u = model.uparams;
+    std::printf("inclination %.17g\nimg_width %d\nimg_height %d\nfov_x %.17g\nfov_y %.17g\nr_cam %.17g\n",
+                u.inclination, u.img_width, u.img_height, u.fov_x, u.fov_y, u.r_cam);
+    std::printf("t_init %.17g\npa %.17g\nphi_cam %.17g\nnfreqs %d\n", u.t_init, u.pa, u.phi_cam, u.nfreqs);
+    for (int f = 0; f < u.nfreqs; f++) std::printf("frequency%d %.17g\n", f, u.frequencies[f]);
+    std::printf("mass %.17g\ndistance %.17g\nstepsize %.17g\nmax_steps %d\nhorizon_margin %.17g\nmax_order %d\n",
+                u.mass, u.distance, (double)u.stepsize, u.max_steps, u.horizon_margin, u.max_order);
+    std::printf("m_unit %.17g\nr_low %.17g\nr_high %.17g\nbeta_c %.17g\nsigma_cut %.17g\n", u.m_unit, u.r_low,
+                u.r_high, u.beta_c, (double)u.sigma_cut);
+    std::printf("thetae_min %.17g\nthetae_max %.17g\nrt_outer %.17g\n", (double)u.thetae_min,
+                (double)u.thetae_max, (double)u.rt_outer);
+    std::printf("nspin %d\nbpol %d\nnlevels %d\nsfc %d\n", u.nspin, u.bpol, u.nlevels, (int)u.sfc);
+    std::printf("outdir %s\nimage %s\nblock_size %d\ndump_file %s\ngrid_file %s\ngpu_decode %d\nshow_header %d\n",
+                params.outdir.c_str(), params.image.c_str(), params.block_size, params.dump_file.c_str(),
+                params.grid_file.c_str(), (int)params.gpu_decode, (int)params.show_header);
+    return 0;
+  }
   try {
     host::config::load_config(cfg, ov, params, &model.uparams);
     host::bhac_io::load_bhac_grid(&model.gparams, params.grid_file);

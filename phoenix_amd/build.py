@@ -19,6 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libphx.so")
+CLI = os.path.join(HERE, "phoenix")
 BUILD = os.path.join(HERE, "_build")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -45,8 +46,12 @@ def build(verbose: bool = False, out: str = LIB, extra_hip_flags=(), device_src:
     _run([HIPCC, *HIP_FLAGS, *extra_hip_flags, "-c", device_src or os.path.join(CSRC, "phx_device.hip"), "-o", dev_o])
     _run([os.environ.get("CXX", "g++"), *CXX_FLAGS, "-c", os.path.join(CSRC, "phx_host.cpp"), "-o", host_o])
     tmp = out + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, dev_o, host_o])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, dev_o, host_o, "-ldl"])
     os.replace(tmp, out)
+    if out == LIB:  # the command-line driver, linked against the library next to it
+        _run([os.environ.get("CXX", "g++"), *CXX_FLAGS, os.path.join(CSRC, "phx_cli.cpp"), "-o", CLI + ".tmp",
+              f"-L{HERE}", "-lphx", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"])
+        os.replace(CLI + ".tmp", CLI)
     if verbose:
         print("built", out)
     return out

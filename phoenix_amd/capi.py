@@ -116,6 +116,12 @@ SIGNATURES = {
     "phx_free_model": (None, [ctypes.POINTER(Model)]),
     "phx_schedule": (c_int, [ctypes.POINTER(Model), c_int, P_int, c_int]),
     "phx_compute_spec": (c_int, [ctypes.POINTER(Model), ctypes.POINTER(FreqOut), c_int, P_dbl]),
+    "phx_compute_tau": (c_int, [ctypes.POINTER(Model), ctypes.POINTER(FreqOut), c_int, P_dbl]),
+    "phx_write_image_text": (c_int, [ctypes.POINTER(Model), ctypes.POINTER(FreqOut), c_int, ctypes.c_char_p]),
+    "phx_write_spectrum": (c_int, [ctypes.POINTER(Model), ctypes.POINTER(FreqOut), ctypes.c_char_p]),
+    "phx_write_image_hdf5": (c_int, [ctypes.POINTER(Model), ctypes.POINTER(FreqOut), c_int, ctypes.c_char_p]),
+    "phx_write_output": (c_int, [ctypes.POINTER(Model), ctypes.POINTER(Config), ctypes.POINTER(FreqOut),
+                                 ctypes.c_char_p, c_int]),
     "phx_set_device": (c_int, [c_int]),
     "phx_dev_warmup": (c_int, []),
     "phx_dev_render": (c_int, [ctypes.POINTER(Model), ctypes.c_char_p, c_int, P_int, c_int, ctypes.POINTER(FreqOut),
@@ -231,6 +237,25 @@ class Run:
         check(lib().phx_dev_render(ctypes.byref(m), self.config.dump_file, precision, jobs.ctypes.data_as(P_int),
                                    jobs.size, rp, ctypes.byref(info), None, None))
         return res.reshape(m.img_width * m.img_height, m.nfreqs, 6), info.as_dict()
+
+    def write_output(self, res: np.ndarray, outdir: str | None = None, image: str | None = None,
+                     block_size: int | None = None, model: Model | None = None) -> str:
+        """phx_write_output: the reference's write_output (HDF5 and/or text image, spectrum).
+        Returns the messages it would print."""
+        m = model or self.model
+        cfg = Config()
+        ctypes.pointer(cfg)[0] = self.config
+        if outdir is not None:
+            cfg.outdir = outdir.encode()
+        if image is not None:
+            cfg.image = image.encode()
+        if block_size is not None:
+            cfg.block_size = block_size
+        r = np.ascontiguousarray(res.reshape(-1, 6), dtype=np.float64)
+        log = ctypes.create_string_buffer(8192)
+        check(lib().phx_write_output(ctypes.byref(m), ctypes.byref(cfg), r.ctypes.data_as(ctypes.POINTER(FreqOut)),
+                                     log, len(log)))
+        return log.value.decode()
 
     def flux(self, res: np.ndarray, model: Model | None = None) -> np.ndarray:
         m = model or self.model

@@ -1241,9 +1241,9 @@ static void morton_pixels(const phx_model* rq, std::vector<int32_t>& out) {
   if (W == H && (W & (W - 1)) == 0 && W <= 65536) {
     for (int64_t k = 0; k < n; k++) {
       uint32_t i = 0, j = 0;
-      for (int b = 0; b < 16; b++) {
-        j |= (uint32_t)((k >> (2 * b)) & 1) << b;
-        i |= (uint32_t)((k >> (2 * b + 1)) & 1) << b;
+      for (int b = 0; b < 16; b++) {  // i in the even bits, j in the odd bits (jobs.hpp:33-35)
+        i |= (uint32_t)((k >> (2 * b)) & 1) << b;
+        j |= (uint32_t)((k >> (2 * b + 1)) & 1) << b;
       }
       out[k] = (int32_t)((int64_t)i * H + j);
     }

@@ -2,7 +2,11 @@
 // schedule and flux reduction.  These keep the reference's behaviour (keys,
 // file formats, error cases); the raymarch itself is device-only
 // (phx_device.hip).
+#include <dlfcn.h>
+#include <sys/stat.h>
+
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -10,6 +14,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <set>
 #include <sstream>
 #include <string>
@@ -546,4 +551,249 @@ int phx_compute_spec(const phx_model* m, const phx_freqout* res, int32_t fi, dou
   });
 }
 
+
+// ---------------------------------------------------------------- output (src/host/io/output.hpp)
+}  // extern "C"
+
+namespace {
+
+const phx_freqout& result_at(const phx_model* m, const phx_freqout* res, int i, int j, int f) {
+  return res[((size_t)i * m->img_height + j) * m->nfreqs + f];
+}
+// camera::pixel_impact (src/core/geometry/camera.hpp:11-23)
+void impact(const phx_model* m, int i, int j, double* a, double* b) {
+  *a = -0.5 * m->fov_x + (i + 0.5) * m->fov_x / m->img_width;
+  *b = -0.5 * m->fov_y + (j + 0.5) * m->fov_y / m->img_height;
+}
+// camera::pixel_solid_angle (camera.hpp:28-32)
+double solid_angle(const phx_model* m) {
+  const double dist = m->distance * K_KPC;
+  return (m->fov_x / m->img_width * m->L / dist) * (m->fov_y / m->img_height * m->L / dist);
+}
+
+// The HDF5 C library, loaded at run time (the image's conda HDF5 >= 1.10, where hid_t is 64-bit),
+// so that libphx itself has no link dependency on it.
+struct H5 {
+  typedef int64_t hid_t;
+  typedef unsigned long long hsize_t;
+  int (*open)() = nullptr;
+  int (*get_libversion)(unsigned*, unsigned*, unsigned*) = nullptr;
+  hid_t (*Fcreate)(const char*, unsigned, hid_t, hid_t) = nullptr;
+  int (*Fclose)(hid_t) = nullptr;
+  hid_t (*Screate_simple)(int, const hsize_t*, const hsize_t*) = nullptr;
+  int (*Sclose)(hid_t) = nullptr;
+  hid_t (*Dcreate2)(hid_t, const char*, hid_t, hid_t, hid_t, hid_t, hid_t) = nullptr;
+  int (*Dwrite)(hid_t, hid_t, hid_t, hid_t, hid_t, const void*) = nullptr;
+  int (*Dclose)(hid_t) = nullptr;
+  hid_t* native_double = nullptr;
+  bool ok = false;
+  std::string why;
+};
+
+const H5& hdf5() {
+  static H5 h;
+  static bool tried = false;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  if (tried) return h;
+  tried = true;
+  const char* names[] = {std::getenv("PHX_HDF5_LIB"), "libhdf5.so", "libhdf5.so.103", "/opt/conda/lib/libhdf5.so",
+                         "/usr/lib/x86_64-linux-gnu/hdf5/serial/libhdf5.so"};
+  void* lib = nullptr;
+  for (const char* n : names)
+    if (n && (lib = dlopen(n, RTLD_NOW | RTLD_LOCAL))) break;
+  if (!lib) {
+    h.why = "no HDF5 library found (set PHX_HDF5_LIB)";
+    return h;
+  }
+  auto sym = [&](const char* n) { return dlsym(lib, n); };
+  h.open = (int (*)())sym("H5open");
+  h.get_libversion = (int (*)(unsigned*, unsigned*, unsigned*))sym("H5get_libversion");
+  h.Fcreate = (H5::hid_t(*)(const char*, unsigned, H5::hid_t, H5::hid_t))sym("H5Fcreate");
+  h.Fclose = (int (*)(H5::hid_t))sym("H5Fclose");
+  h.Screate_simple = (H5::hid_t(*)(int, const H5::hsize_t*, const H5::hsize_t*))sym("H5Screate_simple");
+  h.Sclose = (int (*)(H5::hid_t))sym("H5Sclose");
+  h.Dcreate2 = (H5::hid_t(*)(H5::hid_t, const char*, H5::hid_t, H5::hid_t, H5::hid_t, H5::hid_t, H5::hid_t))sym(
+      "H5Dcreate2");
+  h.Dwrite = (int (*)(H5::hid_t, H5::hid_t, H5::hid_t, H5::hid_t, H5::hid_t, const void*))sym("H5Dwrite");
+  h.Dclose = (int (*)(H5::hid_t))sym("H5Dclose");
+  h.native_double = (H5::hid_t*)sym("H5T_NATIVE_DOUBLE_g");
+  if (!h.open || !h.get_libversion || !h.Fcreate || !h.Fclose || !h.Screate_simple || !h.Sclose || !h.Dcreate2 ||
+      !h.Dwrite || !h.Dclose || !h.native_double) {
+    h.why = "HDF5 library lacks a needed symbol";
+    return h;
+  }
+  unsigned maj = 0, min = 0, rel = 0;
+  if (h.open() < 0 || h.get_libversion(&maj, &min, &rel) < 0 || maj != 1 || min < 10) {
+    h.why = "HDF5 1.10 or newer needed (64-bit hid_t)";
+    return h;
+  }
+  h.ok = true;
+  return h;
+}
+
+enum { W_I, W_Q, W_U, W_V, W_TAU, W_TAUF, W_ALPHA, W_BETA };
+// output.hpp:117-126
+double h5_value(const phx_model* m, const phx_freqout* res, int what, int f, int i, int j) {
+  if (what <= W_V) return result_at(m, res, i, j, f).IQUV[what] * K_JY;
+  if (what == W_TAU) return result_at(m, res, i, j, f).tau;
+  if (what == W_TAUF) return result_at(m, res, i, j, f).tauF;
+  double a, b;
+  impact(m, i, j, &a, &b);
+  return what == W_ALPHA ? a : b;
+}
+// one [nblocks][bs*bs] dataset in the RAPTOR block layout (output.hpp:128-142): block b = bi * nbj + bj,
+// pixel p = pi * bs + pj, image pixel (i, j) = (bi * bs + pi, bj * bs + pj)
+void h5_put(const H5& h, H5::hid_t file, const std::string& name, const phx_model* m, int bs, const phx_freqout* res,
+            int what, int f, std::vector<double>& buf) {
+  const int nbj = m->img_height / bs, nblocks = (m->img_width / bs) * nbj, npb = bs * bs;
+  for (int b = 0; b < nblocks; b++)
+    for (int p = 0; p < npb; p++) {
+      const int i = (b / nbj) * bs + p / bs, j = (b % nbj) * bs + p % bs;
+      buf[(size_t)b * npb + p] = h5_value(m, res, what, f, i, j);
+    }
+  const H5::hsize_t dims[2] = {(H5::hsize_t)nblocks, (H5::hsize_t)npb};
+  const H5::hid_t sp = h.Screate_simple(2, dims, nullptr);
+  if (sp < 0) throw Error(PHX_E_IO, "H5Screate_simple failed");
+  const H5::hid_t ds = h.Dcreate2(file, name.c_str(), *h.native_double, sp, 0, 0, 0);
+  if (ds < 0) {
+    h.Sclose(sp);
+    throw Error(PHX_E_IO, "H5Dcreate2 failed for " + name);
+  }
+  const int rc = h.Dwrite(ds, *h.native_double, 0, 0, 0, buf.data());
+  h.Dclose(ds);
+  h.Sclose(sp);
+  if (rc < 0) throw Error(PHX_E_IO, "H5Dwrite failed for " + name);
+}
+
+void check_out_args(const phx_model* m, const phx_freqout* res, const char* path) {
+  if (!m || !res || !path) throw Error(PHX_E_ARG, "null model, result or path");
+  if (m->img_width <= 0 || m->img_height <= 0 || m->nfreqs < 1) throw Error(PHX_E_ARG, "empty image");
+}
+
+}  // namespace
+
+extern "C" {
+
+int phx_compute_tau(const phx_model* m, const phx_freqout* res, int32_t fi, double T[2]) {
+  return guarded([&] {
+    check_out_args(m, res, "");
+    if (fi < 0 || fi >= m->nfreqs) throw Error(PHX_E_ARG, "frequency index out of range");
+    double s0 = 0., s1 = 0.;
+    size_t n = 0;
+    for (int i = 0; i < m->img_width; i++)
+      for (int j = 0; j < m->img_height; j++) {
+        const phx_freqout& r = result_at(m, res, i, j, fi);
+        if (!std::isfinite(r.IQUV[0] + r.IQUV[1] + r.IQUV[2] + r.IQUV[3] + r.tau + r.tauF)) continue;
+        s0 += r.tau;
+        s1 += r.tauF;
+        n++;
+      }
+    T[0] = n ? s0 / (double)n : 0.;
+    T[1] = n ? s1 / (double)n : 0.;
+  });
+}
+
+int phx_write_image_text(const phx_model* m, const phx_freqout* res, int32_t fi, const char* path) {
+  return guarded([&] {
+    check_out_args(m, res, path);
+    if (fi < 0 || fi >= m->nfreqs) throw Error(PHX_E_ARG, "frequency index out of range");
+    FILE* f = std::fopen(path, "w");
+    if (!f) throw Error(PHX_E_IO, std::string("cannot open ") + path);
+    const double unit = K_JY * solid_angle(m);
+    const double arcsec = 206265.0 * m->L / (m->distance * K_KPC);
+    for (int i = 0; i < m->img_width; i++)
+      for (int j = 0; j < m->img_height; j++) {
+        double a, b;
+        impact(m, i, j, &a, &b);
+        const phx_freqout& r = result_at(m, res, i, j, fi);
+        std::fprintf(f, "%+.15e\t%+.15e\t%+.15e\t%+.15e\t%+.15e\t%+.15e\t%+.15e\t%+.15e\n", a * arcsec, b * arcsec,
+                     r.IQUV[0] * unit, r.IQUV[1] * unit, r.IQUV[2] * unit, r.IQUV[3] * unit, r.tau, r.tauF);
+      }
+    if (std::fclose(f) != 0) throw Error(PHX_E_IO, std::string("write failed: ") + path);
+  });
+}
+
+int phx_write_spectrum(const phx_model* m, const phx_freqout* res, const char* path) {
+  return guarded([&] {
+    check_out_args(m, res, path);
+    FILE* f = std::fopen(path, "w");
+    if (!f) throw Error(PHX_E_IO, std::string("cannot open ") + path);
+    for (int fi = 0; fi < m->nfreqs; fi++) {
+      double S[4], T[2];
+      if (phx_compute_spec(m, res, fi, S) != PHX_OK || phx_compute_tau(m, res, fi, T) != PHX_OK) {
+        std::fclose(f);
+        throw Error(PHX_E_ARG, phx_last_error());
+      }
+      std::fprintf(f, "%+.15e\t%+.15e\t%+.15e\t%+.15e\t%+.15e\t%+.15e\t%+.15e\n", m->frequencies[fi], S[0], S[1],
+                   S[2], S[3], T[0], T[1]);
+    }
+    if (std::fclose(f) != 0) throw Error(PHX_E_IO, std::string("write failed: ") + path);
+  });
+}
+
+int phx_write_image_hdf5(const phx_model* m, const phx_freqout* res, int32_t bs, const char* path) {
+  return guarded([&] {
+    check_out_args(m, res, path);
+    if (bs <= 0 || m->img_width % bs || m->img_height % bs)
+      throw Error(PHX_E_ARG, "output.block_size " + std::to_string(bs) + " must divide the image size " +
+                                 std::to_string(m->img_width) + "x" + std::to_string(m->img_height));
+    const H5& h = hdf5();
+    if (!h.ok) throw Error(PHX_E_IO, h.why);
+    const H5::hid_t file = h.Fcreate(path, 0x0002u /* H5F_ACC_TRUNC */, 0, 0);
+    if (file < 0) throw Error(PHX_E_IO, std::string("cannot create ") + path);
+    try {
+      std::vector<double> buf((size_t)(m->img_width / bs) * (m->img_height / bs) * bs * bs);
+      for (int fi = 0; fi < m->nfreqs; fi++) {
+        char fb[64];
+        std::snprintf(fb, sizeof fb, "%e", m->frequencies[fi]);
+        const std::string fs = fb;
+        h5_put(h, file, "I" + fs, m, bs, res, W_I, fi, buf);
+        h5_put(h, file, "tau" + fs, m, bs, res, W_TAU, fi, buf);
+        h5_put(h, file, "Q" + fs, m, bs, res, W_Q, fi, buf);
+        h5_put(h, file, "U" + fs, m, bs, res, W_U, fi, buf);
+        h5_put(h, file, "V" + fs, m, bs, res, W_V, fi, buf);
+        h5_put(h, file, "tauF" + fs, m, bs, res, W_TAUF, fi, buf);
+      }
+      h5_put(h, file, "alpha", m, bs, res, W_ALPHA, 0, buf);
+      h5_put(h, file, "beta", m, bs, res, W_BETA, 0, buf);
+    } catch (...) {
+      h.Fclose(file);
+      throw;
+    }
+    if (h.Fclose(file) < 0) throw Error(PHX_E_IO, std::string("H5Fclose failed: ") + path);
+  });
+}
+
+int phx_write_output(const phx_model* m, const phx_config* cfg, const phx_freqout* res, char* log, int32_t log_len) {
+  std::string msgs;
+  const int rc = guarded([&] {
+    if (!cfg) throw Error(PHX_E_ARG, "null config");
+    check_out_args(m, res, "");
+    const std::string outdir = cfg->outdir, image = cfg->image;
+    if (::mkdir(outdir.c_str(), 0755) != 0 && errno != EEXIST) throw Error(PHX_E_IO, "cannot create " + outdir);
+    bool want_text = image == "text" || image == "both";
+    const bool want_h5 = image == "hdf5" || image == "both";
+    if (want_h5) {
+      const std::string hp = outdir + "/img_data.h5";
+      if (phx_write_image_hdf5(m, res, cfg->block_size, hp.c_str()) == PHX_OK) {
+        msgs += "Wrote " + hp + "\n";
+      } else {
+        msgs += "Failed to write \"" + hp + "\" (" + phx_last_error() + ") writing text instead...\n";
+        want_text = true;
+      }
+    }
+    if (want_text)
+      for (int f = 0; f < m->nfreqs; f++) {
+        const std::string ip = outdir + "/image" + (m->nfreqs == 1 ? "" : "_" + std::to_string(f)) + ".dat";
+        if (phx_write_image_text(m, res, f, ip.c_str()) != PHX_OK) throw Error(PHX_E_IO, phx_last_error());
+        msgs += "Wrote " + ip + "\n";
+      }
+    const std::string sp = outdir + "/spectrum.dat";
+    if (phx_write_spectrum(m, res, sp.c_str()) != PHX_OK) throw Error(PHX_E_IO, phx_last_error());
+    msgs += "Wrote " + sp + "\n";
+  });
+  if (log && log_len > 0) std::snprintf(log, (size_t)log_len, "%s", msgs.c_str());
+  return rc;
+}
 }  // extern "C"
