@@ -6,7 +6,8 @@ disabled (``-Xclang -target-feature -Xclang -packed-fp32-ops``): some of their
 forms have produced wrong results on the MI355X boxes this runs on.  FP32
 division and square root use the hardware reciprocal / square-root sequences
 (``-fno-hip-fp32-correctly-rounded-divide-sqrt``, ~1 ulp instead of correctly
-rounded); FP64 is unaffected.
+rounded), and FP32 denormals are flushed to zero (``-fgpu-flush-denormals-to-zero``), which
+lets a division be one reciprocal and one multiply; FP64 is unaffected by both.
 """
 from __future__ import annotations
 
@@ -26,7 +27,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-Wall", "-Wno-unused-function",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics",
-             "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+             "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero",
              f"-I{ROOT}/include"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{ROOT}/include"]
 

@@ -389,7 +389,8 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
     int leaf_next = -1;
     bool have_dxc = false;
     T dxc[3] = {T(0), T(0), T(0)};
-    if (alive) {
+    // far from the grid (r >= rt_outer, no emitting sample pending) nothing is looked up
+    if (alive && (r < A.P.rt_outer || emit)) {
       T wx[4] = {cx[0], cx[1], cx[2], cx[3]};
       wrap(wx);
       const bool need_dxc = r < A.P.rt_outer;

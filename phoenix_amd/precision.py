@@ -1,0 +1,126 @@
+"""Precision report: the FP32 and mixed-precision marches against the FP64 march as the judge.
+
+For each snapshot and inclination, one image of `size`^2 pixels at `nf` frequencies log-spaced
+from 86 to 690 GHz is rendered in FP64, FP32 and mixed precision on the resident snapshot, and
+per frequency the report states:
+
+* total Stokes flux errors: |dF_I| / F_I and |dF_Q|, |dF_U|, |dF_V| / F_I;
+* errors of the image-integrated linear / circular polarization fractions
+  LP = sqrt(Q^2 + U^2) / I and CP = V / I (absolute), and of the EVPA = atan2(U, Q) / 2 (degrees);
+* the per-pixel gates (|dI| / I and |dQUV| / I over pixels above 1e-6 of the peak): the worst
+  value, the 99.9th percentile and the number of pixels above 1e-3.
+
+    python -m phoenix_amd.precision [--size 2048] [--nf 8] [--incl 10,60,163] [--snapshots c2,c2_a05]
+
+Prints one JSON line per (snapshot, inclination, precision) and a summary table.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+from phoenix_amd import capi, configs, snapshot as S
+
+SPIN = {"c2": 0.9375, "c2_a05": 0.5, "c1": 0.9375, "c1_a05": 0.5}
+
+
+def fractions(F: np.ndarray):
+    """F: [nf, 4] total flux -> (LP, CP, EVPA [deg]) per frequency"""
+    I, Q, U, V = F[:, 0], F[:, 1], F[:, 2], F[:, 3]
+    return np.hypot(Q, U) / I, V / I, 0.5 * np.degrees(np.arctan2(U, Q))
+
+
+def pixel_figures(a: np.ndarray, b: np.ndarray, floor: float = 1e-6) -> dict:
+    """a, b: [npix, 6] of one frequency (b is the judge)"""
+    fin = np.all(np.isfinite(a[:, :4]), axis=1) & np.all(np.isfinite(b[:, :4]), axis=1)
+    I = b[:, 0]
+    peak = np.max(np.abs(I[fin])) if fin.any() else 0.0
+    m = fin & (np.abs(I) > floor * peak)
+    if not m.any():
+        return {"gated": 0, "rel_I_max": 0.0, "rel_I_p999": 0.0, "quv_max": 0.0, "over_1e-3": 0}
+    rel = np.abs(a[m, 0] - I[m]) / np.abs(I[m])
+    quv = np.max(np.abs(a[m, 1:4] - b[m, 1:4]) / np.abs(I[m])[:, None], axis=1)
+    return {"gated": int(m.sum()), "rel_I_max": float(rel.max()), "rel_I_p999": float(np.percentile(rel, 99.9)),
+            "quv_max": float(quv.max()), "over_1e-3": int(np.sum((rel > 1e-3) | (quv > 1e-3)))}
+
+
+def report(snapshots, incls, size, nf, out=sys.stdout):
+    freqs = tuple(float(f) for f in np.geomspace(86e9, 690e9, nf))
+    rows = []
+    with tempfile.TemporaryDirectory() as td:
+        for name in snapshots:
+            dump, grid, _ = S.ensure_snapshot(name, S.default_dir())
+            cfg = configs.write_config(os.path.join(td, f"{name}.toml"), dump, grid)
+            run = capi.Run(cfg)
+            snap = capi.Snapshot(run, capi.TABLE_FP32 | capi.TABLE_FP64)
+            for incl in incls:
+                rq = run.request(img_width=size, img_height=size, inclination=float(incl), freqs=freqs)
+                t0 = time.time()
+                judge, ij = snap.render(rq, capi.FP64)
+                F64 = run.flux(judge, rq)
+                lp64, cp64, ev64 = fractions(F64)
+                for label, prec in (("fp32", capi.FP32), ("mixed", capi.MIXED)):
+                    res, inf = snap.render(rq, prec)
+                    F = run.flux(res, rq)
+                    lp, cp, ev = fractions(F)
+                    dev = np.abs((ev - ev64 + 90.0) % 180.0 - 90.0)
+                    per_f = []
+                    for f in range(nf):
+                        pf = pixel_figures(res[:, f], judge[:, f])
+                        per_f.append({"freq_hz": freqs[f], "flux_I": float(F64[f, 0]),
+                                      "dF_I": float(abs(F[f, 0] - F64[f, 0]) / abs(F64[f, 0])),
+                                      "dF_Q": float(abs(F[f, 1] - F64[f, 1]) / abs(F64[f, 0])),
+                                      "dF_U": float(abs(F[f, 2] - F64[f, 2]) / abs(F64[f, 0])),
+                                      "dF_V": float(abs(F[f, 3] - F64[f, 3]) / abs(F64[f, 0])),
+                                      "LP": float(lp64[f]), "dLP": float(abs(lp[f] - lp64[f])),
+                                      "CP": float(cp64[f]), "dCP": float(abs(cp[f] - cp64[f])),
+                                      "EVPA_deg": float(ev64[f]), "dEVPA_deg": float(dev[f]), **pf})
+                    row = {"snapshot": name, "spin": SPIN.get(name), "inclination": incl, "size": size, "nf": nf,
+                           "precision": label, "kernel_s": inf["kernel_s"], "fp64_kernel_s": ij["kernel_s"],
+                           "fp64_step_fraction": inf["fp64_steps"] / max(1, inf["steps"]), "freqs": per_f}
+                    rows.append(row)
+                    print(json.dumps(row), file=out, flush=True)
+                print(f"# {name} i={incl}: {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+            snap.close()
+            run.close()
+    return rows
+
+
+def summary_table(rows) -> str:
+    """one line per (snapshot, inclination, precision): worst over the frequencies"""
+    lines = ["| snapshot (spin) | i | precision | max dF_I | max dF_QUV / F_I | max dLP | max dCP | max dEVPA [deg] "
+             "| max pixel dI/I (p99.9) | pixels > 1e-3 | kernel s (FP64 s) |",
+             "|---|---|---|---|---|---|---|---|---|---|---|"]
+    for r in rows:
+        fs = r["freqs"]
+        mx = lambda k: max(f[k] for f in fs)  # noqa: E731
+        quv = max(max(f["dF_Q"], f["dF_U"], f["dF_V"]) for f in fs)
+        over = sum(f["over_1e-3"] for f in fs)
+        gated = sum(f["gated"] for f in fs)
+        lines.append(f"| {r['snapshot']} ({r['spin']}) | {r['inclination']} | {r['precision']} | {mx('dF_I'):.1e} "
+                     f"| {quv:.1e} | {mx('dLP'):.1e} | {mx('dCP'):.1e} | {mx('dEVPA_deg'):.1e} "
+                     f"| {mx('rel_I_max'):.1e} ({mx('rel_I_p999'):.1e}) | {over} of {gated} "
+                     f"| {r['kernel_s']:.2f} ({r['fp64_kernel_s']:.2f}) |")
+    return "\n".join(lines)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--nf", type=int, default=8)
+    ap.add_argument("--incl", default="10,60,163")
+    ap.add_argument("--snapshots", default="c2,c2_a05")
+    args = ap.parse_args()
+    rows = report(args.snapshots.split(","), [float(x) for x in args.incl.split(",")], args.size, args.nf)
+    print(summary_table(rows), file=sys.stderr)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
