@@ -114,15 +114,15 @@ struct StRef {
   }
 };
 // The FP64 RK4 step of the mixed march: classical RK4 on the FP64 state (the same arithmetic
-// as the FP64 march's rk4), written so that only the state, the stage sum, the stage point
-// and the stage derivative are live (the next stage point is formed as soon as a stage's
-// derivative is known), with one library sincos per stage and the four stages kept as a loop:
-// it runs inline in the FP32 kernel, where an unrolled FP64 step or a call would spill.
-PHX_DEV void rk4_fp64_lean(double a, double hs, double X[12], double dt) {
+// as the FP64 march's rk4).  The state stays in its LDS slots (Xs, stride BLOCK) and is read
+// where a stage needs it, so that only the stage sum, the stage point and the stage derivative
+// occupy VGPRs; the four stages are kept as a loop and each uses one compact sincos.  It runs
+// inline in the FP32 kernel, where an unrolled FP64 step or a call would spill.
+PHX_DEV void rk4_fp64_lean(double a, double hs, double* Xs, double dt) {
   double y[12], sum[12];
 #pragma unroll
   for (int i = 0; i < 12; i++) {
-    y[i] = X[i];
+    y[i] = Xs[i * BLOCK];
     sum[i] = 0.;
   }
 #pragma unroll 1
@@ -138,12 +138,12 @@ PHX_DEV void rk4_fp64_lean(double a, double hs, double X[12], double dt) {
 #pragma unroll
     for (int i = 0; i < 12; i++) {
       sum[i] = q == 0 ? d[i] : sum[i] + wsum * d[i];
-      y[i] = X[i] + wdt * d[i];
+      y[i] = Xs[i * BLOCK] + wdt * d[i];
     }
   }
   const double dt6 = dt / 6.;
 #pragma unroll
-  for (int i = 0; i < 12; i++) X[i] += sum[i] * dt6;
+  for (int i = 0; i < 12; i++) Xs[i * BLOCK] += sum[i] * dt6;
 }
 struct StMix {
   static constexpr int REC = 16;  // start record (doubles): x, k, F1, sign of beta
@@ -176,17 +176,19 @@ struct StMix {
   // FP64 (the wave runs one step code instead of both)
   template <class A>
   PHX_DEV bool step(const A& args, float dl) {
-    double X[12];
-#pragma unroll
-    for (int q = 0; q < 12; q++) X[q] = x(q);
-    bool fp64 = X[1] < args.lnr_fp64 || X[2] < args.x2_pole || X[2] > PI_D - args.x2_pole;
+    const double x1 = x(1), x2 = x(2);
+    bool fp64 = x1 < args.lnr_fp64 || x2 < args.x2_pole || x2 > PI_D - args.x2_pole;
     if (args.fp64_wave) fp64 = __ballot(fp64) != 0ull;
-    if (fp64)
-      rk4_fp64_lean(args.a64, args.hs64, X, double(dl));
-    else
+    if (fp64) {
+      rk4_fp64_lean(args.a64, args.hs64, Xs, double(dl));
+    } else {
+      double X[12];
+#pragma unroll
+      for (int q = 0; q < 12; q++) X[q] = x(q);
       rk4_mixed(args.P.a, args.P.hs, X, dl);
 #pragma unroll
-    for (int q = 0; q < 12; q++) Xs[q * BLOCK] = X[q];
+      for (int q = 0; q < 12; q++) Xs[q * BLOCK] = X[q];
+    }
     return fp64;
   }
 };
