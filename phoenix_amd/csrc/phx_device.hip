@@ -381,9 +381,9 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
     // serves the pending sample's cache (leaf_of) and the step's cell cap (grid_dxc)
     T cx[4], ck[4], cf[4];
     st.view(cx, ck, cf);
-    const T r = exp(cx[1]);
+    const T r = exp_r(cx[1]);
     const bool alive = r < A.r_out && r > A.cut_in && steps < A.max_steps && !stop;
-    const T rs = exp(px[1]);  // radius of the pending sample
+    const T rs = exp_r(px[1]);  // radius of the pending sample
     // the pending sample is the state before the last step, i.e. sample number steps - 1; the
     // camera sample (number 0) is never processed (pixel.hpp:186).  A sample at r >= rt_outer
     // cannot emit (pixel.hpp:80) and changes no ray state, so nothing of it is computed.

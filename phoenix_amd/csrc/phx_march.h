@@ -65,6 +65,34 @@ PHX_DEV void sincos_small(float x, float* s, float* c) {
   *c = cs;
 }
 
+// fmod(x, y) for y > 0 and |x| < 2^22 y: the quotient estimated with the reciprocal, the
+// remainder formed by one fma and corrected once when the estimate is off by one.  The true
+// remainder is representable and the fma rounds it exactly, so this is the library's fmod.
+template <class T>
+PHX_DEV T fmod_exact(T x, T y, T inv_y) {
+  const T n = trunc(x * inv_y);
+  T r = fma(-n, y, x);
+  if (x >= T(0)) {
+    if (r < T(0))
+      r = fma(-(n - T(1)), y, x);
+    else if (r >= y)
+      r = fma(-(n + T(1)), y, x);
+  } else {
+    if (r > T(0))
+      r = fma(-(n + T(1)), y, x);
+    else if (r <= -y)
+      r = fma(-(n - T(1)), y, x);
+  }
+  return r;
+}
+
+// the library exp for r = exp(x1) (a reduction around the hardware exp2 was measured: no gain
+// at this instruction mix, and ~2 ulp instead of ~1)
+template <class T>
+PHX_DEV T exp_r(T x) {
+  return exp(x);
+}
+
 // sin/cos of 2 x2 and of theta = x2 + (h/2) sin 2x2 (the MKS-BHAC stretching).  FP32: one
 // sincos of x2, the double-angle identities, and theta = x2 + delta by angle addition with
 // |delta| <= h/2 <= 0.5 expanded in Taylor series (to delta^9 / delta^10, error < 1e-9);
@@ -187,7 +215,7 @@ struct Met {
 template <class T>
 PHX_DEV Met<T> metric(T a, T hs, T x1, T x2) {
   Met<T> g;
-  const T r = exp(x1);
+  const T r = exp_r(x1);
   T s2x, c2x, s, c;
   theta_trig(x2, hs, &s2x, &c2x, &s, &c);
   const T s2 = s * s, rho2 = r * r + a * a * c * c, hf = T(1) + hs * c2x;
@@ -245,7 +273,7 @@ PHX_DEV Geo<T> geo_at(T a, T hs, T x1, T x2) {
   Geo<T> G;
   G.a = a;
   G.a2 = a * a;
-  G.r = exp(x1);
+  G.r = exp_r(x1);
   G.r2 = G.r * G.r;
   T s2x, c2x;
   theta_trig_sel<ONE_SINCOS>(x2, hs, &s2x, &c2x, &G.s, &G.c);
@@ -450,11 +478,15 @@ struct Grid {
   T w0, w1, w2, up;  // base-level block widths, 2^(nlevels-1)
   T lo1, lo2, lo3, hi1, hi2, hi3;
 };
+// fmod of the wrap: fmod_exact in FP32 (the same values, a fraction of the library's cost); the
+// library fmod in FP64, where it measured faster
+PHX_DEV float wrap_mod(float x, float y) { return fmod_exact(x, y, 1.f / y); }
+PHX_DEV double wrap_mod(double x, double y) { return fmod(x, y); }
 template <class T>
 PHX_DEV void wrap(T x[4]) {  // sampler.hpp:29-38
   const T TP = T(2. * PI_D), PI = T(PI_D);
-  x[3] = fmod(x[3], TP);
-  x[2] = fmod(x[2], PI) - T(1e-6);
+  x[3] = wrap_mod(x[3], TP);
+  x[2] = wrap_mod(x[2], PI) - T(1e-6);
   if (x[3] < T(0)) x[3] = TP + x[3];
   if (x[2] < T(0)) {
     x[2] = -x[2];
