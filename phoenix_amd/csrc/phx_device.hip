@@ -223,6 +223,9 @@ PHX_DEV int sample_prepare(const MarchArgs<M>& A, const TaskDev<typename ModeOf<
   }
   Fluid<T> fl;
   if (!sample_fluid(A.G, A.GL, A.P, tk.Ne, tk.Bu, tk.rhigh, pl, leaf, fl)) return 0;
+  // b^mu = 0 (a field-free cell): the pitch angle is pi/2 (tetrad.hpp:144) and the plasma-frame
+  // basis is 0/0, so every frequency's operator is non-finite; known without building the tetrad
+  if (fl.b[0] == T(0) && fl.b[1] == T(0) && fl.b[2] == T(0) && fl.b[3] == T(0)) return 2;
   const T pa = pitch_angle(fl.g, k, fl.b, fl.U);
   if (pa == T(0) || pa == T(PI_D)) return 0;
   T perp[4], para[4];
