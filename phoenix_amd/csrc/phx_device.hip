@@ -738,7 +738,7 @@ struct Tables {
 
 // Launch knobs (picked by the autotuner or from the environment)
 struct Tune {
-  int queue = -1;         // -1: per precision default (queue for M32 / MIX), 0: in place, 1: LDS queue
+  int queue = -1;         // -1: default (the LDS queue), 0: samples in place, 1: LDS queue
   int blocks_per_cu = 0;  // 0: the occupancy limit
   int order = 0;          // batch job order: 0 request after request, 1 requests interleaved per wave
   int fp64_wave = 1;      // mixed march: 1 = a lane's FP64 step takes its whole wave to FP64
@@ -1126,7 +1126,7 @@ static void launch_march(phx_snapshot* s, const LaunchInfo& li, int j0, int j1, 
 
 template <int M, int NF>
 static void launch_nf(phx_snapshot* s, const LaunchInfo& li, int j0, int j1, const Tune& tn, phx_info* info) {
-  const bool q = tn.queue < 0 ? (M != M64) : tn.queue != 0;
+  const bool q = tn.queue != 0;  // default: the LDS sample queue (measured faster in every precision)
   if (q)
     launch_march<M, NF, true>(s, li, j0, j1, tn, info);
   else
