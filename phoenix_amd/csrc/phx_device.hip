@@ -79,7 +79,7 @@ struct MarchArgs {
 };
 
 // ---- ray state of a lane.  StRef: the reference's state (compensated FP32 pair, or FP64);
-// StMix: FP64 state stepped by rk4_mixed, or by the FP64 RK4 near the horizon and the poles.
+// StHam: the mixed-precision march's Hamiltonian state.
 template <class T>
 struct StRef {
   static constexpr int REC = 24;  // start record: hi[12], lo[8] (FP32 compensation of x, k), sign of beta
@@ -113,87 +113,143 @@ struct StRef {
     return false;
   }
 };
-// The FP64 RK4 step of the mixed march: classical RK4 on the FP64 state (the same arithmetic
-// as the FP64 march's rk4).  The state stays in its LDS slots (Xs, stride BLOCK) and is read
-// where a stage needs it, so that only the stage sum, the stage point and the stage derivative
-// occupy VGPRs; the four stages are kept as a loop and each uses one compact sincos.  It runs
-// inline in the FP32 kernel, where an unrolled FP64 step or a call would spill.
-PHX_DEV void rk4_fp64_lean(double a, double hs, double* Xs, double dt) {
-  double y[12], sum[12];
-#pragma unroll
-  for (int i = 0; i < 12; i++) {
-    y[i] = Xs[i * BLOCK];
-    sum[i] = 0.;
-  }
-#pragma unroll 1
-  for (int q = 0; q < 4; q++) {
-    double d[12];
-    const Geo<double> G = geo_at<double, true>(a, hs, y[1], y[2]);
-#pragma unroll
-    for (int i = 0; i < 4; i++) d[i] = y[4 + i];
-    chris(G, y + 4, y + 4, d + 4);
-    chris(G, y + 4, y + 8, d + 8);
-    const double wsum = (q == 0 || q == 3) ? 1. : 2.;
-    const double wdt = (q == 2 ? 1.0 : 0.5) * dt;
-#pragma unroll
-    for (int i = 0; i < 12; i++) {
-      sum[i] = q == 0 ? d[i] : sum[i] + wsum * d[i];
-      y[i] = Xs[i * BLOCK] + wdt * d[i];
-    }
-  }
-  const double dt6 = dt / 6.;
-#pragma unroll
-  for (int i = 0; i < 12; i++) Xs[i * BLOCK] += sum[i] * dt6;
-}
-struct StMix {
-  static constexpr int REC = 16;  // start record (doubles): x, k, F1, sign of beta
-  // the FP64 state lives in LDS, component-major ([12][BLOCK], conflict-free), so that it holds
-  // no VGPRs outside the step that updates it
-  double* Xs = nullptr;
-  PHX_DEV double x(int q) const { return Xs[q * BLOCK]; }
+// Ray state of the mixed-precision march: the null geodesic in Hamiltonian form (ham_rhs), the
+// FP64 state (x1, x2, x3, p_1, p_2) in VGPRs, the Killing constants p_t and p_phi, and the ray's
+// Walker-Penrose constant in place of a transported polarization vector (the reference transports
+// F1 with the Christoffel symbols, src/core/geometry/geodesic.hpp:180-195; kappa determines the
+// same polarization angle at every sample, wp_angle).  A step runs in FP32 from the FP64 state,
+// mirrored into the northern hemisphere when x2 > pi/2 as the FP32 march does, or in FP64 where
+// r < 2 r_H or |cos theta| > 0.99.  The step's first RK4 stage is evaluated by begin(): its
+// derivative is the wavevector the pass needs (turning points, step rule, the sample).
+struct StHam {
+  static constexpr int REC = 12;  // start record (doubles): x1, x2, x3, p_1, p_2, p_t, p_phi, kappa (2), sign of beta
+  double X[5];
+  double pt, pp;
+  float kap1, kap2;
+  double d1[5];  // first stage of the coming step: FP64, or the (mirrored) FP32 derivative widened
+  bool f64 = false;
   PHX_DEV void load(const double* st, float* bsign) {
 #pragma unroll
-    for (int q = 0; q < 12; q++) Xs[q * BLOCK] = st[q];
-    *bsign = float(st[12]);
+    for (int q = 0; q < 5; q++) X[q] = st[q];
+    pt = st[5];
+    pp = st[6];
+    kap1 = float(st[7]);
+    kap2 = float(st[8]);
+    *bsign = float(st[9]);
   }
   PHX_DEV void view(float cx[4], float ck[4], float cf[4]) const {
+    cx[0] = 0.f;  // t: nothing on the path depends on it (a stationary snapshot)
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      cx[q] = float(x(q));
-      ck[q] = float(x(4 + q));
-      cf[q] = float(x(8 + q));
-    }
+    for (int q = 0; q < 3; q++) cx[1 + q] = float(X[q]);
+    cf[0] = kap1;
+    cf[1] = kap2;
+    cf[2] = cf[3] = 0.f;
+    (void)ck;
   }
   PHX_DEV void lpos(const float*, double pl[4]) const {
+    pl[0] = 0.;
 #pragma unroll
-    for (int q = 0; q < 4; q++) pl[q] = x(q);
+    for (int q = 0; q < 3; q++) pl[1 + q] = X[q];
   }
-  PHX_DEV float x2p() const {
-    const double x2 = x(2);
-    return float(x2 > 0.5 * PI_D ? PI_D - x2 : x2);
+  PHX_DEV float x2p() const { return float(X[1] > 0.5 * PI_D ? PI_D - X[1] : X[1]); }
+  // FP32 copy of the state, mirrored to x2 < pi/2 (x2 -> pi - x2, p_2 -> -p_2, under which the
+  // Hamiltonian is symmetric) when x2 > pi/2
+  PHX_DEV bool base32(float b[5]) const {
+    const bool south = X[1] > 0.5 * PI_D;
+#pragma unroll
+    for (int q = 0; q < 5; q++) b[q] = float(X[q]);
+    if (south) {
+      b[1] = float(PI_D - X[1]);
+      b[4] = -b[4];
+    }
+    return south;
   }
-  // returns whether the step ran in FP64; WAVE: one lane's FP64 step takes its whole wave to
-  // FP64 (the wave runs one step code instead of both)
+  // precision of the coming step and its first stage; ck = the wavevector at the current point
+  template <class A>
+  PHX_DEV void begin(const A& args, bool alive, float ck[4]) {
+    f64 = alive && (X[0] < args.lnr_fp64 || X[1] < args.x2_pole || X[1] > PI_D - args.x2_pole);
+    if (args.fp64_wave) f64 = __ballot(f64) != 0ull;
+    if (f64) {
+      double k0;
+      ham_rhs<double, true, true>(args.a64, args.hs64, pt, pp, X, d1, &k0);
+      ck[0] = float(k0);
+#pragma unroll
+      for (int q = 0; q < 3; q++) ck[1 + q] = float(d1[q]);
+    } else {
+      float b[5], d[5], k0;
+      const bool south = base32(b);
+      ham_rhs<float>(args.P.a, args.P.hs, float(pt), float(pp), b, d, &k0);
+#pragma unroll
+      for (int q = 0; q < 5; q++) d1[q] = d[q];
+      ck[0] = k0;
+      ck[1] = d[0];
+      ck[2] = south ? -d[1] : d[1];
+      ck[3] = d[2];
+    }
+  }
+  // classical RK4 (weights 1/2, 1/2, 1; (k1 + 2 k2 + 2 k3 + k4) dt / 6) from the stage begin()
+  // evaluated; the increment is added to the FP64 state.  Returns whether it ran in FP64.
   template <class A>
   PHX_DEV bool step(const A& args, float dl) {
-    const double x1 = x(1), x2 = x(2);
-    bool fp64 = x1 < args.lnr_fp64 || x2 < args.x2_pole || x2 > PI_D - args.x2_pole;
-    if (args.fp64_wave) fp64 = __ballot(fp64) != 0ull;
-    if (fp64) {
-      rk4_fp64_lean(args.a64, args.hs64, Xs, double(dl));
+    if (f64) {
+      const double dt = dl;
+      double sum[5], y[5];
+#pragma unroll
+      for (int i = 0; i < 5; i++) {
+        sum[i] = d1[i];
+        y[i] = X[i] + 0.5 * dt * d1[i];
+      }
+#pragma unroll 1
+      for (int q = 1; q < 4; q++) {
+        double d[5], k0;
+        ham_rhs<double, true, true>(args.a64, args.hs64, pt, pp, y, d, &k0);
+        const double w = q == 3 ? 1. : 2., h = q == 2 ? dt : 0.5 * dt;
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+          sum[i] += w * d[i];
+          y[i] = X[i] + h * d[i];
+        }
+      }
+      const double dt6 = dt / 6.;
+#pragma unroll
+      for (int i = 0; i < 5; i++) X[i] += sum[i] * dt6;
     } else {
-      double X[12];
+      float b[5], sum[5], y[5];
+      const bool south = base32(b);
+      const float pt32 = float(pt), pp32 = float(pp);
 #pragma unroll
-      for (int q = 0; q < 12; q++) X[q] = x(q);
-      rk4_mixed(args.P.a, args.P.hs, X, dl);
+      for (int i = 0; i < 5; i++) {
+        const float d = float(d1[i]);
+        sum[i] = d;
+        y[i] = b[i] + 0.5f * dl * d;
+      }
 #pragma unroll
-      for (int q = 0; q < 12; q++) Xs[q * BLOCK] = X[q];
+      for (int q = 1; q < 4; q++) {
+        float d[5], k0;
+        ham_rhs<float>(args.P.a, args.P.hs, pt32, pp32, y, d, &k0);
+        const float w = q == 3 ? 1.f : 2.f, h = q == 2 ? dl : 0.5f * dl;
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+          sum[i] += w * d[i];
+          if (q < 3) y[i] = b[i] + h * d[i];
+        }
+      }
+      const float dt6 = dl / 6.f;
+      float inc[5];
+#pragma unroll
+      for (int i = 0; i < 5; i++) inc[i] = sum[i] * dt6;
+      if (south) {
+        inc[1] = -inc[1];
+        inc[4] = -inc[4];
+      }
+#pragma unroll
+      for (int i = 0; i < 5; i++) X[i] += double(inc[i]);
     }
-    return fp64;
+    return f64;
   }
 };
 template <int M>
-using StateOf = typename std::conditional<M == MIX, StMix, StRef<typename ModeOf<M>::T>>::type;
+using StateOf = typename std::conditional<M == MIX, StHam, StRef<typename ModeOf<M>::T>>::type;
 template <int M>
 using RecOf = typename std::conditional<M == M32, float, double>::type;  // start record scalar
 
@@ -230,10 +286,15 @@ PHX_DEV int sample_prepare(const MarchArgs<M>& A, const TaskDev<typename ModeOf<
   if (pa == T(0) || pa == T(PI_D)) return 0;
   T perp[4], para[4];
   frame(fl.g, A.P.a, k, fl.U, fl.b, perp, para);
-  const T Cp = dot(fl.g, perp, pf), Sp = dot(fl.g, para, pf);
-  const T nn = Cp * Cp + Sp * Sp;
-  sp.c2 = (Cp * Cp - Sp * Sp) / nn;
-  sp.s2 = T(2) * Cp * Sp / nn;
+  if constexpr (M == MIX) {
+    // pf holds the ray's Walker-Penrose constant (StHam)
+    wp_angle(A.P.a, fl.g, k, perp, para, pf[0], pf[1], &sp.c2, &sp.s2);
+  } else {
+    const T Cp = dot(fl.g, perp, pf), Sp = dot(fl.g, para, pf);  // chi_from_F1 (tetrad.hpp:160-182)
+    const T nn = Cp * Cp + Sp * Sp;
+    sp.c2 = (Cp * Cp - Sp * Sp) / nn;
+    sp.s2 = T(2) * Cp * Sp / nn;
+  }
   sp.ratio = -dot(fl.g, k, fl.U);
   if (__builtin_isnan(sp.c2) || __builtin_isnan(sp.s2) || __builtin_isnan(sp.ratio)) return 2;
   sp.te = fl.te;
@@ -299,16 +360,18 @@ PHX_DEV void accumulate(T AA[16], T SS[4], T SC[4], const T OA[16], const T OS[4
 // leaf as its cache, turning points, step size, RK4.
 //
 // QUEUE: a per-wave sample queue in LDS.  A lane whose pending sample can emit pushes it
-// (position, wavevector, polarization vector, step length, cache leaf, task) into its wave's
-// queue instead of processing it at once.  When the queue holds 64 samples, when a lane holds
-// KMAX of them, or when a ray with queued samples ends, all active lanes of the wave compute
-// the queued samples' segment operators together, one sample per lane, and each owning lane
-// then folds its own operators into its accumulator in the order it queued them, which is
-// the reference's order (pixel.hpp:186-188).  The divergent, expensive part of a sample
-// (fluid, tetrad, coefficients, 4x4 inverse) thus runs on full waves.  Without QUEUE the
-// lane processes its sample in place.
+// (position, wavevector, polarization vector or Walker-Penrose constant, step length, cache
+// leaf, task) into its wave's queue instead of processing it at once.  When the queue holds 64
+// samples, when a lane holds KMAX of them, or when a ray with queued samples ends, all active
+// lanes of the wave work through the queue together, one sample per lane: first the
+// frequency-independent part of every sample, then, on the compacted list of the samples that
+// emit, the segment operators; each owning lane folds its own operators into its accumulator in
+// the order it queued them, which is the reference's order (pixel.hpp:186-188).  The divergent,
+// expensive part of a sample (fluid, tetrad, coefficients, 4x4 inverse) thus runs on full waves.
+// Without QUEUE the lane processes its sample in place.
 constexpr int QCAP = 128;  // slots per wave (at most 63 + 64 are in use at a flush)
-constexpr int QW = 24;     // scalars per slot: 21 in (x, k, F1, dl, leaf, FP64 position, task), 24 out
+constexpr int QW = 24;     // scalars per slot: 21 in (x, k, F1, dl, leaf, FP64 position, task), then the
+                           // prepared sample, then the operators; 23 = position in the emitting list
 constexpr int KMAX = 4;    // queued samples a lane may hold
 
 PHX_DEV void wave_sync() {
@@ -329,12 +392,12 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
   using R = RecOf<M>;
   constexpr bool KAHAN = M == MIX;
   __shared__ __align__(16) T qbuf[QUEUE ? BLOCK / 64 : 1][QUEUE ? QCAP * QW : 1];
-  __shared__ double xstate[M == MIX ? 12 * BLOCK : 1];
+  __shared__ int clbuf[QUEUE ? BLOCK / 64 : 1][QUEUE ? QCAP : 1];  // per wave: the emitting samples of a flush
   T* const Q = qbuf[QUEUE ? (threadIdx.x >> 6) : 0];
+  int* const cl = clbuf[QUEUE ? (threadIdx.x >> 6) : 0];
   const int lane = threadIdx.x & 63;
   const unsigned long long lt = (1ull << lane) - 1ull;
   St st;
-  if constexpr (M == MIX) st.Xs = xstate + threadIdx.x;
   T px[4] = {T(0), T(0), T(0), T(0)}, pk[4] = {T(0), T(0), T(0), T(0)}, pf[4] = {T(0), T(0), T(0), T(0)};
   L pl[4] = {L(0), L(0), L(0), L(0)};
   T pdl = T(0);
@@ -386,6 +449,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
     st.view(cx, ck, cf);
     const T r = exp_r(cx[1]);
     const bool alive = r < A.r_out && r > A.cut_in && steps < A.max_steps && !stop;
+    if constexpr (M == MIX) st.begin(A, alive, ck);
     const T rs = exp_r(px[1]);  // radius of the pending sample
     // the pending sample is the state before the last step, i.e. sample number steps - 1; the
     // camera sample (number 0) is never processed (pixel.hpp:186).  A sample at r >= rt_outer
@@ -483,18 +547,24 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
     }
 
     if constexpr (QUEUE) {
-      // ---- flush: all active lanes compute the queued samples, owners accumulate in order
+      // ---- flush, in two passes over the queue.  Pass 1: all active lanes take the queued
+      // samples one per lane and compute their frequency-independent part (fluid, frame,
+      // polarization angle, Bessel ratios), which most samples end in (no emission, or a
+      // field-free cell); the emitting ones are listed in queue order.  Pass 2: all active lanes
+      // take the listed samples one per lane and compute the segment operators of each
+      // frequency; after each frequency the owners fold their operators in in queue order.
       const bool want = qcount >= QCAP - 64 || nmine >= KMAX || (!alive && nmine > 0);
       if (__ballot(want) != 0ull) {
         const unsigned long long act = __ballot(1);
         const int nact = (int)__popcll(act), rank = (int)__popcll(act & lt);
+        int* const clist = cl;
+        int ncomp = 0;  // the same in every active lane
         for (int base = 0; base < qcount; base += nact) {
           const int e = base + rank;
           const bool mine = e < qcount;
-          Prep<T> sp;
-          int status = 0, etask = 0;
+          int status = 0;
           if (mine) {
-            const T* in = Q + e * QW;
+            T* const in = Q + e * QW;
             T x[4], k[4], f1[4];
             L xl[4];
 #pragma unroll
@@ -509,41 +579,88 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
 #pragma unroll
               for (int q = 0; q < 3; q++) xl[1 + q] = e64[q];
             }
-            etask = (int)in[20];
-            status = sample_prepare<M>(A, A.tasks[etask], x, xl, k, f1, in[12], (int)in[13], sp);
+            const T etask = in[20];
+            Prep<T> sp;
+            status = sample_prepare<M>(A, A.tasks[(int)etask], x, xl, k, f1, in[12], (int)in[13], sp);
+            // the slot now holds the prepared sample: Prep at 0..10, task 11, status 12
+            in[0] = sp.te;
+            in[1] = sp.ne;
+            in[2] = sp.B;
+            in[3] = sp.sth;
+            in[4] = sp.cth;
+            in[5] = sp.kr1;
+            in[6] = sp.kr0;
+            in[7] = sp.c2;
+            in[8] = sp.s2;
+            in[9] = sp.ratio;
+            in[10] = sp.dl;
+            in[11] = etask;
+            in[12] = T(status);
           }
-          const int enf = mine ? A.tasks[etask].nf : 0;
+          const unsigned long long em = __ballot(status == 1);
+          const int ci = ncomp + (int)__popcll(em & lt);
+          if (mine) {
+            Q[e * QW + 23] = T(status == 1 ? ci : -1);  // position in the emitting list
+            if (status == 1) clist[ci] = e;
+          }
+          ncomp += (int)__popcll(em);
+        }
+        wave_sync();
+        // the owners count their samples; a sample whose rotation or frequency ratio is NaN has
+        // every frequency's operator non-finite, dropped by the reference (pixel.hpp:107-116)
+#pragma unroll
+        for (int j = 0; j < KMAX; j++) {
+          if (j < nmine) {
+            const int status = (int)Q[myslot[j] * QW + 12];
+            if (status != 0) np++;
+            if (status == 2) bad += nf;
+          }
+        }
+        for (int cb = 0; cb < ncomp; cb += nact) {
+          const int ci = cb + rank;
+          const bool mine = ci < ncomp;
+          Prep<T> sp;
+          int e = 0, etask = 0, enf = 0;
+          if (mine) {
+            e = clist[ci];
+            const T* in = Q + e * QW;
+            sp.te = in[0];
+            sp.ne = in[1];
+            sp.B = in[2];
+            sp.sth = in[3];
+            sp.cth = in[4];
+            sp.kr1 = in[5];
+            sp.kr0 = in[6];
+            sp.c2 = in[7];
+            sp.s2 = in[8];
+            sp.ratio = in[9];
+            sp.dl = in[10];
+            etask = (int)in[11];
+            enf = A.tasks[etask].nf;
+          }
 #pragma unroll
           for (int f = 0; f < NF; f++) {
             if (mine && f < enf) {
               T* out = Q + e * QW;
-              int flag = 0;  // 1: fold in, 2: dropped as non-finite
-              if (status == 1) {
-                T OA[16], OS[4], dtau, dtauF;
-                flag = sample_op(A.tasks[etask], sp, f, OA, OS, &dtau, &dtauF) ? 1 : 2;
-                if (flag == 1) {
+              T OA[16], OS[4], dtau, dtauF;
+              const bool fin = sample_op(A.tasks[etask], sp, f, OA, OS, &dtau, &dtauF);
+              if (fin) {
 #pragma unroll
-                  for (int q = 0; q < 16; q++) out[q] = OA[q];
+                for (int q = 0; q < 16; q++) out[q] = OA[q];
 #pragma unroll
-                  for (int q = 0; q < 4; q++) out[16 + q] = OS[q];
-                  out[20] = dtau;
-                  out[21] = dtauF;
-                }
-              } else if (status == 2) {
-                flag = 2;
+                for (int q = 0; q < 4; q++) out[16 + q] = OS[q];
+                out[20] = dtau;
+                out[21] = dtauF;
               }
-              out[22] = T(flag);
-              out[23] = T(status != 0 ? 1 : 0);
+              out[22] = T(fin ? 1 : 2);  // 1: fold in, 2: dropped as non-finite
             }
             wave_sync();
 #pragma unroll
             for (int j = 0; j < KMAX; j++) {
-              const int sl = myslot[j];
-              if (j < nmine && sl >= base && sl < base + nact && f < nf) {
-                const T* rr = Q + sl * QW;
-                const int flag = (int)rr[22];
-                if (f == 0 && rr[23] != T(0)) np++;
-                if (flag == 1) {
+              const T* rr = Q + myslot[j] * QW;
+              const int pos = j < nmine ? (int)rr[23] : -1;
+              if (pos >= cb && pos < cb + nact && f < nf) {
+                if (rr[22] == T(1)) {
                   T OA[16], OS[4];
 #pragma unroll
                   for (int q = 0; q < 16; q++) OA[q] = rr[q];
@@ -552,7 +669,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
                   accumulate<T, KAHAN>(AA[f], SS[f], SC[f], OA, OS);
                   tau[f] += rr[20];
                   tauF[f] += rr[21];
-                } else if (flag == 2) {
+                } else {
                   bad++;
                 }
               }
@@ -636,13 +753,24 @@ __global__ __launch_bounds__(BLOCK) void k_rays(const Cam* __restrict__ cams, co
   camera_photon(C, alpha, beta, ph);
   const R bs = beta < 0. ? R(-1) : beta > 0. ? R(1) : R(0);
   if constexpr (M == MIX) {
-    double F1[4];
+    // Hamiltonian start: p_mu = g_mu nu k^nu and the Walker-Penrose constant of (k, F1), in FP64
+    double F1[4], kd[4], q1, q2;
     camera_F1<double>(C.a, C.hs, ph, F1);
-    double* o = out + (size_t)n * StMix::REC;
-    for (int q = 0; q < 8; q++) o[q] = ph[q];
-    for (int q = 0; q < 4; q++) o[8 + q] = F1[q];
-    o[12] = bs;
-    o[13] = o[14] = o[15] = 0.;
+    const Met<double> g = metric<double>(C.a, C.hs, ph[1], ph[2]);
+    lower(g, ph + 4, kd);
+    wp_kappa<double>(C.a, g.r, g.s, g.c, g.hf, ph + 4, F1, &q1, &q2);
+    double* o = out + (size_t)n * StHam::REC;
+    o[0] = ph[1];
+    o[1] = ph[2];
+    o[2] = ph[3];
+    o[3] = kd[1];
+    o[4] = kd[2];
+    o[5] = kd[0];
+    o[6] = kd[3];
+    o[7] = q1;
+    o[8] = q2;
+    o[9] = bs;
+    o[10] = o[11] = 0.;
   } else {
     T F1[4];
     camera_F1<T>(T(C.a), T(C.hs), ph, F1);
@@ -1139,7 +1267,7 @@ template <int M>
 static void march_jobs(phx_snapshot* s, const LaunchInfo& li, const Tune& tn, phx_info* info,
                        phx_progress_fn progress, void* user) {
   using R = RecOf<M>;
-  const int rec = M == MIX ? StMix::REC : StRef<typename ModeOf<M>::T>::REC;
+  const int rec = M == MIX ? StHam::REC : StRef<typename ModeOf<M>::T>::REC;
   ensure(s->start, (size_t)std::max(li.njobs, 1) * rec * sizeof(R));
   if (li.njobs > 0)
     k_rays<M><<<(li.njobs + BLOCK - 1) / BLOCK, BLOCK, 0, s->stream>>>((const Cam*)s->cams.p,

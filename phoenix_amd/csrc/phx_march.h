@@ -392,51 +392,89 @@ PHX_DEV void rk4(T a, T hs, Ray<T>& g, T dt) {
       g.hi[i] += incr;
   }
 }
-// RK4 of the mixed-precision march: the state is held in FP64 and every stage is evaluated
-// in FP32 from the FP32 rounding of the state plus the stage offset; the weighted stage sum
-// is added to the FP64 state.  The south-hemisphere fold mirrors x2 about pi in FP64 first,
-// as the FP32 march does with its compensated pair.
-PHX_DEV void rk4_mixed(float a, float hs, double X[12], float dt) {
-  const bool south = X[2] > 0.5 * PI_D;
-  float base[12];
-#pragma unroll
-  for (int i = 0; i < 12; i++) base[i] = float(X[i]);
-  if (south) base[2] = float(PI_D - X[2]);
-  float sum[12], inc[12];
-#pragma unroll
-  for (int i = 0; i < 12; i++) inc[i] = 0.f;
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    float y[12];
-#pragma unroll
-    for (int i = 0; i < 12; i++) y[i] = base[i] + inc[i];
-    if (south) {
-      y[2] = base[2] - inc[2];
-      y[6] = -y[6];
-      y[10] = -y[10];
-    }
-    float d[12];
-    const Geo<float> G = geo_at(a, hs, y[1], y[2]);
-#pragma unroll
-    for (int i = 0; i < 4; i++) d[i] = y[4 + i];
-    chris(G, y + 4, y + 4, d + 4);
-    chris(G, y + 4, y + 8, d + 8);
-    if (south) {
-      d[2] = -d[2];
-      d[6] = -d[6];
-      d[10] = -d[10];
-    }
-    const float wsum = (q == 0 || q == 3) ? 1.f : 2.f;
-#pragma unroll
-    for (int i = 0; i < 12; i++) sum[i] = q == 0 ? d[i] : sum[i] + wsum * d[i];
-    if (q == 3) break;
-    const float wdt = (q == 2 ? 1.f : 0.5f) * dt;
-#pragma unroll
-    for (int i = 0; i < 12; i++) inc[i] = wdt * d[i];
+// ------------------------------------------------------------------ Hamiltonian form
+// The same null geodesic as Hamilton's equations of H = g^{mu nu} p_mu p_nu / 2, with the
+// contravariant MKS-BHAC Kerr-Schild metric of src/core/geometry/metric.hpp:64-91.  H depends
+// on neither t nor phi, so p_t and p_phi are constants of the motion and the state reduces to
+// (x1, x2, x3, p_1, p_2).  With Sigma = r^2 + a^2 cos^2 theta and theta' = d theta / d x2,
+//   2H = -p_t^2 + Q / Sigma,
+//   Q  = -2 r p_t^2 + 4 p_t p_1 + (Delta / r^2) p_1^2 + (2 a / r) p_1 p_phi + p_2^2 / theta'^2
+//        + p_phi^2 / sin^2 theta,
+// so only d/dr and d/dtheta of Q and Sigma enter the force.  d[0..2] = k^{1,2,3} = dH/dp_i,
+// d[3..4] = dp_{1,2}/dlambda = -dH/dx^{1,2} (x1 = ln r: d/dx1 = r d/dr; the x2 derivative adds
+// the explicit theta'(x2) of the p_2 term), *k0 = k^t = g^{tt} p_t + g^{t1} p_1.
+// RECIP: S, 1/r, 1/sin theta and 1/theta' from one reciprocal of their product (FP64, where a
+// division is a long sequence); otherwise four reciprocals (FP32, hardware rcp).
+template <class T, bool ONE_SINCOS = false, bool RECIP = false>
+PHX_DEV void ham_rhs(T a, T hs, T pt, T pp, const T y[5], T d[5], T* k0) {
+  const T r = exp_r(y[0]);
+  T s2x, c2x, s, c;
+  theta_trig_sel<ONE_SINCOS>(y[1], hs, &s2x, &c2x, &s, &c);
+  const T tp = T(1) + hs * c2x, tpp = T(-2) * hs * s2x;
+  const T a2 = a * a;
+  const T Sig = r * r + a2 * c * c;
+  T S, ir, is, itp;
+  if (RECIP) {
+    const T rs = r * s, tS = tp * Sig;
+    const T inv = T(1) / (rs * tS);
+    S = inv * rs * tp;
+    ir = inv * s * tS;
+    is = inv * r * tS;
+    itp = inv * rs * Sig;
+  } else {
+    S = T(1) / Sig;
+    ir = T(1) / r;
+    is = T(1) / s;
+    itp = T(1) / tp;
   }
-  const float dt6 = dt / 6.f;
-#pragma unroll
-  for (int i = 0; i < 12; i++) X[i] += double(sum[i] * dt6);
+  const T is2 = is * is, itp2 = itp * itp;
+  const T p1 = y[3], p2 = y[4];
+  const T dr2 = T(1) - T(2) * ir + a2 * ir * ir;  // Delta / r^2
+  const T ptt = pt * pt, q22 = p2 * p2 * itp2, q33 = pp * pp * is2;
+  const T Q = T(-2) * r * ptt + T(4) * pt * p1 + dr2 * p1 * p1 + T(2) * a * ir * p1 * pp + q22 + q33;
+  const T dQr = T(-2) * ptt + T(2) * ir * ir * (p1 * p1 * (T(1) - a2 * ir) - a * p1 * pp);
+  const T dQth = T(-2) * q33 * c * is;
+  const T SQ = S * Q;
+  d[0] = S * (T(2) * pt + dr2 * p1 + a * ir * pp);
+  d[1] = S * p2 * itp2;
+  d[2] = S * (a * ir * p1 + pp * is2);
+  d[3] = T(-0.5) * r * S * (dQr - T(2) * r * SQ);
+  d[4] = T(-0.5) * S * (tp * (dQth + T(2) * a2 * s * c * SQ) - T(2) * q22 * tpp * itp);
+  *k0 = -pt - T(2) * S * (r * pt - p1);
+}
+// The Walker-Penrose constant kappa = (A - i B)(r - i a cos theta) of a vector f at a point with
+// wavevector k, from the Kerr-Schild components W^{mu nu} = k^mu f^nu - k^nu f^mu in
+// (t, r, theta, phi) (k^r = r k^1, k^theta = theta' k^2 for the MKS components given):
+//   A = W^{tr} + a sin^2 theta W^{r phi},  B = sin theta [(r^2 + a^2) W^{phi theta} - a (W^{t theta} + W^{r theta})].
+// This is the Boyer-Lindquist form (Walker & Penrose 1970; Chandrasekhar 1983) with dt and dphi
+// shifted by the Kerr-Schild terms 2r/Delta dr and a/Delta dr, which cancel to the regular form
+// above.  kappa is constant along a null geodesic for a parallel-transported f, and it is
+// real-linear in f, with kappa(k) = 0.
+template <class T>
+PHX_DEV void wp_kappa(T a, T r, T s, T c, T tp, const T k[4], const T f[4], T* q1, T* q2) {
+  const T kr = r * k[1], kth = tp * k[2], fr = r * f[1], fth = tp * f[2];
+  const T Wtr = k[0] * fr - kr * f[0], Wrp = kr * f[3] - k[3] * fr, Wpth = k[3] * fth - kth * f[3];
+  const T Wtth = k[0] * fth - kth * f[0], Wrth = kr * fth - kth * fr;
+  const T A = Wtr + a * s * s * Wrp;
+  const T B = s * ((r * r + a * a) * Wpth - a * (Wtth + Wrth));
+  *q1 = A * r - B * a * c;
+  *q2 = -(A * a * c + B * r);
+}
+// cos 2 psi, sin 2 psi of the polarization vector that carries the ray's Walker-Penrose constant
+// kap, in the plasma-frame basis (perp, para) at the sample (both orthogonal to k): with
+// f = f1 perp + f2 para + (multiple of k), kap = f1 kappa(perp) + f2 kappa(para), a 2 x 2 real
+// system whose solution (up to a common factor) is (f1, f2); 2 psi follows as in chi_from_F1
+// (src/core/geometry/tetrad.hpp:160-182).
+template <class T>
+PHX_DEV void wp_angle(T a, const Met<T>& g, const T k[4], const T perp[4], const T para[4], T kap1, T kap2, T* c2,
+                      T* s2) {
+  T e1r, e1i, e2r, e2i;
+  wp_kappa(a, g.r, g.s, g.c, g.hf, k, perp, &e1r, &e1i);
+  wp_kappa(a, g.r, g.s, g.c, g.hf, k, para, &e2r, &e2i);
+  const T f1 = kap1 * e2i - e2r * kap2, f2 = e1r * kap2 - kap1 * e1i;
+  const T nn = f1 * f1 + f2 * f2;
+  *c2 = (f1 * f1 - f2 * f2) / nn;
+  *s2 = T(2) * f1 * f2 / nn;
 }
 template <class T>
 PHX_DEV T pole_x2(const Ray<T>& g) {  // geodesic.hpp:200-205
