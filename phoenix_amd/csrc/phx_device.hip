@@ -259,8 +259,93 @@ template <class T>
 struct Prep {
   T te, ne, B, sth, cth, kr1, kr0, c2, s2, ratio, dl;
 };
-// 0: no emission; 1: emits; 2: emits, but the rotation or the frequency ratio is NaN so every
-// frequency's operator will be non-finite and dropped
+// What the second half of a sample needs from the first, a candidate record of 23 scalars (one
+// queue slot): [0..1] the position the sampler used (x1, x2), [2..5] the (re-solved) wavevector,
+// [6..9] U^mu, [10..13] b^mu, [14..17] the ray's polarization vector (or its Walker-Penrose
+// constant), [18] Theta_e, [19] n_e, [20] |B|, [21] the step length, [22] the task.
+constexpr int CREC = 23;
+// First half of a sample (src/core/render/pixel.hpp:52-79): null re-solve of k, fluid at the
+// sample (wrap, cached leaf / lookup, cell, trilinear gather, derive_fluid, cuts).
+// 0: no emission; 2: emits nothing finite (b^mu = 0, a field-free cell: the pitch angle is pi/2,
+// tetrad.hpp:144, and the plasma-frame basis is 0/0, so every frequency's operator is non-finite);
+// 3: a candidate for emission, written to c.
+template <int M>
+PHX_DEV int sample_fluid_stage(const MarchArgs<M>& A, const TaskDev<typename ModeOf<M>::T>& tk,
+                               const typename ModeOf<M>::T px[4], const typename ModeOf<M>::L pl[4],
+                               const typename ModeOf<M>::T pk[4], const typename ModeOf<M>::T pf[4],
+                               typename ModeOf<M>::T dl, int leaf, int task, typename ModeOf<M>::T* c) {
+  using T = typename ModeOf<M>::T;
+  T k[4] = {pk[0], pk[1], pk[2], pk[3]};
+  const T rs = exp(px[1]);
+  const Met<T> gs = metric(A.P.a, A.P.hs, px[1], px[2]);
+  if (fabs(dot(gs, k, k)) > T(1e-6) && rs > T(2)) {  // re-solve k^t for k.k = 0 (geodesic.hpp:297-306)
+    const T aa = gs.d00;
+    const T bc = -(gs.d01 * k[1]) / aa - (gs.d03 * k[3]) / aa;
+    const T gz = -(gs.d11 * k[1] * k[1]) / aa - (T(2) * gs.d13 * k[1] * k[3]) / aa - (gs.d22 * k[2] * k[2]) / aa -
+                 (gs.d33 * k[3] * k[3]) / aa;
+    k[0] = bc + sqrt(bc * bc + gz);
+  }
+  Fluid<T> fl;
+  if (!sample_fluid(A.G, A.GL, A.P, tk.Ne, tk.Bu, tk.rhigh, pl, leaf, fl)) return 0;
+  if (fl.b[0] == T(0) && fl.b[1] == T(0) && fl.b[2] == T(0) && fl.b[3] == T(0)) return 2;
+  c[0] = fl.x[1];
+  c[1] = fl.x[2];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    c[2 + q] = k[q];
+    c[6 + q] = fl.U[q];
+    c[10 + q] = fl.b[q];
+    c[14 + q] = pf[q];
+  }
+  c[18] = fl.te;
+  c[19] = fl.ne;
+  c[20] = fl.B;
+  c[21] = dl;
+  c[22] = T(task);
+  return 3;
+}
+// Second half (pixel.hpp:80-94) of the candidate c: pitch angle, plasma-frame tetrad,
+// polarization angle, frequency ratio, Bessel ratios.  The metric at the sample is evaluated
+// again (the same function of the same position as in the first half).  0: no emission (pitch
+// angle 0 or pi); 1: emits; 2: the rotation or the frequency ratio is NaN, so every frequency's
+// operator is non-finite and dropped.
+template <int M>
+PHX_DEV int sample_frame_stage(const MarchArgs<M>& A, const typename ModeOf<M>::T* c, Prep<typename ModeOf<M>::T>& sp) {
+  using T = typename ModeOf<M>::T;
+  const Met<T> g = metric(A.P.a, A.P.hs, c[0], c[1]);
+  T k[4], U[4], b[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    k[q] = c[2 + q];
+    U[q] = c[6 + q];
+    b[q] = c[10 + q];
+  }
+  const T pa = pitch_angle(g, k, b, U);
+  if (pa == T(0) || pa == T(PI_D)) return 0;
+  T perp[4], para[4];
+  frame(g, A.P.a, k, U, b, perp, para);
+  if constexpr (M == MIX) {
+    // the record holds the ray's Walker-Penrose constant (StHam)
+    wp_angle(A.P.a, g, k, perp, para, c[14], c[15], &sp.c2, &sp.s2);
+  } else {
+    const T f[4] = {c[14], c[15], c[16], c[17]};
+    const T Cp = dot(g, perp, f), Sp = dot(g, para, f);  // chi_from_F1 (tetrad.hpp:160-182)
+    const T nn = Cp * Cp + Sp * Sp;
+    sp.c2 = (Cp * Cp - Sp * Sp) / nn;
+    sp.s2 = T(2) * Cp * Sp / nn;
+  }
+  sp.ratio = -dot(g, k, U);
+  if (__builtin_isnan(sp.c2) || __builtin_isnan(sp.s2) || __builtin_isnan(sp.ratio)) return 2;
+  sp.te = c[18];
+  sp.ne = c[19];
+  sp.B = c[20];
+  sp.dl = c[21];
+  scs(pa, &sp.sth, &sp.cth);
+  kratios(sp.te, &sp.kr1, &sp.kr0);
+  return 1;
+}
+// Both halves at once, for the sample processed in place by its own lane (the reference
+// kernel's way): 0 no emission, 1 emits, 2 every frequency's operator is dropped.
 template <int M>
 PHX_DEV int sample_prepare(const MarchArgs<M>& A, const TaskDev<typename ModeOf<M>::T>& tk,
                            const typename ModeOf<M>::T px[4], const typename ModeOf<M>::L pl[4],
@@ -279,15 +364,12 @@ PHX_DEV int sample_prepare(const MarchArgs<M>& A, const TaskDev<typename ModeOf<
   }
   Fluid<T> fl;
   if (!sample_fluid(A.G, A.GL, A.P, tk.Ne, tk.Bu, tk.rhigh, pl, leaf, fl)) return 0;
-  // b^mu = 0 (a field-free cell): the pitch angle is pi/2 (tetrad.hpp:144) and the plasma-frame
-  // basis is 0/0, so every frequency's operator is non-finite; known without building the tetrad
   if (fl.b[0] == T(0) && fl.b[1] == T(0) && fl.b[2] == T(0) && fl.b[3] == T(0)) return 2;
   const T pa = pitch_angle(fl.g, k, fl.b, fl.U);
   if (pa == T(0) || pa == T(PI_D)) return 0;
   T perp[4], para[4];
   frame(fl.g, A.P.a, k, fl.U, fl.b, perp, para);
   if constexpr (M == MIX) {
-    // pf holds the ray's Walker-Penrose constant (StHam)
     wp_angle(A.P.a, fl.g, k, perp, para, pf[0], pf[1], &sp.c2, &sp.s2);
   } else {
     const T Cp = dot(fl.g, perp, pf), Sp = dot(fl.g, para, pf);  // chi_from_F1 (tetrad.hpp:160-182)
@@ -361,18 +443,19 @@ PHX_DEV void accumulate(T AA[16], T SS[4], T SC[4], const T OA[16], const T OS[4
 //
 // QUEUE: a per-wave sample queue in LDS.  A lane whose pending sample can emit pushes it
 // (position, wavevector, polarization vector or Walker-Penrose constant, step length, cache
-// leaf, task) into its wave's queue instead of processing it at once.  When the queue holds 64
-// samples, when a lane holds KMAX of them, or when a ray with queued samples ends, all active
-// lanes of the wave work through the queue together, one sample per lane: first the
-// frequency-independent part of every sample, then, on the compacted list of the samples that
-// emit, the segment operators; each owning lane folds its own operators into its accumulator in
-// the order it queued them, which is the reference's order (pixel.hpp:186-188).  The divergent,
-// expensive part of a sample (fluid, tetrad, coefficients, 4x4 inverse) thus runs on full waves.
-// Without QUEUE the lane processes its sample in place.
+// leaf, task, owner lane) into its wave's queue instead of processing it at once.  When the
+// queue holds 64 samples, or when a ray with samples still queued ends, all active lanes of the
+// wave work through the queue together, one sample per lane: first the fluid half of every
+// sample; then, on the candidates for emission (these are a minority: most samples lie outside
+// the cuts or in field-free cells) in whole waves of 64, the tetrad half and the segment
+// operators.  Each owning lane folds its own operators into its accumulator in the order it
+// queued them, which is the reference's order (pixel.hpp:186-188).  The divergent, expensive
+// parts of a sample (fluid, tetrad, coefficients, 4x4 inverse) thus run on full waves.  Without
+// QUEUE the lane processes its sample in place.
 constexpr int QCAP = 128;  // slots per wave (at most 63 + 64 are in use at a flush)
 constexpr int QW = 24;     // scalars per slot: 21 in (x, k, F1, dl, leaf, FP64 position, task), then the
                            // prepared sample, then the operators; 23 = position in the emitting list
-constexpr int KMAX = 4;    // queued samples a lane may hold
+constexpr int LCAP = 64;   // leftover candidate slots per wave (fewer than a wave's lanes remain)
 
 PHX_DEV void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -391,11 +474,23 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
   using St = StateOf<M>;
   using R = RecOf<M>;
   constexpr bool KAHAN = M == MIX;
-  __shared__ __align__(16) T qbuf[QUEUE ? BLOCK / 64 : 1][QUEUE ? QCAP * QW : 1];
-  __shared__ int clbuf[QUEUE ? BLOCK / 64 : 1][QUEUE ? QCAP : 1];  // per wave: the emitting samples of a flush
-  T* const Q = qbuf[QUEUE ? (threadIdx.x >> 6) : 0];
-  int* const cl = clbuf[QUEUE ? (threadIdx.x >> 6) : 0];
+  constexpr int NW = QUEUE ? BLOCK / 64 : 1;  // per-wave LDS areas of the sample queue
+  __shared__ __align__(16) T qbuf[NW][QUEUE ? QCAP * QW : 1];  // queued samples, then their candidates
+  __shared__ __align__(16) T lqbuf[NW][QUEUE ? LCAP * QW : 1];  // candidates left over from earlier flushes
+  __shared__ int clbuf[NW][QUEUE ? QCAP : 1];                  // slots of this flush's candidates, in order
+  __shared__ unsigned long long ombuf[NW][QUEUE ? 64 : 1];     // per owner: the lanes holding its records
+  __shared__ int pcbuf[NW][QUEUE ? 64 : 1];                    // per owner: candidates made in a flush
+  const int wv = QUEUE ? (threadIdx.x >> 6) : 0;
+  T* const Q = qbuf[wv];
+  T* const LQ = lqbuf[wv];
+  int* const cl = clbuf[wv];
+  unsigned long long* const om = ombuf[wv];
+  int* const pc = pcbuf[wv];
   const int lane = threadIdx.x & 63;
+  if constexpr (QUEUE) {
+    om[lane] = 0ull;
+    pc[lane] = 0;
+  }
   const unsigned long long lt = (1ull << lane) - 1ull;
   St st;
   T px[4] = {T(0), T(0), T(0), T(0)}, pk[4] = {T(0), T(0), T(0), T(0)}, pf[4] = {T(0), T(0), T(0), T(0)};
@@ -407,10 +502,9 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
   bool bneg = false, bpos = false, have = false, stats = false;
   unsigned s_steps = 0, s_np = 0, s_bad = 0;  // per-lane sums (a lane marches well under 2^32 steps)
   unsigned s_fp64 = 0;                        // MIX: steps taken in FP64
-  int qcount = 0, nmine = 0;                  // qcount is the same in every active lane of the wave
-  int myslot[KMAX];
-#pragma unroll
-  for (int j = 0; j < KMAX; j++) myslot[j] = 0;
+  // queue bookkeeping: qcount and nleft are the same in every active lane of the wave; nmine and
+  // ncm count this lane's queued samples and its candidates not yet folded in
+  int qcount = 0, nleft = 0, nmine = 0, ncm = 0;
 
   for (;;) {
     if (!have) {
@@ -495,9 +589,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
           for (int q = 0; q < 3; q++) e64[q] = pl[1 + q];
         }
         e[20] = T(task);
-#pragma unroll
-        for (int j = 0; j < KMAX; j++)
-          if (j == nmine) myslot[j] = slot;
+        e[21] = T(lane);
         nmine++;
       }
       qcount += (int)__popcll(pm);
@@ -547,23 +639,23 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
     }
 
     if constexpr (QUEUE) {
-      // ---- flush, in two passes over the queue.  Pass 1: all active lanes take the queued
-      // samples one per lane and compute their frequency-independent part (fluid, frame,
-      // polarization angle, Bessel ratios), which most samples end in (no emission, or a
-      // field-free cell); the emitting ones are listed in queue order.  Pass 2: all active lanes
-      // take the listed samples one per lane and compute the segment operators of each
-      // frequency; after each frequency the owners fold their operators in in queue order.
-      const bool want = qcount >= QCAP - 64 || nmine >= KMAX || (!alive && nmine > 0);
+      // ---- flush.  Pass 1: all active lanes take the queued samples, one per lane, and run their
+      // first half (fluid, cuts), which most samples end in; the candidates for emission stay in
+      // their slots, listed in queue order after the candidates left over from earlier flushes.
+      // Pass 2: all active lanes take candidates, one per lane, in whole waves of them (the
+      // remainder waits for the next flush, unless a ray that ended needs its own), run the second
+      // half (tetrad, polarization angle, Bessel ratios) and each frequency's segment operator;
+      // after each frequency the owners fold their operators in, in queue order.  Run statistics
+      // are counted by the lane that computes the sample.
+      const bool want = qcount >= QCAP - 64 || (!alive && (nmine > 0 || ncm > 0));
       if (__ballot(want) != 0ull) {
         const unsigned long long act = __ballot(1);
         const int nact = (int)__popcll(act), rank = (int)__popcll(act & lt);
-        int* const clist = cl;
         int ncomp = 0;  // the same in every active lane
         for (int base = 0; base < qcount; base += nact) {
           const int e = base + rank;
-          const bool mine = e < qcount;
           int status = 0;
-          if (mine) {
+          if (e < qcount) {
             T* const in = Q + e * QW;
             T x[4], k[4], f1[4];
             L xl[4];
@@ -579,104 +671,115 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
 #pragma unroll
               for (int q = 0; q < 3; q++) xl[1 + q] = e64[q];
             }
-            const T etask = in[20];
-            Prep<T> sp;
-            status = sample_prepare<M>(A, A.tasks[(int)etask], x, xl, k, f1, in[12], (int)in[13], sp);
-            // the slot now holds the prepared sample: Prep at 0..10, task 11, status 12
-            in[0] = sp.te;
-            in[1] = sp.ne;
-            in[2] = sp.B;
-            in[3] = sp.sth;
-            in[4] = sp.cth;
-            in[5] = sp.kr1;
-            in[6] = sp.kr0;
-            in[7] = sp.c2;
-            in[8] = sp.s2;
-            in[9] = sp.ratio;
-            in[10] = sp.dl;
-            in[11] = etask;
-            in[12] = T(status);
+            const int etask = (int)in[20], owner = (int)in[21];
+            const TaskDev<T>& tk = A.tasks[etask];
+            // the candidate record overwrites the slot; everything of the queued sample is read first
+            const T sdl = in[12];
+            const int sleaf = (int)in[13];
+            status = sample_fluid_stage<M>(A, tk, x, xl, k, f1, sdl, sleaf, etask, in);
+            if (status == 2) {
+              // every frequency's operator non-finite: dropped by the reference (pixel.hpp:107-116)
+              if (tk.stats) s_np++;
+              s_bad += tk.nf;
+            } else if (status == 3) {
+              in[23] = T(owner);
+              atomicAdd(&pc[owner], 1);
+            }
           }
-          const unsigned long long em = __ballot(status == 1);
-          const int ci = ncomp + (int)__popcll(em & lt);
-          if (mine) {
-            Q[e * QW + 23] = T(status == 1 ? ci : -1);  // position in the emitting list
-            if (status == 1) clist[ci] = e;
-          }
+          const unsigned long long em = __ballot(status == 3);
+          if (status == 3) cl[ncomp + (int)__popcll(em & lt)] = e;
           ncomp += (int)__popcll(em);
         }
         wave_sync();
-        // the owners count their samples; a sample whose rotation or frequency ratio is NaN has
-        // every frequency's operator non-finite, dropped by the reference (pixel.hpp:107-116)
+        ncm += pc[lane];
+        pc[lane] = 0;
+        const int ncand = nleft + ncomp;
+        const bool force = __ballot(!alive && ncm > 0) != 0ull;
+        const int nproc = force ? ncand : (ncand / nact) * nact;
+        for (int cb = 0; cb < nproc; cb += nact) {
+          const int pos = cb + rank;
+          const bool mine = pos < nproc;
+          T* const rec = pos < nleft ? LQ + pos * QW : Q + cl[mine ? pos - nleft : 0] * QW;
+          const int owner = mine ? (int)rec[23] : 0;
+          // the lanes holding each owner's records: intersect the ballots of the owner id's bits
+          unsigned long long same = __ballot(mine);
 #pragma unroll
-        for (int j = 0; j < KMAX; j++) {
-          if (j < nmine) {
-            const int status = (int)Q[myslot[j] * QW + 12];
-            if (status != 0) np++;
-            if (status == 2) bad += nf;
+          for (int bit = 0; bit < 6; bit++) {
+            const bool on = (owner >> bit) & 1;
+            const unsigned long long m = __ballot(mine && on);
+            same &= on ? m : ~m;
           }
-        }
-        for (int cb = 0; cb < ncomp; cb += nact) {
-          const int ci = cb + rank;
-          const bool mine = ci < ncomp;
+          if (mine && (same & lt) == 0ull) om[owner] = same;
           Prep<T> sp;
-          int e = 0, etask = 0, enf = 0;
+          int status = 0, enf = 0;
+          int ctask = 0;
           if (mine) {
-            e = clist[ci];
-            const T* in = Q + e * QW;
-            sp.te = in[0];
-            sp.ne = in[1];
-            sp.B = in[2];
-            sp.sth = in[3];
-            sp.cth = in[4];
-            sp.kr1 = in[5];
-            sp.kr0 = in[6];
-            sp.c2 = in[7];
-            sp.s2 = in[8];
-            sp.ratio = in[9];
-            sp.dl = in[10];
-            etask = (int)in[11];
-            enf = A.tasks[etask].nf;
+            ctask = (int)rec[22];
+            status = sample_frame_stage<M>(A, rec, sp);
+            const TaskDev<T>& tk = A.tasks[ctask];
+            enf = tk.nf;
+            if (status != 0 && tk.stats) s_np++;
+            if (status == 2) s_bad += enf;
           }
+          wave_sync();
+          const unsigned long long myrec = om[lane];
+          om[lane] = 0ull;
 #pragma unroll
           for (int f = 0; f < NF; f++) {
             if (mine && f < enf) {
-              T* out = Q + e * QW;
-              T OA[16], OS[4], dtau, dtauF;
-              const bool fin = sample_op(A.tasks[etask], sp, f, OA, OS, &dtau, &dtauF);
-              if (fin) {
+              int flag = 0;  // 1: fold in; 0: nothing to fold (no emission, or dropped)
+              if (status == 1) {
+                T OA[16], OS[4], dtau, dtauF;
+                if (sample_op(A.tasks[ctask], sp, f, OA, OS, &dtau, &dtauF)) {
 #pragma unroll
-                for (int q = 0; q < 16; q++) out[q] = OA[q];
+                  for (int q = 0; q < 16; q++) rec[q] = OA[q];
 #pragma unroll
-                for (int q = 0; q < 4; q++) out[16 + q] = OS[q];
-                out[20] = dtau;
-                out[21] = dtauF;
+                  for (int q = 0; q < 4; q++) rec[16 + q] = OS[q];
+                  rec[20] = dtau;
+                  rec[21] = dtauF;
+                  flag = 1;
+                } else {
+                  s_bad++;
+                }
               }
-              out[22] = T(fin ? 1 : 2);  // 1: fold in, 2: dropped as non-finite
+              rec[22] = T(flag);
             }
             wave_sync();
+            unsigned long long mm = myrec;
+            while (mm != 0ull) {
+              const int hl = __ffsll((long long)mm) - 1;
+              mm &= mm - 1ull;
+              const int pp = cb + (int)__popcll(act & ((1ull << hl) - 1ull));
+              const T* rr = pp < nleft ? LQ + pp * QW : Q + cl[pp - nleft] * QW;
+              if (f < nf && rr[22] == T(1)) {
+                T OA[16], OS[4];
 #pragma unroll
-            for (int j = 0; j < KMAX; j++) {
-              const T* rr = Q + myslot[j] * QW;
-              const int pos = j < nmine ? (int)rr[23] : -1;
-              if (pos >= cb && pos < cb + nact && f < nf) {
-                if (rr[22] == T(1)) {
-                  T OA[16], OS[4];
+                for (int q = 0; q < 16; q++) OA[q] = rr[q];
 #pragma unroll
-                  for (int q = 0; q < 16; q++) OA[q] = rr[q];
-#pragma unroll
-                  for (int q = 0; q < 4; q++) OS[q] = rr[16 + q];
-                  accumulate<T, KAHAN>(AA[f], SS[f], SC[f], OA, OS);
-                  tau[f] += rr[20];
-                  tauF[f] += rr[21];
-                } else {
-                  bad++;
-                }
+                for (int q = 0; q < 4; q++) OS[q] = rr[16 + q];
+                accumulate<T, KAHAN>(AA[f], SS[f], SC[f], OA, OS);
+                tau[f] += rr[20];
+                tauF[f] += rr[21];
               }
             }
             wave_sync();
           }
+          ncm -= (int)__popcll(myrec);
         }
+        // the candidates not processed move to the front of the leftover area (sources lie at or
+        // after their destinations, or in the queue slots, so the copy needs no ordering)
+        const int nrem = ncand - nproc;
+        if (rank < nrem) {
+          const int pos = nproc + rank;
+          if (pos >= nleft || nproc > 0) {
+            const T* src = pos < nleft ? LQ + pos * QW : Q + cl[pos - nleft] * QW;
+            T* dst = LQ + rank * QW;
+#pragma unroll
+            for (int q = 0; q < QW; q++) dst[q] = src[q];
+          }
+        }
+        wave_sync();
+        nleft = nrem;
         qcount = 0;
         nmine = 0;
       }
@@ -689,7 +792,9 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
       for (int f = 0; f < NF; f++) {
         if (f >= nf) break;
         double2* o = reinterpret_cast<double2*>(o0 + f);
-        if (np > 0) {
+        // with the queue, np is counted where the samples are computed; a ray with no plasma
+        // segment has folded nothing in, so its sums are the zeros written otherwise
+        if (QUEUE || np > 0) {
           o[0] = make_double2((double)SS[f][0], (double)SS[f][1]);
           o[1] = make_double2((double)SS[f][2], (double)SS[f][3]);
           o[2] = make_double2((double)tau[f], (double)tauF[f]);
