@@ -59,17 +59,21 @@ def workload_config(tmpdir: str, preset: str = "c2", name: str = "bench"):
 
 def algorithmic_flops(info: dict, nf: int, precision: str) -> tuple:
     """(FP32 flops, FP64 flops) of one launch: counted ops per step and per emitting segment x
-    frequency (the oracle's operation count of this workload) times the launch's own counts.
-    The mixed path's FP64 steps are counted as FP64 work."""
+    frequency (tests/golden/op_counts.json: the oracle's count of the reference's method for the
+    FP32 and FP64 paths, the Hamiltonian twin's count for the mixed path) times the launch's own
+    step / segment / dropped-segment counts.  The mixed path's FP64 steps are FP64 work."""
     with open(os.path.join(ROOT, "tests", "golden", "op_counts.json")) as f:
         oc = json.load(f)["c2_1024_230GHz"]
     useful_segf = info["nplasma"] * nf - info["bad"]
     if precision == "fp64":
         o = oc["fp64"]
         return 0.0, info["steps"] * o["F_step"] + useful_segf * o["F_segf"]
-    o32, o64 = oc["fp32"], oc["fp64"]
-    s64 = info.get("fp64_steps", 0) if precision == "mixed" else 0
-    return (info["steps"] - s64) * o32["F_step"] + useful_segf * o32["F_segf"], s64 * o64["F_step"]
+    if precision == "mixed":
+        o = oc["mixed"]
+        s64 = info.get("fp64_steps", 0)
+        return (info["steps"] - s64) * o["F_step"] + useful_segf * o["F_segf"], s64 * o["F_step"]
+    o = oc["fp32"]
+    return info["steps"] * o["F_step"] + useful_segf * o["F_segf"], 0.0
 
 
 def measured_traffic(precision: str):

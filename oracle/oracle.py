@@ -98,6 +98,10 @@ def lib():
                                  ctypes.c_double, dp, dp]
         L.orc_geodesic.argtypes = [ctypes.c_double, ctypes.c_double, dp, ctypes.c_double, ctypes.c_int, dp,
                                    ctypes.c_int]
+        L.orc_render_ham.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long), ctypes.c_long, dp,
+                                     ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+        L.orc_wp_kappa.argtypes = [ctypes.c_double, ctypes.c_double, dp, dp, dp, dp]
+        L.orc_ham_rhs.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp, dp, dp]
         L.orc_step_len.restype = ctypes.c_double
         L.orc_step_len.argtypes = [ctypes.c_double, dp, dp, dp]
         _lib = L
@@ -145,6 +149,20 @@ class Oracle:
                          threads)
         return res, np.array(list(st), dtype=np.int64)
 
+    def render_ham(self, pixels=None, threads: int = 0):
+        """The Hamiltonian twin (ham_twin.cpp: Hamiltonian geodesic, Walker-Penrose polarization)
+        on a handle of precision 64 or 164.  Returns (res [npix, nf, 6], stats)."""
+        if pixels is None:
+            pixels = np.arange(self.W * self.H, dtype=np.int64)
+        pixels = np.ascontiguousarray(pixels, dtype=np.int64)
+        res = np.zeros((pixels.size, self.nf, 6))
+        st = (ctypes.c_longlong * 3)()
+        rc = lib().orc_render_ham(self.h, pixels.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), pixels.size,
+                                  ptr(res), st, threads)
+        if rc:
+            raise RuntimeError("the Hamiltonian twin runs at precision 64 or 164")
+        return res, np.array(list(st), dtype=np.int64)
+
     def info(self):
         iv = np.zeros(9, dtype=np.int32)
         dv = np.zeros(7)
@@ -176,6 +194,22 @@ class Oracle:
         out = np.zeros((len(X), 14))
         lib().orc_fluid(self.h, ptr(X), len(X), ptr(out))
         return out
+
+
+def wp_kappa(a: float, hs: float, X, k, f) -> np.ndarray:
+    """Walker-Penrose constant (kappa_1, kappa_2) of (k, f) at X (MKS-BHAC Kerr-Schild components)."""
+    X, k, f = (np.ascontiguousarray(v, dtype=np.float64) for v in (X, k, f))
+    q = np.zeros(2)
+    lib().orc_wp_kappa(a, hs, ptr(X), ptr(k), ptr(f), ptr(q))
+    return q
+
+
+def ham_rhs(a: float, hs: float, pt: float, pp: float, y):
+    """Hamilton's equations at y = (x1, x2, x3, p_1, p_2): returns (dy/dlambda, k^mu)."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    d, k = np.zeros(5), np.zeros(4)
+    lib().orc_ham_rhs(a, hs, pt, pp, ptr(y), ptr(d), ptr(k))
+    return d, k
 
 
 def op_counts() -> np.ndarray:

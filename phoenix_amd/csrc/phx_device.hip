@@ -651,11 +651,15 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
       if (__ballot(want) != 0ull) {
         const unsigned long long act = __ballot(1);
         const int nact = (int)__popcll(act), rank = (int)__popcll(act & lt);
+        // whole waves of queued samples, oldest first, unless a ray that ended needs its own: the
+        // newest samples (fewer than a wave) wait for the next flush, in queue order
+        const bool force = __ballot(!alive && (nmine > 0 || ncm > 0)) != 0ull;
+        const int n1 = force ? qcount : (qcount / nact) * nact;
         int ncomp = 0;  // the same in every active lane
-        for (int base = 0; base < qcount; base += nact) {
+        for (int base = 0; base < n1; base += nact) {
           const int e = base + rank;
           int status = 0;
-          if (e < qcount) {
+          if (e < n1) {
             T* const in = Q + e * QW;
             T x[4], k[4], f1[4];
             L xl[4];
@@ -683,19 +687,24 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
               s_bad += tk.nf;
             } else if (status == 3) {
               in[23] = T(owner);
-              atomicAdd(&pc[owner], 1);
             }
+            // tell the owner: one queued sample done (high half), one candidate made (low half)
+            atomicAdd(&pc[owner], status == 3 ? 0x10001 : 0x10000);
           }
           const unsigned long long em = __ballot(status == 3);
           if (status == 3) cl[ncomp + (int)__popcll(em & lt)] = e;
           ncomp += (int)__popcll(em);
         }
         wave_sync();
-        ncm += pc[lane];
-        pc[lane] = 0;
+        {
+          const int v = pc[lane];
+          pc[lane] = 0;
+          ncm += v & 0xffff;
+          nmine -= v >> 16;
+        }
         const int ncand = nleft + ncomp;
-        const bool force = __ballot(!alive && ncm > 0) != 0ull;
-        const int nproc = force ? ncand : (ncand / nact) * nact;
+        const bool force2 = __ballot(!alive && ncm > 0) != 0ull;
+        const int nproc = force2 ? ncand : (ncand / nact) * nact;
         for (int cb = 0; cb < nproc; cb += nact) {
           const int pos = cb + rank;
           const bool mine = pos < nproc;
@@ -780,8 +789,18 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
         }
         wave_sync();
         nleft = nrem;
-        qcount = 0;
-        nmine = 0;
+        // the queued samples not processed move to the front of the queue (after the candidates in
+        // those slots have been moved out; more slots were processed than remain, so the copy needs
+        // no ordering)
+        const int qrem = qcount - n1;
+        if (rank < qrem) {
+          const T* src = Q + (n1 + rank) * QW;
+          T* dst = Q + rank * QW;
+#pragma unroll
+          for (int q = 0; q < QW; q++) dst[q] = src[q];
+        }
+        wave_sync();
+        qcount = qrem;
       }
     }
 

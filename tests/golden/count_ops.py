@@ -8,6 +8,10 @@ sinh, fmod) and comparisons.  Phases: ray setup, geodesic step (view, AMR lookup
 step rule, RK4), plasma sample (frequency independent part of process_sample),
 and sample x frequency (coefficients, operator, accumulation).
 
+The "mixed" entry counts the Hamiltonian twin (oracle/ham_twin.cpp), the formulation the
+mixed-precision march runs: Hamiltonian geodesic steps and the Walker-Penrose polarization angle,
+with the same sampler, transfer and accumulation.
+
 The per-unit constants that the benchmark's roofline uses:
   F_step   = (setup + step + sample ops) / geodesic steps
   F_segf   = (sample x frequency ops) / (plasma segments x frequencies)
@@ -39,14 +43,14 @@ OPS = ["add", "mul", "div", "sqrt", "trans", "cmp"]
 PHASES = ["setup", "step", "sample", "freq"]
 
 
-def count(preset: str, precision: int, npix: int, W: int = 1024, seed: int = 7, **params):
+def count(preset: str, precision: int, npix: int, W: int = 1024, seed: int = 7, ham: bool = False, **params):
     dump, grid, info = S.ensure_snapshot(preset, S.default_dir())
     o = O.Oracle(dump, grid, precision, img_width=W, img_height=W, **params)
     rng = np.random.default_rng(seed)
     pix = np.sort(rng.choice(W * W, size=npix, replace=False))
     O.op_counts_reset()
     t0 = time.time()
-    res, st = o.render(pix, threads=1)
+    res, st = o.render_ham(pix, threads=1) if ham else o.render(pix, threads=1)
     dt = time.time() - t0
     c = O.op_counts()
     o.close()
@@ -70,7 +74,8 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--pixels", type=int, default=600)
     a = ap.parse_args()
-    out = {"c2_1024_230GHz": {"fp32": count("c2", 132, a.pixels), "fp64": count("c2", 164, a.pixels)}}
+    out = {"c2_1024_230GHz": {"fp32": count("c2", 132, a.pixels), "fp64": count("c2", 164, a.pixels),
+                              "mixed": count("c2", 164, a.pixels, ham=True)}}
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1)
     for k, v in out["c2_1024_230GHz"].items():
