@@ -543,7 +543,16 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
     st.view(cx, ck, cf);
     const T r = exp_r(cx[1]);
     const bool alive = r < A.r_out && r > A.cut_in && steps < A.max_steps && !stop;
-    if constexpr (M == MIX) st.begin(A, alive, ck);
+    if constexpr (M == MIX) {
+      st.begin(A, alive, ck);
+      // the pending sample's FP32 position and polarization constant follow from its FP64
+      // position and the ray's constant: not carried from pass to pass
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        px[q] = T(pl[q]);
+        pf[q] = cf[q];
+      }
+    }
     const T rs = exp_r(px[1]);  // radius of the pending sample
     // the pending sample is the state before the last step, i.e. sample number steps - 1; the
     // camera sample (number 0) is never processed (pixel.hpp:186).  A sample at r >= rt_outer
@@ -621,9 +630,11 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
 
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      px[q] = cx[q];
       pk[q] = ck[q];
-      pf[q] = cf[q];
+      if constexpr (M != MIX) {
+        px[q] = cx[q];
+        pf[q] = cf[q];
+      }
     }
     st.lpos(cx, pl);
     pdl = dlp;
