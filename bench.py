@@ -317,28 +317,30 @@ def main() -> int:
                       "algorithmic_tflop_per_launch": sum(algorithmic_flops(info, nf, args.precision)) / 1e12,
                       "frac_of_unpacked_fp32_peak": roof["achieved"] / 78.6})
         log(f"timed {args.steps} renders ({args.precision}): {wall:.3f} s, {value:.4g} rays*freq/s")
-        # the same image by the other paths; "plain_fp32" is the reference's method as the
-        # reference's kernel runs it (samples processed in place)
-        others = [("fp32", capi.FP32, False), ("plain_fp32", capi.FP32, True), ("fp64", capi.FP64, False),
-                  ("mixed", capi.MIXED, False)]
-        for name, p2, plain in others:
+        # the same image by the other paths.  "plain_fp32" is the reference's FP32 method as the
+        # reference's kernel runs it: each lane marches its pixel with the samples processed in place,
+        # and a wave takes new pixels when all its lanes are done (kernel.cu:80-96 on a wave).  The
+        # "_queue" variants refill each lane at once and process the samples through the per-wave
+        # LDS queue.
+        others = [("plain_fp32", capi.FP32, {"queue": 0, "refill": 64}), ("fp32_queue", capi.FP32, {"queue": 1}),
+                  ("fp64", capi.FP64, None), ("mixed", capi.MIXED, None), ("mixed_queue", capi.MIXED, {"queue": 1})]
+        for name, p2, tune in others:
             if name == args.precision:
                 continue
             steps = 1 if name == "fp64" else args.steps
-            for i in range(1):
-                snap.render(None, p2, to_host=False, plain=plain)
+            snap.render(None, p2, to_host=False, tune=tune)
             sync()
             t0 = time.perf_counter()
             k2, i2 = [], None
             for _ in range(steps):
-                _, i2 = snap.render(None, p2, to_host=False, plain=plain)
+                _, i2 = snap.render(None, p2, to_host=False, tune=tune)
                 k2.append(i2["kernel_s"])
             sync()
             w2 = time.perf_counter() - t0
             extra[f"{name}_rays_freq_per_s"] = steps * rays / w2
             extra[f"{name}_kernel_ms"] = float(np.mean(k2)) * 1e3
-            extra[f"{name}_roofline_frac"] = roofline(i2, nf, "fp32" if name == "plain_fp32" else name,
-                                                      float(np.mean(k2)))["frac"]
+            extra[f"{name}_roofline_frac"] = roofline(i2, nf, name.split("_")[-2] if name.endswith("_queue") else
+                                                      name.split("_")[-1], float(np.mean(k2)))["frac"]
         if "plain_fp32_rays_freq_per_s" in extra:
             extra["value_vs_plain_fp32"] = value / extra["plain_fp32_rays_freq_per_s"]
         if not args.quick:

@@ -204,10 +204,12 @@ int phx_snapshot_render_batch(phx_snapshot* snap, const phx_model* const* reques
 
 /* launch configuration of batch renders */
 typedef struct phx_tune {
-  int32_t queue;         /* -1 default, 0 samples processed in place, 1 per-wave LDS sample queue */
+  int32_t queue;         /* -1 default (in place), 0 samples processed in place, 1 per-wave LDS sample queue */
   int32_t blocks_per_cu; /* 0 = occupancy limit */
   int32_t order;         /* 0 request after request, 1 requests interleaved one wave (64 jobs) at a time */
   int32_t fp64_wave;     /* PHX_MIXED: 1 = a lane's FP64 step takes its whole wave to FP64 */
+  int32_t refill;        /* samples in place: idle lanes a wave gathers before they take new jobs;
+                            64 (or 0, the default) = the whole wave, 1 = each lane at once; the queue uses 1 */
 } phx_tune;
 /* Autotuner: times every configuration on every sample_stride-th wave of the batch's jobs,
  * keeps the fastest for the snapshot's batch renders, returns it and its sampled rays*freq/s.

@@ -70,6 +70,7 @@ struct MarchArgs {
   T stepsize, r_out, cut_in;
   double a64, hs64, lnr_fp64, x2_pole;  // MIX: FP64 steps where ln r < lnr_fp64 or x2 within x2_pole of a pole
   int fp64_wave;                        // MIX: a lane's FP64 step takes its whole wave to FP64
+  int refill;                           // idle lanes a wave gathers before they take new jobs (1: at once)
   int max_steps, max_order;
   const TaskDev<T>* tasks;
   const int2* jobs;
@@ -480,6 +481,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
   __shared__ int clbuf[NW][QUEUE ? QCAP : 1];                  // slots of this flush's candidates, in order
   __shared__ unsigned long long ombuf[NW][QUEUE ? 64 : 1];     // per owner: the lanes holding its records
   __shared__ int pcbuf[NW][QUEUE ? 64 : 1];                    // per owner: candidates made in a flush
+  __shared__ int qsbuf[NW][2];  // the wave's queue counts (qcount, nleft), for lanes that sat out passes
   const int wv = QUEUE ? (threadIdx.x >> 6) : 0;
   T* const Q = qbuf[wv];
   T* const LQ = lqbuf[wv];
@@ -487,9 +489,12 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
   unsigned long long* const om = ombuf[wv];
   int* const pc = pcbuf[wv];
   const int lane = threadIdx.x & 63;
+  int* const qs = qsbuf[wv];
   if constexpr (QUEUE) {
     om[lane] = 0ull;
     pc[lane] = 0;
+    if (lane == 0) qs[0] = qs[1] = 0;
+    wave_sync();
   }
   const unsigned long long lt = (1ull << lane) - 1ull;
   St st;
@@ -507,7 +512,17 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
   int qcount = 0, nleft = 0, nmine = 0, ncm = 0;
 
   for (;;) {
-    if (!have) {
+    // refill: a lane whose ray ended waits until at least A.refill lanes of its wave are idle (or
+    // none is busy), and those lanes then take consecutive jobs together.  1 refills each lane at
+    // once; 64 refills the wave only when all its lanes are done, which is how the reference
+    // kernel's per-thread job loop (kernel.cu:80-96) runs on a wave whose lanes share one
+    // instruction stream.  Rays started together stay coherent (neighbouring pixels).
+    bool wait = false;
+    if (A.refill > 1) {
+      const unsigned long long busy = __ballot(have);
+      wait = busy != 0ull && 64 - (int)__popcll(busy) < A.refill;
+    }
+    if (!have && !wait) {
       const unsigned long long m = __ballot(1);
       const int leader = __ffsll((long long)m) - 1;
       int base = 0;
@@ -535,6 +550,12 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
       bneg = bsign < T(0);
       bpos = bsign > T(0);
       have = true;
+    }
+    if (!have) continue;  // waiting for the wave's refill
+    if constexpr (QUEUE) {
+      // a lane that waited for the refill skipped the queue updates of the passes it sat out
+      qcount = qs[0];
+      nleft = qs[1];
     }
 
     // ---- view of the current state, liveness, and the one AMR lookup of this pass: it
@@ -813,6 +834,9 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
         wave_sync();
         qcount = qrem;
       }
+      qs[0] = qcount;
+      qs[1] = nleft;
+      wave_sync();
     }
 
     if (!alive) {
@@ -1001,10 +1025,11 @@ struct Tables {
 
 // Launch knobs (picked by the autotuner or from the environment)
 struct Tune {
-  int queue = -1;         // -1: default (the LDS queue), 0: samples in place, 1: LDS queue
+  int queue = -1;         // -1: default (samples in place), 0: samples in place, 1: LDS queue
   int blocks_per_cu = 0;  // 0: the occupancy limit
   int order = 0;          // batch job order: 0 request after request, 1 requests interleaved per wave
   int fp64_wave = 1;      // mixed march: 1 = a lane's FP64 step takes its whole wave to FP64
+  int refill = 64;        // idle lanes a wave gathers before they take new jobs (64: the whole wave)
 };
 // grow-only device buffer
 struct DevBuf {
@@ -1311,6 +1336,8 @@ static Tune tune_from_env(Tune t) {
   if (const char* e = std::getenv("PHX_BLOCKS_PER_CU"))
     if (std::atoi(e) > 0) t.blocks_per_cu = std::atoi(e);
   if (const char* e = std::getenv("PHX_FP64_WAVE")) t.fp64_wave = std::atoi(e) != 0;
+  if (const char* e = std::getenv("PHX_REFILL"))
+    t.refill = !std::strcmp(e, "wave") ? 64 : !std::strcmp(e, "lane") ? 1 : std::atoi(e);
   return t;
 }
 
@@ -1348,6 +1375,7 @@ static void launch_march(phx_snapshot* s, const LaunchInfo& li, int j0, int j1, 
   A.lnr_fp64 = std::log(2. * (1. + std::sqrt(std::max(0., 1. - m.a * m.a))));  // r < 2 r_H
   A.x2_pole = pole_x2_limit(m.hslope);                                          // |cos theta| > 0.99
   A.fp64_wave = tn.fp64_wave;
+  A.refill = Q ? 1 : std::min(64, std::max(1, tn.refill));  // the queue refills each lane at once
   A.max_steps = rq->max_steps;
   A.max_order = rq->max_order;
   A.tasks = (const TaskDev<T>*)s->tasks.p;
@@ -1389,7 +1417,9 @@ static void launch_march(phx_snapshot* s, const LaunchInfo& li, int j0, int j1, 
 
 template <int M, int NF>
 static void launch_nf(phx_snapshot* s, const LaunchInfo& li, int j0, int j1, const Tune& tn, phx_info* info) {
-  const bool q = tn.queue != 0;  // default: the LDS sample queue (measured faster in every precision)
+  // default: samples in place with whole-wave refill (measured faster in every precision: rays of
+  // one 8 x 8 pixel tile stay coherent; the LDS queue pays off only with per-lane refill)
+  const bool q = tn.queue == 1;
   if (q)
     launch_march<M, NF, true>(s, li, j0, j1, tn, info);
   else
@@ -1756,11 +1786,13 @@ static void autotune(phx_snapshot* s, const phx_model* const* rqs, int n, int pr
   double rate_best = -1.;
   Tune tb;
   for (int q = 0; q <= 1; q++)
+    for (int rf = 0; rf <= (q ? 0 : 1); rf++)
     for (int bpc = 1; bpc <= max_bpc; bpc++)
       for (int order = 0; order <= (n > 1 ? 1 : 0); order++)
         for (int fw = 0; fw <= (precision == PHX_MIXED ? 1 : 0); fw++) {
         Tune t;
         t.queue = q;
+        t.refill = (q || rf) ? 1 : 64;
         t.blocks_per_cu = bpc;
         t.order = order;
         t.fp64_wave = fw;
@@ -1778,6 +1810,7 @@ static void autotune(phx_snapshot* s, const phx_model* const* rqs, int n, int pr
     best->blocks_per_cu = tb.blocks_per_cu;
     best->order = tb.order;
     best->fp64_wave = tb.fp64_wave;
+    best->refill = tb.refill;
   }
   if (best_rate) *best_rate = rate_best;
 }
@@ -2086,12 +2119,13 @@ int phx_snapshot_set_tune(phx_snapshot* s, const phx_tune* t) {
   return guarded([&] {
     if (!s || !t) throw Error(PHX_E_ARG, "null snapshot or tune");
     if (t->queue < -1 || t->queue > 1 || t->blocks_per_cu < 0 || t->order < 0 || t->order > 1 || t->fp64_wave < 0 ||
-        t->fp64_wave > 1)
+        t->fp64_wave > 1 || t->refill < 0 || t->refill > 64)
       throw Error(PHX_E_ARG, "tune value out of range");
     s->tune.queue = t->queue;
     s->tune.blocks_per_cu = t->blocks_per_cu;
     s->tune.order = t->order;
     s->tune.fp64_wave = t->fp64_wave;
+    s->tune.refill = t->refill > 0 ? t->refill : 64;
   });
 }
 

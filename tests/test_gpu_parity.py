@@ -135,6 +135,30 @@ def test_resident_snapshot_reuse_and_partial_jobs(c1_run):
     snap.close()
 
 
+@pytest.mark.parametrize("precision", [capi.FP32, capi.MIXED])
+@pytest.mark.parametrize("refill", [1, 16])
+def test_wave_synchronous_refill_matches_lane_refill(c1_run, precision, refill):
+    """The default whole-wave refill (a wave takes new jobs only when all its lanes are done, as the
+    reference's per-thread job loop runs on a wave) and a refill whenever `refill` lanes are idle
+    march every ray with the same arithmetic as the per-lane refill: bit-identical results and
+    statistics.  With the sample queue (whose lanes are refilled at once) the samples are computed
+    by a differently compiled split of the same arithmetic: the same geodesics and statistics, the
+    Stokes sums equal to a few float32 ulps."""
+    snap = capi.Snapshot(c1_run, capi.TABLE_FP32)
+    rq = c1_run.request(img_width=40, img_height=40, inclination=30.0)
+    wave, i1 = snap.render(rq, precision, plain=True)
+    lane, i2 = snap.render(rq, precision, tune={"queue": 0, "refill": refill})
+    queued, i3 = snap.render(rq, precision, tune={"queue": 1, "refill": 64})
+    snap.close()
+    assert np.array_equal(lane, wave)
+    assert (i1["steps"], i1["nplasma"], i1["bad"]) == (i2["steps"], i2["nplasma"], i2["bad"])
+    assert (i1["steps"], i1["nplasma"], i1["bad"]) == (i3["steps"], i3["nplasma"], i3["bad"])
+    scale = np.abs(wave[..., 0]).max()  # Stokes I, Q, U, V against the image's brightest I
+    assert np.abs(queued[..., :4] - wave[..., :4]).max() <= 1e-5 * scale
+    for c in (4, 5):  # optical depths, against each one's largest value
+        assert np.abs(queued[..., c] - wave[..., c]).max() <= 1e-5 * np.abs(wave[..., c]).max()
+
+
 def test_errors_are_reported(c1_run):
     snap = capi.Snapshot(c1_run, capi.TABLE_FP32)
     with pytest.raises(capi.PhxError) as e:
