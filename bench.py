@@ -319,11 +319,16 @@ def main() -> int:
         log(f"timed {args.steps} renders ({args.precision}): {wall:.3f} s, {value:.4g} rays*freq/s")
         # the same image by the other paths.  "plain_fp32" is the reference's FP32 method as the
         # reference's kernel runs it: each lane marches its pixel with the samples processed in place,
-        # and a wave takes new pixels when all its lanes are done (kernel.cu:80-96 on a wave).  The
+        # a wave takes new pixels when all its lanes are done (kernel.cu:80-96 on a wave), and every
+        # ray marches on to r_out.  "_escape" ends a ray once it moves outward beyond rt_outer (the
+        # same results, fewer steps; the mixed path's default), "_full" marches it to r_out.  The
         # "_queue" variants refill each lane at once and process the samples through the per-wave
         # LDS queue.
-        others = [("plain_fp32", capi.FP32, {"queue": 0, "refill": 64}), ("fp32_queue", capi.FP32, {"queue": 1}),
-                  ("fp64", capi.FP64, None), ("mixed", capi.MIXED, None), ("mixed_queue", capi.MIXED, {"queue": 1})]
+        others = [("plain_fp32", capi.FP32, {"queue": 0, "refill": 64, "escape": 1}),
+                  ("fp32_escape", capi.FP32, {"queue": 0, "refill": 64, "escape": 2}),
+                  ("fp32_queue", capi.FP32, {"queue": 1, "escape": 2}),
+                  ("fp64", capi.FP64, None), ("mixed", capi.MIXED, None), ("mixed_full", capi.MIXED, {"escape": 1}),
+                  ("mixed_queue", capi.MIXED, {"queue": 1})]
         for name, p2, tune in others:
             if name == args.precision:
                 continue
@@ -339,10 +344,13 @@ def main() -> int:
             w2 = time.perf_counter() - t0
             extra[f"{name}_rays_freq_per_s"] = steps * rays / w2
             extra[f"{name}_kernel_ms"] = float(np.mean(k2)) * 1e3
-            extra[f"{name}_roofline_frac"] = roofline(i2, nf, name.split("_")[-2] if name.endswith("_queue") else
-                                                      name.split("_")[-1], float(np.mean(k2)))["frac"]
+            extra[f"{name}_steps_per_ray"] = i2["steps"] / (W * H)
+            extra[f"{name}_roofline_frac"] = roofline(i2, nf, next(p for p in ("fp32", "fp64", "mixed") if p in name),
+                                                      float(np.mean(k2)))["frac"]
         if "plain_fp32_rays_freq_per_s" in extra:
             extra["value_vs_plain_fp32"] = value / extra["plain_fp32_rays_freq_per_s"]
+        if "fp32_escape_rays_freq_per_s" in extra:
+            extra["value_vs_fp32_escape"] = value / extra["fp32_escape_rays_freq_per_s"]
         if not args.quick:
             try:
                 extra["batch_c4"] = bench_batch(run, snap, capi, prec, sync)

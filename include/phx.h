@@ -210,6 +210,9 @@ typedef struct phx_tune {
   int32_t fp64_wave;     /* PHX_MIXED: 1 = a lane's FP64 step takes its whole wave to FP64 */
   int32_t refill;        /* samples in place: idle lanes a wave gathers before they take new jobs;
                             64 (or 0, the default) = the whole wave, 1 = each lane at once; the queue uses 1 */
+  int32_t escape;        /* 0 default by precision (PHX_MIXED 2, PHX_FP32 / PHX_FP64 1); 1 every ray marches to
+                            r_out / cut_in / max_steps as the reference does; 2 a ray moving outward beyond
+                            rt_outer ends there (it cannot emit again: the same results, fewer steps) */
 } phx_tune;
 /* Autotuner: times every configuration on every sample_stride-th wave of the batch's jobs,
  * keeps the fastest for the snapshot's batch renders, returns it and its sampled rays*freq/s.

@@ -76,7 +76,7 @@ class Config(ctypes.Structure):
 
 class Tune(ctypes.Structure):
     _fields_ = [("queue", c_int), ("blocks_per_cu", c_int), ("order", c_int), ("fp64_wave", c_int),
-                ("refill", c_int)]
+                ("refill", c_int), ("escape", c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -277,7 +277,7 @@ class Snapshot:
         self.run = run
         self.h = ctypes.c_void_p()
         check(lib().phx_snapshot_open(ctypes.byref(run.model), run.config.dump_file, tables, ctypes.byref(self.h)))
-        self._tune = {"queue": -1, "blocks_per_cu": 0, "order": 0, "fp64_wave": 1, "refill": 0}
+        self._tune = {"queue": -1, "blocks_per_cu": 0, "order": 0, "fp64_wave": 1, "refill": 0, "escape": 0}
 
     def close(self):
         if self.h:
@@ -297,11 +297,12 @@ class Snapshot:
     def render(self, request: Model | None = None, precision: int = FP32, jobs=None, to_host: bool = True,
                plain: bool = False, tune: dict | None = None):
         """One render on the resident snapshot.  plain: the reference kernel's way, each lane marching
-        its pixel with the samples processed in place and the wave taking new pixels when all its
-        lanes are done (queue 0, refill 64).  tune: launch settings for this render only."""
+        its pixel with the samples processed in place, the wave taking new pixels when all its
+        lanes are done and every ray marched to r_out (queue 0, refill 64, escape 1).  tune: launch
+        settings for this render only."""
         if plain or tune:
             saved = dict(self._tune)
-            self.set_tune(**dict(saved, **(dict(queue=0, refill=64) if plain else {}), **(tune or {})))
+            self.set_tune(**dict(saved, **(dict(queue=0, refill=64, escape=1) if plain else {}), **(tune or {})))
             try:
                 return self.render(request, precision, jobs, to_host)
             finally:
@@ -351,8 +352,9 @@ class Snapshot:
         self._tune = t.as_dict()
         return t.as_dict(), rate.value
 
-    def set_tune(self, queue: int = -1, blocks_per_cu: int = 0, order: int = 0, fp64_wave: int = 1, refill: int = 0):
-        t = Tune(queue, blocks_per_cu, order, fp64_wave, refill)
+    def set_tune(self, queue: int = -1, blocks_per_cu: int = 0, order: int = 0, fp64_wave: int = 1, refill: int = 0,
+                 escape: int = 0):
+        t = Tune(queue, blocks_per_cu, order, fp64_wave, refill, escape)
         check(lib().phx_snapshot_set_tune(self.h, ctypes.byref(t)))
         self._tune = t.as_dict()
 

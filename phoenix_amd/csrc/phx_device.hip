@@ -71,6 +71,7 @@ struct MarchArgs {
   double a64, hs64, lnr_fp64, x2_pole;  // MIX: FP64 steps where ln r < lnr_fp64 or x2 within x2_pole of a pole
   int fp64_wave;                        // MIX: a lane's FP64 step takes its whole wave to FP64
   int refill;                           // idle lanes a wave gathers before they take new jobs (1: at once)
+  int escape;                           // 1: a ray moving outward beyond rt_outer ends (it cannot emit again)
   int max_steps, max_order;
   const TaskDev<T>* tasks;
   const int2* jobs;
@@ -563,7 +564,16 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
     T cx[4], ck[4], cf[4];
     st.view(cx, ck, cf);
     const T r = exp_r(cx[1]);
-    const bool alive = r < A.r_out && r > A.cut_in && steps < A.max_steps && !stop;
+    // ln r of the pending sample (the state before the last step; MIX keeps it in FP64 only)
+    const T lrs = M == MIX ? T(pl[1]) : px[1];
+    const T rs = exp_r(lrs);  // radius of the pending sample
+    // escape: a ray moving outward at r >= rt_outer, whose pending sample lies there too, can
+    // never emit again (a null geodesic in Kerr has at most one radial turning point outside the
+    // photon orbits, so it does not return), and the rest of its march changes none of its
+    // outputs: it ends here instead of at r_out (the reference marches on to r_out,
+    // pixel.hpp:163, and only its step count differs)
+    const bool escaped = A.escape && steps >= 2 && r >= A.P.rt_outer && rs >= A.P.rt_outer && cx[1] > lrs;
+    const bool alive = r < A.r_out && r > A.cut_in && steps < A.max_steps && !stop && !escaped;
     if constexpr (M == MIX) {
       st.begin(A, alive, ck);
       // the pending sample's FP32 position and polarization constant follow from its FP64
@@ -574,7 +584,6 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
         pf[q] = cf[q];
       }
     }
-    const T rs = exp_r(px[1]);  // radius of the pending sample
     // the pending sample is the state before the last step, i.e. sample number steps - 1; the
     // camera sample (number 0) is never processed (pixel.hpp:186).  A sample at r >= rt_outer
     // cannot emit (pixel.hpp:80) and changes no ray state, so nothing of it is computed.
@@ -1030,6 +1039,8 @@ struct Tune {
   int order = 0;          // batch job order: 0 request after request, 1 requests interleaved per wave
   int fp64_wave = 1;      // mixed march: 1 = a lane's FP64 step takes its whole wave to FP64
   int refill = 64;        // idle lanes a wave gathers before they take new jobs (64: the whole wave)
+  int escape = 0;         // 0: by precision (MIXED ends escaping rays at rt_outer, FP32 / FP64 march them
+                          // to r_out as the reference does), 1: march to r_out, 2: end at rt_outer
 };
 // grow-only device buffer
 struct DevBuf {
@@ -1336,6 +1347,8 @@ static Tune tune_from_env(Tune t) {
   if (const char* e = std::getenv("PHX_BLOCKS_PER_CU"))
     if (std::atoi(e) > 0) t.blocks_per_cu = std::atoi(e);
   if (const char* e = std::getenv("PHX_FP64_WAVE")) t.fp64_wave = std::atoi(e) != 0;
+  if (const char* e = std::getenv("PHX_ESCAPE"))
+    t.escape = !std::strcmp(e, "full") ? 1 : !std::strcmp(e, "exit") ? 2 : t.escape;
   if (const char* e = std::getenv("PHX_REFILL"))
     t.refill = !std::strcmp(e, "wave") ? 64 : !std::strcmp(e, "lane") ? 1 : std::atoi(e);
   return t;
@@ -1376,6 +1389,7 @@ static void launch_march(phx_snapshot* s, const LaunchInfo& li, int j0, int j1, 
   A.x2_pole = pole_x2_limit(m.hslope);                                          // |cos theta| > 0.99
   A.fp64_wave = tn.fp64_wave;
   A.refill = Q ? 1 : std::min(64, std::max(1, tn.refill));  // the queue refills each lane at once
+  A.escape = tn.escape == 2 || (tn.escape == 0 && M == MIX);
   A.max_steps = rq->max_steps;
   A.max_order = rq->max_order;
   A.tasks = (const TaskDev<T>*)s->tasks.p;
@@ -1811,6 +1825,7 @@ static void autotune(phx_snapshot* s, const phx_model* const* rqs, int n, int pr
     best->order = tb.order;
     best->fp64_wave = tb.fp64_wave;
     best->refill = tb.refill;
+    best->escape = tb.escape;
   }
   if (best_rate) *best_rate = rate_best;
 }
@@ -2119,13 +2134,14 @@ int phx_snapshot_set_tune(phx_snapshot* s, const phx_tune* t) {
   return guarded([&] {
     if (!s || !t) throw Error(PHX_E_ARG, "null snapshot or tune");
     if (t->queue < -1 || t->queue > 1 || t->blocks_per_cu < 0 || t->order < 0 || t->order > 1 || t->fp64_wave < 0 ||
-        t->fp64_wave > 1 || t->refill < 0 || t->refill > 64)
+        t->fp64_wave > 1 || t->refill < 0 || t->refill > 64 || t->escape < 0 || t->escape > 2)
       throw Error(PHX_E_ARG, "tune value out of range");
     s->tune.queue = t->queue;
     s->tune.blocks_per_cu = t->blocks_per_cu;
     s->tune.order = t->order;
     s->tune.fp64_wave = t->fp64_wave;
     s->tune.refill = t->refill > 0 ? t->refill : 64;
+    s->tune.escape = t->escape;
   });
 }
 

@@ -147,8 +147,8 @@ def test_wave_synchronous_refill_matches_lane_refill(c1_run, precision, refill):
     snap = capi.Snapshot(c1_run, capi.TABLE_FP32)
     rq = c1_run.request(img_width=40, img_height=40, inclination=30.0)
     wave, i1 = snap.render(rq, precision, plain=True)
-    lane, i2 = snap.render(rq, precision, tune={"queue": 0, "refill": refill})
-    queued, i3 = snap.render(rq, precision, tune={"queue": 1, "refill": 64})
+    lane, i2 = snap.render(rq, precision, tune={"queue": 0, "refill": refill, "escape": 1})
+    queued, i3 = snap.render(rq, precision, tune={"queue": 1, "refill": 64, "escape": 1})
     snap.close()
     assert np.array_equal(lane, wave)
     assert (i1["steps"], i1["nplasma"], i1["bad"]) == (i2["steps"], i2["nplasma"], i2["bad"])
@@ -157,6 +157,33 @@ def test_wave_synchronous_refill_matches_lane_refill(c1_run, precision, refill):
     assert np.abs(queued[..., :4] - wave[..., :4]).max() <= 1e-5 * scale
     for c in (4, 5):  # optical depths, against each one's largest value
         assert np.abs(queued[..., c] - wave[..., c]).max() <= 1e-5 * np.abs(wave[..., c]).max()
+
+
+@pytest.mark.parametrize("queue", [0, 1])
+@pytest.mark.parametrize("precision", [capi.FP32, capi.FP64, capi.MIXED])
+def test_escaping_rays_end_at_rt_outer_with_the_same_results(c1_run, precision, queue):
+    """A ray moving outward beyond rt_outer cannot emit again: ending it there (escape 2) instead of
+    marching it on to r_out as the reference does (escape 1) gives bit-identical results with the same
+    plasma-segment counts and fewer steps.  The mixed march is compared with per-lane FP64 promotion:
+    with whole-wave promotion an escaped lane no longer votes, so its wave-mates may take an FP32
+    step where they took an FP64 one (checked within the parity gates)."""
+    snap = capi.Snapshot(c1_run, capi.TABLE_FP32 | capi.TABLE_FP64)
+    for incl in (10.0, 163.0):
+        rq = c1_run.request(img_width=32, img_height=32, inclination=incl)
+        for fw in ((0, 1) if precision == capi.MIXED else (1,)):
+            t = {"queue": queue, "fp64_wave": fw}
+            full, i1 = snap.render(rq, precision, tune=dict(t, escape=1))
+            cut, i2 = snap.render(rq, precision, tune=dict(t, escape=2))
+            print(f"\nfp{precision} queue {queue} i={incl} fp64_wave {fw}: steps {i1['steps']} -> {i2['steps']}, "
+                  f"kernel {i1['kernel_s'] * 1e3:.2f} -> {i2['kernel_s'] * 1e3:.2f} ms")
+            assert i2["steps"] < 0.9 * i1["steps"]
+            if fw == 1 and precision == capi.MIXED:
+                figs = parity.compare(cut, full)
+                assert parity.passes(figs), parity.summary(figs)
+                continue
+            assert np.array_equal(cut, full)
+            assert (i1["nplasma"], i1["bad"]) == (i2["nplasma"], i2["bad"])
+    snap.close()
 
 
 def test_errors_are_reported(c1_run):
