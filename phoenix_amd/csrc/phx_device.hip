@@ -511,6 +511,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
   // queue bookkeeping: qcount and nleft are the same in every active lane of the wave; nmine and
   // ncm count this lane's queued samples and its candidates not yet folded in
   int qcount = 0, nleft = 0, nmine = 0, ncm = 0;
+  LeafCache<T> lcache;  // the lane's last map cell and its leaf (valid across rays: one snapshot per launch)
 
   for (;;) {
     // refill: a lane whose ray ended waits until at least A.refill lanes of its wave are idle (or
@@ -598,12 +599,13 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
       const bool need_dxc = r < A.P.rt_outer;
       const bool need_leaf = emit && !off_domain(A.G, wx);
       if (need_dxc || need_leaf) {
-        const int Lf = find_leaf(A.G, wx);
+        bool cached;
+        const int Lf = find_leaf_cached(A.G, wx, lcache, &cached);
         if (need_leaf) leaf_next = Lf;
         if (need_dxc && Lf >= 0) {
           have_dxc = true;
 #pragma unroll
-          for (int d = 0; d < 3; d++) dxc[d] = A.G.blk[12 * Lf + 6 + d];
+          for (int d = 0; d < 3; d++) dxc[d] = cached ? lcache.dxc[d] : A.G.blk[12 * Lf + 6 + d];
         }
       }
     }
@@ -1122,7 +1124,7 @@ static void snapshot_free(phx_snapshot* s) {
 }
 
 // block records: lower corner, upper corner (lo + nx * dxc in the table's precision, as
-// src/core/model/sampler.hpp:93-97 forms it), cell size, padding
+// src/core/model/sampler.hpp:93-97 forms it), cell size, reciprocal cell size
 template <class T>
 static std::vector<T> block_records(const phx_model* m) {
 #pragma clang fp contract(off)
@@ -1134,6 +1136,7 @@ static std::vector<T> block_records(const phx_model* m) {
       rec[12 * q + d] = lo;
       rec[12 * q + 3 + d] = lo + span;
       rec[12 * q + 6 + d] = dx;
+      rec[12 * q + 9 + d] = T(1) / dx;  // the mixed march locates cells with it (FP64 table)
     }
   return rec;
 }

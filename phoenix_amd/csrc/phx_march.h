@@ -509,17 +509,18 @@ PHX_DEV T step_len(T stepsize, T x2, const T k[4], bool have_dxc, const T dxc[3]
 // (src/core/model/sampler.hpp:55-82).
 template <class T>
 struct Grid {
-  const T* blk;    // [nleafs][12]
+  const T* blk;    // [nleafs][12]: lo[3], hi[3], dxc[3], 1/dxc[3]
   const T* prim;   // [nleafs * cells][8]
   const int* map;  // [fd0][fd1][fd2]
   int nleafs, cells, nx0, nx1, nx2, fd0, fd1, fd2;
   T w0, w1, w2, up;  // base-level block widths, 2^(nlevels-1)
   T lo1, lo2, lo3, hi1, hi2, hi3;
 };
-// fmod of the wrap: fmod_exact in FP32 (the same values, a fraction of the library's cost); the
-// library fmod in FP64, where it measured faster
+// fmod of the wrap: fmod_exact in FP32 (the same values, a fraction of the library's cost); in
+// FP64 x itself when |x| < y (fmod's exact value there; x2 and x3 almost always lie in range),
+// else the library fmod
 PHX_DEV float wrap_mod(float x, float y) { return fmod_exact(x, y, 1.f / y); }
-PHX_DEV double wrap_mod(double x, double y) { return fmod(x, y); }
+PHX_DEV double wrap_mod(double x, double y) { return fabs(x) < y ? x : fmod(x, y); }
 template <class T>
 PHX_DEV void wrap(T x[4]) {  // sampler.hpp:29-38
   const T TP = T(2. * PI_D), PI = T(PI_D);
@@ -561,6 +562,58 @@ PHX_DEV int find_leaf(const Grid<T>& G, const T x[4]) {
     }
   }
   if (cand >= 0 && in_leaf(G, cand, x, small)) return cand;
+  const T eps = T(1e-6);
+  if (cand < 0 && (x[1] < G.lo1 - eps || x[1] > G.hi1 + eps || x[2] < G.lo2 - eps || x[2] > G.hi2 + eps ||
+                   x[3] < G.lo3 - eps || x[3] > G.hi3 + eps))
+    return -1;
+  for (int q = 0; q < G.nleafs; q++)  // the reference's linear fallback scan
+    if (in_leaf(G, q, x, small)) return q;
+  return -1;
+}
+// A lane's record of its last lookup: the finest-level map cell, the leaf the map names there, and
+// that leaf's box and cell size.  A ray stays in one map cell for dozens of steps, and while it does
+// find_leaf's map candidate is the cached leaf, so the lookup needs no memory access.
+template <class T>
+struct LeafCache {
+  int cell = -1, leaf = -1;
+  T box[6], dxc[3];
+};
+// find_leaf with the lane's cache: the same candidate, validation and fallbacks as find_leaf, so
+// the same leaf.  *cached tells whether the result is the cached leaf (its cell size is in c.dxc).
+template <class T>
+PHX_DEV int find_leaf_cached(const Grid<T>& G, const T x[4], LeafCache<T>& c, bool* cached) {
+  const T small = x[2] > T(PI_D / 2.) ? T(-1e-9) : T(1e-9);
+  int cell = -1;
+  {
+    const T fi = (x[1] + small - G.lo1) / G.w0 * G.up;
+    const T fj = (x[2] + small - G.lo2) / G.w1 * G.up;
+    const T fk = (x[3] + small - G.lo3) / G.w2 * G.up;
+    if (!(fi < T(0) || fj < T(0) || fk < T(0))) {
+      const int i = (int)fi, j = (int)fj, k = (int)fk;
+      if (i < G.fd0 && j < G.fd1 && k < G.fd2) cell = (i * G.fd1 + j) * G.fd2 + k;
+    }
+  }
+  int cand = -1;
+  if (cell >= 0) {
+    if (cell != c.cell) {
+      c.cell = cell;
+      c.leaf = G.map[cell];
+      if (c.leaf >= 0) {
+        const T* b = G.blk + 12 * c.leaf;
+#pragma unroll
+        for (int q = 0; q < 6; q++) c.box[q] = b[q];
+#pragma unroll
+        for (int q = 0; q < 3; q++) c.dxc[q] = b[6 + q];
+      }
+    }
+    cand = c.leaf;
+  }
+  *cached = false;
+  if (cand >= 0 && !(x[1] + small < c.box[0] || x[1] + small >= c.box[3] || x[2] + small < c.box[1] ||
+                     x[2] + small >= c.box[4] || x[3] + small < c.box[2] || x[3] + small >= c.box[5])) {
+    *cached = true;
+    return cand;
+  }
   const T eps = T(1e-6);
   if (cand < 0 && (x[1] < G.lo1 - eps || x[1] > G.hi1 + eps || x[2] < G.lo2 - eps || x[2] > G.hi2 + eps ||
                    x[3] < G.lo3 - eps || x[3] > G.hi3 + eps))
@@ -621,11 +674,18 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Grid<L>& GL, const Plasma<T>& P
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     const L lb = b[d], dx = b[6 + d];
-    const L u = (x[1 + d] - lb) / dx;
+    // the mixed march (FP64 location, FP32 fluid) multiplies by the reciprocal cell width: a
+    // last-ulp change of u only moves a sample across a cell face, where the block-local trilinear
+    // weights are continuous; FP32 and FP64 divide as the reference does
+    constexpr bool MIXED = !std::is_same<T, L>::value;
+    const L u = MIXED ? (x[1 + d] - lb) * b[9 + d] : (x[1 + d] - lb) / dx;
     c[d] = (int)u;
     if (c[d] >= nxs[d]) c[d] = nxs[d] - 1;
     const int ix = F32 ? (int)floor(u) : (int)(u + L(1000)) - 1000;
-    del[d] = (ix < 0) ? T(0) : (ix > nxs[d] - 2) ? T(1) : T((x[1 + d] - (L(ix) * dx + lb)) / dx);
+    del[d] = (ix < 0)             ? T(0)
+             : (ix > nxs[d] - 2)  ? T(1)
+             : MIXED              ? T(u - L(ix))
+                                  : T((x[1 + d] - (L(ix) * dx + lb)) / dx);
   }
   const int ip = c[0] + 1 < G.nx0 ? c[0] + 1 : c[0];
   const int jp = c[1] + 1 < G.nx1 ? c[1] + 1 : c[1];
