@@ -328,7 +328,7 @@ def main() -> int:
                   ("fp32_escape", capi.FP32, {"queue": 0, "refill": 64, "escape": 2}),
                   ("fp32_queue", capi.FP32, {"queue": 1, "escape": 2}),
                   ("fp64", capi.FP64, None), ("mixed", capi.MIXED, None), ("mixed_full", capi.MIXED, {"escape": 1}),
-                  ("mixed_queue", capi.MIXED, {"queue": 1})]
+                  ("mixed_queue", capi.MIXED, {"queue": 1}), ("mixed_queue_lane", capi.MIXED, {"queue": 1, "refill": 1})]
         for name, p2, tune in others:
             if name == args.precision:
                 continue

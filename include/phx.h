@@ -208,8 +208,8 @@ typedef struct phx_tune {
   int32_t blocks_per_cu; /* 0 = occupancy limit */
   int32_t order;         /* 0 request after request, 1 requests interleaved one wave (64 jobs) at a time */
   int32_t fp64_wave;     /* PHX_MIXED: 1 = a lane's FP64 step takes its whole wave to FP64 */
-  int32_t refill;        /* samples in place: idle lanes a wave gathers before they take new jobs;
-                            64 (or 0, the default) = the whole wave, 1 = each lane at once; the queue uses 1 */
+  int32_t refill;        /* idle lanes a wave gathers before they take new jobs;
+                            64 (or 0, the default) = the whole wave, 1 = each lane at once */
   int32_t escape;        /* 0 default by precision (PHX_MIXED 2, PHX_FP32 / PHX_FP64 1); 1 every ray marches to
                             r_out / cut_in / max_steps as the reference does; 2 a ray moving outward beyond
                             rt_outer ends there (it cannot emit again: the same results, fewer steps) */

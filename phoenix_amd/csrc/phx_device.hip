@@ -255,6 +255,17 @@ using StateOf = typename std::conditional<M == MIX, StHam, StRef<typename ModeOf
 template <int M>
 using RecOf = typename std::conditional<M == M32, float, double>::type;  // start record scalar
 
+// re-solve k^t for k.k = 0 when |k.k| > 1e-6 and r > 2 (geodesic.hpp:297-306, pixel.hpp:69-71)
+template <class T>
+PHX_DEV void null_resolve(const Met<T>& g, T k[4], T r) {
+  if (fabs(dot(g, k, k)) > T(1e-6) && r > T(2)) {
+    const T aa = g.d00;
+    const T bc = -(g.d01 * k[1]) / aa - (g.d03 * k[3]) / aa;
+    const T gz = -(g.d11 * k[1] * k[1]) / aa - (T(2) * g.d13 * k[1] * k[3]) / aa - (g.d22 * k[2] * k[2]) / aa -
+                 (g.d33 * k[3] * k[3]) / aa;
+    k[0] = bc + sqrt(bc * bc + gz);
+  }
+}
 // ---- one plasma sample, split into its frequency-independent part and the per-frequency
 // segment operator (src/core/render/pixel.hpp:52-123)
 template <class T>
@@ -278,18 +289,14 @@ PHX_DEV int sample_fluid_stage(const MarchArgs<M>& A, const TaskDev<typename Mod
                                typename ModeOf<M>::T dl, int leaf, int task, typename ModeOf<M>::T* c) {
   using T = typename ModeOf<M>::T;
   T k[4] = {pk[0], pk[1], pk[2], pk[3]};
-  const T rs = exp(px[1]);
-  const Met<T> gs = metric(A.P.a, A.P.hs, px[1], px[2]);
-  if (fabs(dot(gs, k, k)) > T(1e-6) && rs > T(2)) {  // re-solve k^t for k.k = 0 (geodesic.hpp:297-306)
-    const T aa = gs.d00;
-    const T bc = -(gs.d01 * k[1]) / aa - (gs.d03 * k[3]) / aa;
-    const T gz = -(gs.d11 * k[1] * k[1]) / aa - (T(2) * gs.d13 * k[1] * k[3]) / aa - (gs.d22 * k[2] * k[2]) / aa -
-                 (gs.d33 * k[3] * k[3]) / aa;
-    k[0] = bc + sqrt(bc * bc + gz);
-  }
+  if constexpr (M != MIX) null_resolve(metric(A.P.a, A.P.hs, px[1], px[2]), k, exp(px[1]));
   Fluid<T> fl;
   if (!sample_fluid(A.G, A.GL, A.P, tk.Ne, tk.Bu, tk.rhigh, pl, leaf, fl)) return 0;
   if (fl.b[0] == T(0) && fl.b[1] == T(0) && fl.b[2] == T(0) && fl.b[3] == T(0)) return 2;
+  // the mixed march's wavevector k^mu = g^{mu nu} p_nu is null by construction up to FP32 rounding;
+  // it is checked (and re-solved) with the metric at the sampler's position, the one every later
+  // contraction with k uses, and only for samples that reach this point
+  if constexpr (M == MIX) null_resolve(fl.g, k, fl.g.r);
   c[0] = fl.x[1];
   c[1] = fl.x[2];
 #pragma unroll
@@ -355,18 +362,14 @@ PHX_DEV int sample_prepare(const MarchArgs<M>& A, const TaskDev<typename ModeOf<
                            typename ModeOf<M>::T dl, int leaf, Prep<typename ModeOf<M>::T>& sp) {
   using T = typename ModeOf<M>::T;
   T k[4] = {pk[0], pk[1], pk[2], pk[3]};
-  const T rs = exp(px[1]);
-  const Met<T> gs = metric(A.P.a, A.P.hs, px[1], px[2]);
-  if (fabs(dot(gs, k, k)) > T(1e-6) && rs > T(2)) {  // re-solve k^t for k.k = 0 (geodesic.hpp:297-306)
-    const T aa = gs.d00;
-    const T bc = -(gs.d01 * k[1]) / aa - (gs.d03 * k[3]) / aa;
-    const T gz = -(gs.d11 * k[1] * k[1]) / aa - (T(2) * gs.d13 * k[1] * k[3]) / aa - (gs.d22 * k[2] * k[2]) / aa -
-                 (gs.d33 * k[3] * k[3]) / aa;
-    k[0] = bc + sqrt(bc * bc + gz);
-  }
+  if constexpr (M != MIX) null_resolve(metric(A.P.a, A.P.hs, px[1], px[2]), k, exp(px[1]));
   Fluid<T> fl;
   if (!sample_fluid(A.G, A.GL, A.P, tk.Ne, tk.Bu, tk.rhigh, pl, leaf, fl)) return 0;
   if (fl.b[0] == T(0) && fl.b[1] == T(0) && fl.b[2] == T(0) && fl.b[3] == T(0)) return 2;
+  // the mixed march's wavevector k^mu = g^{mu nu} p_nu is null by construction up to FP32 rounding;
+  // it is checked (and re-solved) with the metric at the sampler's position, the one every later
+  // contraction with k uses, and only for samples that reach this point
+  if constexpr (M == MIX) null_resolve(fl.g, k, fl.g.r);
   const T pa = pitch_angle(fl.g, k, fl.b, fl.U);
   if (pa == T(0) || pa == T(PI_D)) return 0;
   T perp[4], para[4];
@@ -511,7 +514,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
   // queue bookkeeping: qcount and nleft are the same in every active lane of the wave; nmine and
   // ncm count this lane's queued samples and its candidates not yet folded in
   int qcount = 0, nleft = 0, nmine = 0, ncm = 0;
-  LeafCache<T> lcache;  // the lane's last map cell and its leaf (valid across rays: one snapshot per launch)
+  LeafCache<T> lcache;  // the lane's last leaf and its box (valid across rays: one snapshot per launch)
 
   for (;;) {
     // refill: a lane whose ray ended waits until at least A.refill lanes of its wave are idle (or
@@ -1391,7 +1394,7 @@ static void launch_march(phx_snapshot* s, const LaunchInfo& li, int j0, int j1, 
   A.lnr_fp64 = std::log(2. * (1. + std::sqrt(std::max(0., 1. - m.a * m.a))));  // r < 2 r_H
   A.x2_pole = pole_x2_limit(m.hslope);                                          // |cos theta| > 0.99
   A.fp64_wave = tn.fp64_wave;
-  A.refill = Q ? 1 : std::min(64, std::max(1, tn.refill));  // the queue refills each lane at once
+  A.refill = std::min(64, std::max(1, tn.refill));
   A.escape = tn.escape == 2 || (tn.escape == 0 && M == MIX);
   A.max_steps = rq->max_steps;
   A.max_order = rq->max_order;
@@ -1803,13 +1806,13 @@ static void autotune(phx_snapshot* s, const phx_model* const* rqs, int n, int pr
   double rate_best = -1.;
   Tune tb;
   for (int q = 0; q <= 1; q++)
-    for (int rf = 0; rf <= (q ? 0 : 1); rf++)
+    for (int rf = 0; rf <= 1; rf++)
     for (int bpc = 1; bpc <= max_bpc; bpc++)
       for (int order = 0; order <= (n > 1 ? 1 : 0); order++)
         for (int fw = 0; fw <= (precision == PHX_MIXED ? 1 : 0); fw++) {
         Tune t;
         t.queue = q;
-        t.refill = (q || rf) ? 1 : 64;
+        t.refill = rf ? 1 : 64;
         t.blocks_per_cu = bpc;
         t.order = order;
         t.fp64_wave = fw;

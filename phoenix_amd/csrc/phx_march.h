@@ -570,49 +570,47 @@ PHX_DEV int find_leaf(const Grid<T>& G, const T x[4]) {
     if (in_leaf(G, q, x, small)) return q;
   return -1;
 }
-// A lane's record of its last lookup: the finest-level map cell, the leaf the map names there, and
-// that leaf's box and cell size.  A ray stays in one map cell for dozens of steps, and while it does
-// find_leaf's map candidate is the cached leaf, so the lookup needs no memory access.
+// A lane's record of its last lookup: the leaf and its box and cell size.  A ray stays in one leaf
+// for dozens of steps, and while its position lies in that leaf's (half-open, nudged) box the
+// lookup needs no memory access.  Leaves tile the domain, so the leaf whose box holds a point is
+// the one find_leaf returns; the two differ only where rounding makes two boxes overlap by an ulp
+// and the map names the other leaf.
 template <class T>
 struct LeafCache {
-  int cell = -1, leaf = -1;
+  int leaf = -1;
   T box[6], dxc[3];
 };
-// find_leaf with the lane's cache: the same candidate, validation and fallbacks as find_leaf, so
-// the same leaf.  *cached tells whether the result is the cached leaf (its cell size is in c.dxc).
 template <class T>
 PHX_DEV int find_leaf_cached(const Grid<T>& G, const T x[4], LeafCache<T>& c, bool* cached) {
   const T small = x[2] > T(PI_D / 2.) ? T(-1e-9) : T(1e-9);
-  int cell = -1;
+  *cached = c.leaf >= 0 && !(x[1] + small < c.box[0] || x[1] + small >= c.box[3] || x[2] + small < c.box[1] ||
+                             x[2] + small >= c.box[4] || x[3] + small < c.box[2] || x[3] + small >= c.box[5]);
+  if (*cached) return c.leaf;
+  int cand = -1;
   {
     const T fi = (x[1] + small - G.lo1) / G.w0 * G.up;
     const T fj = (x[2] + small - G.lo2) / G.w1 * G.up;
     const T fk = (x[3] + small - G.lo3) / G.w2 * G.up;
     if (!(fi < T(0) || fj < T(0) || fk < T(0))) {
       const int i = (int)fi, j = (int)fj, k = (int)fk;
-      if (i < G.fd0 && j < G.fd1 && k < G.fd2) cell = (i * G.fd1 + j) * G.fd2 + k;
+      if (i < G.fd0 && j < G.fd1 && k < G.fd2) cand = G.map[(i * G.fd1 + j) * G.fd2 + k];
     }
   }
-  int cand = -1;
-  if (cell >= 0) {
-    if (cell != c.cell) {
-      c.cell = cell;
-      c.leaf = G.map[cell];
-      if (c.leaf >= 0) {
-        const T* b = G.blk + 12 * c.leaf;
+  if (cand >= 0) {
+    const T* b = G.blk + 12 * cand;
+    T box[6];
 #pragma unroll
-        for (int q = 0; q < 6; q++) c.box[q] = b[q];
+    for (int q = 0; q < 6; q++) box[q] = b[q];
+    if (!(x[1] + small < box[0] || x[1] + small >= box[3] || x[2] + small < box[1] || x[2] + small >= box[4] ||
+          x[3] + small < box[2] || x[3] + small >= box[5])) {
+      c.leaf = cand;
 #pragma unroll
-        for (int q = 0; q < 3; q++) c.dxc[q] = b[6 + q];
-      }
+      for (int q = 0; q < 6; q++) c.box[q] = box[q];
+#pragma unroll
+      for (int q = 0; q < 3; q++) c.dxc[q] = b[6 + q];
+      *cached = true;
+      return cand;
     }
-    cand = c.leaf;
-  }
-  *cached = false;
-  if (cand >= 0 && !(x[1] + small < c.box[0] || x[1] + small >= c.box[3] || x[2] + small < c.box[1] ||
-                     x[2] + small >= c.box[4] || x[3] + small < c.box[2] || x[3] + small >= c.box[5])) {
-    *cached = true;
-    return cand;
   }
   const T eps = T(1e-6);
   if (cand < 0 && (x[1] < G.lo1 - eps || x[1] > G.hi1 + eps || x[2] < G.lo2 - eps || x[2] > G.hi2 + eps ||
