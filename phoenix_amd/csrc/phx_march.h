@@ -710,6 +710,30 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Grid<L>& GL, const Plasma<T>& P
   }
   // derive_fluid
   const T tiny = T(1.e-6);
+  if (qv[5] == T(0) && qv[6] == T(0) && qv[7] == T(0) && isfinite(qv[2]) && isfinite(qv[3]) && isfinite(qv[4])) {
+    // a field-free sample (B^i = 0 at all 8 corners): b^mu = 0 exactly, whatever the velocity and the
+    // metric (alpha and the Lorentz factor are finite: g^tt < 0 and the Kerr-Schild spatial metric is
+    // positive definite), so |B|, beta, Theta_e and the cuts follow from rho and u alone, with the
+    // same expressions as below; U^mu and the metric are not needed by any caller of a sample with b = 0
+    const T rho = qv[0], uu = qv[1];
+#pragma unroll
+    for (int i = 0; i < 4; i++) fl.b[i] = T(0);
+    fl.ne = rho * Ne + tiny;
+    const T Bsq = tiny;
+    fl.B = sqrt(Bsq) * Bu;
+    const T beta = uu * (P.gam - T(1)) / (T(0.5) * (Bsq + tiny));
+    const T bb = beta * P.ibc;
+    const T b2 = bb * bb;
+    const T trat = rhigh * b2 / (T(1) + b2) + P.rlow / (T(1) + b2);
+    const T te_unit = T(1. / 3. * (MP / ME)) / (trat + T(1));
+    fl.te = (uu / rho) * te_unit;
+    if ((Bsq / (rho + T(1e-20)) > P.sigma_cut) || exp_r(fl.x[1]) > P.rt_outer || fl.te > P.te_max ||
+        fl.te < P.te_min) {
+      fl.ne = T(0);
+      return 0;
+    }
+    return 1;
+  }
   const Met<T> g = metric(P.a, P.hs, fl.x[1], fl.x[2]);
   const MetU<T> u = metric_up(P.a, g);
   fl.g = g;
