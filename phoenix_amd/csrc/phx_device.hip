@@ -64,7 +64,7 @@ struct MarchArgs {
   using T = typename ModeOf<M>::T;
   using L = typename ModeOf<M>::L;
   Grid<T> G;    // cell table and the per-step lookup table
-  Grid<L> GL;   // the block table a sample's cell is located in (FP64 for MIX)
+  Loc<double> GL;  // MIX: the FP64 block table a sample's cell is located in (else G's own table)
   Plasma<T> P;
   const void* start;  // per-job start records, in job order
   T stepsize, r_out, cut_in;
@@ -255,6 +255,14 @@ using StateOf = typename std::conditional<M == MIX, StHam, StRef<typename ModeOf
 template <int M>
 using RecOf = typename std::conditional<M == M32, float, double>::type;  // start record scalar
 
+// the table a sample's cell is located in
+template <int M>
+PHX_DEV Loc<typename ModeOf<M>::L> loc_for(const MarchArgs<M>& A) {
+  if constexpr (M == MIX)
+    return A.GL;
+  else
+    return loc_of(A.G);
+}
 // re-solve k^t for k.k = 0 when |k.k| > 1e-6 and r > 2 (geodesic.hpp:297-306, pixel.hpp:69-71)
 template <class T>
 PHX_DEV void null_resolve(const Met<T>& g, T k[4], T r) {
@@ -291,7 +299,7 @@ PHX_DEV int sample_fluid_stage(const MarchArgs<M>& A, const TaskDev<typename Mod
   T k[4] = {pk[0], pk[1], pk[2], pk[3]};
   if constexpr (M != MIX) null_resolve(metric(A.P.a, A.P.hs, px[1], px[2]), k, exp(px[1]));
   Fluid<T> fl;
-  if (!sample_fluid(A.G, A.GL, A.P, tk.Ne, tk.Bu, tk.rhigh, pl, leaf, fl)) return 0;
+  if (!sample_fluid(A.G, loc_for(A), A.P, tk.Ne, tk.Bu, tk.rhigh, pl, leaf, fl)) return 0;
   if (fl.b[0] == T(0) && fl.b[1] == T(0) && fl.b[2] == T(0) && fl.b[3] == T(0)) return 2;
   // the mixed march's wavevector k^mu = g^{mu nu} p_nu is null by construction up to FP32 rounding;
   // it is checked (and re-solved) with the metric at the sampler's position, the one every later
@@ -364,7 +372,7 @@ PHX_DEV int sample_prepare(const MarchArgs<M>& A, const TaskDev<typename ModeOf<
   T k[4] = {pk[0], pk[1], pk[2], pk[3]};
   if constexpr (M != MIX) null_resolve(metric(A.P.a, A.P.hs, px[1], px[2]), k, exp(px[1]));
   Fluid<T> fl;
-  if (!sample_fluid(A.G, A.GL, A.P, tk.Ne, tk.Bu, tk.rhigh, pl, leaf, fl)) return 0;
+  if (!sample_fluid(A.G, loc_for(A), A.P, tk.Ne, tk.Bu, tk.rhigh, pl, leaf, fl)) return 0;
   if (fl.b[0] == T(0) && fl.b[1] == T(0) && fl.b[2] == T(0) && fl.b[3] == T(0)) return 2;
   // the mixed march's wavevector k^mu = g^{mu nu} p_nu is null by construction up to FP32 rounding;
   // it is checked (and re-solved) with the metric at the sampler's position, the one every later
@@ -1373,7 +1381,10 @@ static void launch_march(phx_snapshot* s, const LaunchInfo& li, int j0, int j1, 
   MarchArgs<M> A;
   std::memset(&A, 0, sizeof A);
   A.G = grid_of<T>(s, tables_of<T>(s));
-  A.GL = grid_of<Lt>(s, tables_of<Lt>(s));
+  if constexpr (M == MIX) {
+    const Grid<double> g64 = grid_of<double>(s, tables_of<double>(s));
+    A.GL = Loc<double>{g64.blk, g64.lo1, g64.lo2, g64.lo3, g64.hi1, g64.hi2, g64.hi3};
+  }
   const phx_model& m = s->geo;
   const phx_model* rq = li.uni;
   A.P.a = T(m.a);

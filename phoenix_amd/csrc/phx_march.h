@@ -547,7 +547,61 @@ PHX_DEV bool in_leaf_closed(const Grid<T>& G, int q, const T x[4]) {  // inclusi
   const T* b = G.blk + 12 * q;
   return !(x[1] < b[0] || x[1] > b[3] || x[2] < b[1] || x[2] > b[4] || x[3] < b[2] || x[3] > b[5]);
 }
+// The table a sample's cell is located in: the block records and the domain bounds, in the
+// locating precision L (FP64 in the mixed-precision march, else the grid's own table)
+template <class L>
+struct Loc {
+  const L* blk;
+  L lo1, lo2, lo3, hi1, hi2, hi3;
+};
+template <class T>
+PHX_DEV Loc<T> loc_of(const Grid<T>& G) {
+  return Loc<T>{G.blk, G.lo1, G.lo2, G.lo3, G.hi1, G.hi2, G.hi3};
+}
+template <class L>
+PHX_DEV bool off_domain(const Loc<L>& G, const L x[4]) {  // sampler.hpp:43-50
+  return x[1] > G.hi1 || x[1] < G.lo1 || x[2] < G.lo2 || x[2] > G.hi2 || x[3] < G.lo3 || x[3] > G.hi3;
+}
+template <class L>
+PHX_DEV bool in_leaf_closed(const Loc<L>& G, int q, const L x[4]) {  // inclusive (sampler.hpp:109-122)
+  const L* b = G.blk + 12 * q;
+  return !(x[1] < b[0] || x[1] > b[3] || x[2] < b[1] || x[2] > b[4] || x[3] < b[2] || x[3] > b[5]);
+}
 // find_leaf (sampler.hpp:143-180) on an already wrapped position
+template <class T>
+PHX_DEV int find_leaf(const Grid<T>& G, const T x[4]);
+// find_leaf of an FP64 position on the FP32 grid's map and the FP64 block table GL (the mixed
+// march): the map candidate is computed from the FP64 offset rounded to FP32, and the candidate is
+// validated, and the linear fallback scan run, on the FP64 boxes, so the result is the leaf whose
+// FP64 box holds the point (a candidate moved by the rounding fails validation and falls back to
+// the scan).  Keeps the FP64 grid's map parameters out of the launch arguments.
+template <class L>
+PHX_DEV int find_leaf_located(const Grid<float>& G, const Loc<L>& GL, const L x[4]) {
+  const L small = x[2] > L(PI_D / 2.) ? L(-1e-9) : L(1e-9);
+  int cand = -1;
+  {
+    const float fi = float(x[1] + small - GL.lo1) / G.w0 * G.up;
+    const float fj = float(x[2] + small - GL.lo2) / G.w1 * G.up;
+    const float fk = float(x[3] + small - GL.lo3) / G.w2 * G.up;
+    if (!(fi < 0.f || fj < 0.f || fk < 0.f)) {
+      const int i = (int)fi, j = (int)fj, k = (int)fk;
+      if (i < G.fd0 && j < G.fd1 && k < G.fd2) cand = G.map[(i * G.fd1 + j) * G.fd2 + k];
+    }
+  }
+  auto inside = [&](int q) {
+    const L* b = GL.blk + 12 * q;
+    return !(x[1] + small < b[0] || x[1] + small >= b[3] || x[2] + small < b[1] || x[2] + small >= b[4] ||
+             x[3] + small < b[2] || x[3] + small >= b[5]);
+  };
+  if (cand >= 0 && inside(cand)) return cand;
+  const L eps = L(1e-6);
+  if (cand < 0 && (x[1] < GL.lo1 - eps || x[1] > GL.hi1 + eps || x[2] < GL.lo2 - eps || x[2] > GL.hi2 + eps ||
+                   x[3] < GL.lo3 - eps || x[3] > GL.hi3 + eps))
+    return -1;
+  for (int q = 0; q < G.nleafs; q++)
+    if (inside(q)) return q;
+  return -1;
+}
 template <class T>
 PHX_DEV int find_leaf(const Grid<T>& G, const T x[4]) {
   const T small = x[2] > T(PI_D / 2.) ? T(-1e-9) : T(1e-9);
@@ -649,7 +703,7 @@ PHX_DEV void load8(const double* p, double v[8]) {
 // derivation run in T (the cell table G); L == T except in the mixed-precision march, which
 // locates the cell in FP64.  Ne, Bu, rhigh are the request's (per image) plasma units.
 template <class T, class L>
-PHX_DEV int sample_fluid(const Grid<T>& G, const Grid<L>& GL, const Plasma<T>& P, T Ne, T Bu, T rhigh, const L X[4],
+PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P, T Ne, T Bu, T rhigh, const L X[4],
                          int cache, Fluid<T>& fl) {
   constexpr bool F32 = Lim<L>::fp32;
   L x[4] = {X[0], X[1], X[2], X[3]};
@@ -660,7 +714,10 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Grid<L>& GL, const Plasma<T>& P
   int q = cache;
   if (q == -1 || !in_leaf_closed(GL, q, x)) {
     wrap(x);
-    q = find_leaf(GL, x);
+    if constexpr (std::is_same<T, L>::value)
+      q = find_leaf(G, x);
+    else
+      q = find_leaf_located(G, GL, x);
   }
 #pragma unroll
   for (int i = 0; i < 4; i++) fl.x[i] = T(x[i]);
