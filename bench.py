@@ -57,23 +57,26 @@ def workload_config(tmpdir: str, preset: str = "c2", name: str = "bench"):
     return cfg, dump, grid, info
 
 
-def algorithmic_flops(info: dict, nf: int, precision: str) -> tuple:
-    """(FP32 flops, FP64 flops) of one launch: counted ops per step and per emitting segment x
-    frequency (tests/golden/op_counts.json: the oracle's count of the reference's method for the
-    FP32 and FP64 paths, the Hamiltonian twin's count for the mixed path) times the launch's own
-    step / segment / dropped-segment counts.  The mixed path's FP64 steps are FP64 work."""
+def algorithmic_flops(info: dict, nf: int, precision: str, rays: int) -> tuple:
+    """(FP32 flops, FP64 flops) of one launch, from the operation counts of the reference's
+    algorithm (tests/golden/op_counts.json: the oracle's count for the FP32 and FP64 paths, the
+    Hamiltonian twin's count for the mixed path, per phase on 2000 pixels of this workload):
+    per ray the camera setup and the frequency-independent work of its samples (the same whether
+    or not an escaping ray's march ends at rt_outer), per geodesic step the step phase (view, AMR
+    lookup, step rule, RK4), per emitting segment x frequency the coefficients, operator and
+    accumulation, times the launch's own ray / step / segment / dropped-segment counts.  The mixed
+    path's FP64 steps are FP64 work."""
     with open(os.path.join(ROOT, "tests", "golden", "op_counts.json")) as f:
-        oc = json.load(f)["c2_1024_230GHz"]
+        o = json.load(f)["c2_1024_230GHz"]["mixed" if precision == "mixed" else precision]
+    fl = {ph: sum(o["ops"][ph][k] for k in ("add", "mul", "div", "sqrt", "trans")) for ph in o["ops"]}
+    per_ray = (fl["setup"] + fl["sample"]) / o["pixels"]
+    per_step = fl["step"] / o["steps"]
     useful_segf = info["nplasma"] * nf - info["bad"]
+    work = rays * per_ray + useful_segf * o["F_segf"]
     if precision == "fp64":
-        o = oc["fp64"]
-        return 0.0, info["steps"] * o["F_step"] + useful_segf * o["F_segf"]
-    if precision == "mixed":
-        o = oc["mixed"]
-        s64 = info.get("fp64_steps", 0)
-        return (info["steps"] - s64) * o["F_step"] + useful_segf * o["F_segf"], s64 * o["F_step"]
-    o = oc["fp32"]
-    return info["steps"] * o["F_step"] + useful_segf * o["F_segf"], 0.0
+        return 0.0, work + info["steps"] * per_step
+    s64 = info.get("fp64_steps", 0) if precision == "mixed" else 0
+    return work + (info["steps"] - s64) * per_step, s64 * per_step
 
 
 def measured_traffic(precision: str):
@@ -89,8 +92,8 @@ def measured_traffic(precision: str):
     return None
 
 
-def roofline(info: dict, nf: int, precision: str, kernel_s: float) -> dict:
-    w32, w64 = algorithmic_flops(info, nf, precision)
+def roofline(info: dict, nf: int, precision: str, kernel_s: float, rays: int = W * H) -> dict:
+    w32, w64 = algorithmic_flops(info, nf, precision, rays)
     achieved = (w32 + w64) / kernel_s / 1e12
     # the peak of this instruction mix: FP32 work at the FP32 peak, FP64 work at the FP64 peak
     t_peak = w32 / (PEAK_TFLOPS["fp32"] * 1e12) + w64 / (PEAK_TFLOPS["fp64"] * 1e12)
@@ -314,7 +317,7 @@ def main() -> int:
                       "vgprs": info["vgprs"], "scratch_bytes": info["scratch_bytes"], "lds_bytes": info["lds_bytes"],
                       "waves_per_cu": info["waves_per_cu"], "grid": [info["blocks"], info["block"]],
                       "device": info["device"], "decode_s": info["decode_s"],
-                      "algorithmic_tflop_per_launch": sum(algorithmic_flops(info, nf, args.precision)) / 1e12,
+                      "algorithmic_tflop_per_launch": sum(algorithmic_flops(info, nf, args.precision, W * H)) / 1e12,
                       "frac_of_unpacked_fp32_peak": roof["achieved"] / 78.6})
         log(f"timed {args.steps} renders ({args.precision}): {wall:.3f} s, {value:.4g} rays*freq/s")
         # the same image by the other paths.  "plain_fp32" is the reference's FP32 method as the
