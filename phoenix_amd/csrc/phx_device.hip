@@ -65,6 +65,7 @@ struct TaskDev {
 // VGPR lanes, each use then costing a v_readlane.
 template <int M>
 struct SideArgs {
+  Grid<typename ModeOf<M>::T> G;  // cell table and the lookup's block table and map
   Plasma<typename ModeOf<M>::T> P;
   Loc<double> GL;  // MIX: the FP64 block table a sample's cell is located in (else G's own table)
 };
@@ -73,9 +74,9 @@ template <int M>
 struct MarchArgs {
   using T = typename ModeOf<M>::T;
   using L = typename ModeOf<M>::L;
-  Grid<T> G;    // cell table and the per-step lookup table
-  const SideArgs<M>* side;  // MIX: the sample parameters in device memory (else P below)
-  Plasma<T> P;
+  Grid<T> G;                // FP32 / FP64: cell table and the lookup's block table and map
+  const SideArgs<M>* side;  // MIX: the grid and sample parameters in device memory (else G and P)
+  Plasma<T> P;              // a, hs and rt_outer are read by every march from here
   const void* start;  // per-job start records, in job order
   T stepsize, r_out, cut_in;
   double a64, hs64, lnr_fp64, x2_pole;  // MIX: FP64 steps where ln r < lnr_fp64 or x2 within x2_pole of a pole
@@ -265,14 +266,15 @@ using StateOf = typename std::conditional<M == MIX, StHam, StRef<typename ModeOf
 template <int M>
 using RecOf = typename std::conditional<M == M32, float, double>::type;  // start record scalar
 
-// the sample parameters.  The mixed march loads them (scalar loads) where a sample needs them: the
-// empty asm makes the pointer a new value at each use, so the loads are not hoisted out of the march
-// loop (measured: mixed 112 -> 107 ms at 1024^2).  The FP32 and FP64 marches read the launch
-// arguments, which measured faster there (FP32 162 vs 172 ms).
+// the grid and sample parameters.  The mixed march loads them (scalar loads) where a lookup miss or
+// a sample needs them: the empty asm makes the pointer a new value at each use, so the loads are not
+// hoisted out of the march loop.  The FP32 and FP64 marches read the launch arguments, which
+// measured faster there (FP32 162 vs 172 ms).
 template <int M>
 PHX_DEV SideArgs<M> side_of(const MarchArgs<M>& A) {
   if constexpr (M != MIX) {
     SideArgs<M> out;
+    out.G = A.G;
     out.P = A.P;
     return out;
   }
@@ -293,7 +295,7 @@ PHX_DEV Loc<typename ModeOf<M>::L> loc_for(const MarchArgs<M>& A, const SideArgs
   if constexpr (M == MIX)
     return S.GL;
   else
-    return loc_of(A.G);
+    return loc_of(S.G);
 }
 // re-solve k^t for k.k = 0 when |k.k| > 1e-6 and r > 2 (geodesic.hpp:297-306, pixel.hpp:69-71)
 template <class T>
@@ -332,7 +334,7 @@ PHX_DEV int sample_fluid_stage(const MarchArgs<M>& A, const TaskDev<typename Mod
   const SideArgs<M> S = side_of(A);
   if constexpr (M != MIX) null_resolve(metric(S.P.a, S.P.hs, px[1], px[2]), k, exp(px[1]));
   Fluid<T> fl;
-  if (!sample_fluid(A.G, loc_for(A, S), S.P, tk.Ne, tk.Bu, tk.rhigh, pl, leaf, fl)) return 0;
+  if (!sample_fluid(S.G, loc_for(A, S), S.P, tk.Ne, tk.Bu, tk.rhigh, pl, leaf, fl)) return 0;
   if (fl.b[0] == T(0) && fl.b[1] == T(0) && fl.b[2] == T(0) && fl.b[3] == T(0)) return 2;
   // the mixed march's wavevector k^mu = g^{mu nu} p_nu is null by construction up to FP32 rounding;
   // it is checked (and re-solved) with the metric at the sampler's position, the one every later
@@ -407,7 +409,7 @@ PHX_DEV int sample_prepare(const MarchArgs<M>& A, const TaskDev<typename ModeOf<
   const SideArgs<M> S = side_of(A);
   if constexpr (M != MIX) null_resolve(metric(S.P.a, S.P.hs, px[1], px[2]), k, exp(px[1]));
   Fluid<T> fl;
-  if (!sample_fluid(A.G, loc_for(A, S), S.P, tk.Ne, tk.Bu, tk.rhigh, pl, leaf, fl)) return 0;
+  if (!sample_fluid(S.G, loc_for(A, S), S.P, tk.Ne, tk.Bu, tk.rhigh, pl, leaf, fl)) return 0;
   if (fl.b[0] == T(0) && fl.b[1] == T(0) && fl.b[2] == T(0) && fl.b[3] == T(0)) return 2;
   // the mixed march's wavevector k^mu = g^{mu nu} p_nu is null by construction up to FP32 rounding;
   // it is checked (and re-solved) with the metric at the sampler's position, the one every later
@@ -643,15 +645,27 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
       T wx[4] = {cx[0], cx[1], cx[2], cx[3]};
       wrap(wx);
       const bool need_dxc = r < A.P.rt_outer;
-      const bool need_leaf = emit && !off_domain(A.G, wx);
+      // leaf_of (sampler.hpp:184-190) is -1 off the domain.  The mixed march hands the sample its
+      // leaf as a hint only: the sample tests the domain and the leaf itself, in FP64
+      bool need_leaf = emit;
+      if constexpr (M != MIX) need_leaf = need_leaf && !off_domain(A.G, wx);
       if (need_dxc || need_leaf) {
-        bool cached;
-        const int Lf = find_leaf_cached(A.G, wx, lcache, &cached);
+        bool cached = leaf_cache_hit(lcache, wx);
+        int Lf = lcache.leaf;
+        if (!cached) {
+          const Grid<T> G = side_of(A).G;
+          Lf = find_leaf_fill(G, wx, lcache, &cached);
+          if (need_dxc && Lf >= 0 && !cached) {  // a leaf of the fallback scan (not cached)
+#pragma unroll
+            for (int d = 0; d < 3; d++) lcache.dxc[d] = G.blk[12 * Lf + 6 + d];
+            lcache.leaf = -1;
+          }
+        }
         if (need_leaf) leaf_next = Lf;
         if (need_dxc && Lf >= 0) {
           have_dxc = true;
 #pragma unroll
-          for (int d = 0; d < 3; d++) dxc[d] = cached ? lcache.dxc[d] : A.G.blk[12 * Lf + 6 + d];
+          for (int d = 0; d < 3; d++) dxc[d] = lcache.dxc[d];
         }
       }
     }
@@ -1418,6 +1432,7 @@ static void launch_march(phx_snapshot* s, const LaunchInfo& li, int j0, int j1, 
   A.G = grid_of<T>(s, tables_of<T>(s));
   SideArgs<M> side;
   std::memset(&side, 0, sizeof side);
+  side.G = A.G;
   if constexpr (M == MIX) {
     const Grid<double> g64 = grid_of<double>(s, tables_of<double>(s));
     side.GL = Loc<double>{g64.blk, g64.lo1, g64.lo2, g64.lo3, g64.hi1, g64.hi2, g64.hi3};

@@ -626,7 +626,7 @@ PHX_DEV int find_leaf(const Grid<T>& G, const T x[4]) {
 }
 // A lane's record of its last lookup: the leaf and its box and cell size.  A ray stays in one leaf
 // for dozens of steps, and while its position lies in that leaf's (half-open, nudged) box the
-// lookup needs no memory access.  Leaves tile the domain, so the leaf whose box holds a point is
+// lookup needs no memory access (and no grid parameter).  Leaves tile the domain, so the leaf whose box holds a point is
 // the one find_leaf returns; the two differ only where rounding makes two boxes overlap by an ulp
 // and the map names the other leaf.
 template <class T>
@@ -634,12 +634,19 @@ struct LeafCache {
   int leaf = -1;
   T box[6], dxc[3];
 };
+// whether the lane's cached leaf holds x (half-open box with find_leaf's nudge)
 template <class T>
-PHX_DEV int find_leaf_cached(const Grid<T>& G, const T x[4], LeafCache<T>& c, bool* cached) {
+PHX_DEV bool leaf_cache_hit(const LeafCache<T>& c, const T x[4]) {
   const T small = x[2] > T(PI_D / 2.) ? T(-1e-9) : T(1e-9);
-  *cached = c.leaf >= 0 && !(x[1] + small < c.box[0] || x[1] + small >= c.box[3] || x[2] + small < c.box[1] ||
-                             x[2] + small >= c.box[4] || x[3] + small < c.box[2] || x[3] + small >= c.box[5]);
-  if (*cached) return c.leaf;
+  return c.leaf >= 0 && !(x[1] + small < c.box[0] || x[1] + small >= c.box[3] || x[2] + small < c.box[1] ||
+                          x[2] + small >= c.box[4] || x[3] + small < c.box[2] || x[3] + small >= c.box[5]);
+}
+// find_leaf on a miss of the lane's cache: the map candidate, once validated, refills the cache
+// (*cached then tells that c.dxc holds the leaf's cell size); the fallbacks are find_leaf's
+template <class T>
+PHX_DEV int find_leaf_fill(const Grid<T>& G, const T x[4], LeafCache<T>& c, bool* cached) {
+  const T small = x[2] > T(PI_D / 2.) ? T(-1e-9) : T(1e-9);
+  *cached = false;
   int cand = -1;
   {
     const T fi = (x[1] + small - G.lo1) / G.w0 * G.up;
