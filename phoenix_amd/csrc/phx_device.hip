@@ -68,6 +68,12 @@ struct SideArgs {
   Grid<typename ModeOf<M>::T> G;  // cell table and the lookup's block table and map
   Plasma<typename ModeOf<M>::T> P;
   Loc<double> GL;  // MIX: the FP64 block table a sample's cell is located in (else G's own table)
+  const TaskDev<typename ModeOf<M>::T>* tasks;
+  const int2* jobs;
+  const void* start;  // per-job start records, in job order
+  int* counter;
+  unsigned long long* tot;
+  int j0, j1;
 };
 
 template <int M>
@@ -276,6 +282,13 @@ PHX_DEV SideArgs<M> side_of(const MarchArgs<M>& A) {
     SideArgs<M> out;
     out.G = A.G;
     out.P = A.P;
+    out.tasks = A.tasks;
+    out.jobs = A.jobs;
+    out.start = A.start;
+    out.counter = A.counter;
+    out.tot = A.tot;
+    out.j0 = A.j0;
+    out.j1 = A.j1;
     return out;
   }
   const SideArgs<M>* p = A.side;
@@ -573,20 +586,21 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
       wait = busy != 0ull && 64 - (int)__popcll(busy) < A.refill;
     }
     if (!have && !wait) {
+      const SideArgs<M> S = side_of(A);
       const unsigned long long m = __ballot(1);
       const int leader = __ffsll((long long)m) - 1;
       int base = 0;
-      if (lane == leader) base = atomicAdd(A.counter, (int)__popcll(m));
+      if (lane == leader) base = atomicAdd(S.counter, (int)__popcll(m));
       base = __shfl(base, leader);
-      const int n = A.j0 + base + (int)__popcll(m & lt);
-      if (n >= A.j1) break;
-      const int2 jb = A.jobs[n];
+      const int n = S.j0 + base + (int)__popcll(m & lt);
+      if (n >= S.j1) break;
+      const int2 jb = S.jobs[n];
       task = jb.x;
       pix = jb.y;
-      nf = A.tasks[task].nf;
-      stats = A.tasks[task].stats != 0;
+      nf = S.tasks[task].nf;
+      stats = S.tasks[task].stats != 0;
       T bsign;
-      st.load(static_cast<const R*>(A.start) + (size_t)n * St::REC, &bsign);
+      st.load(static_cast<const R*>(S.start) + (size_t)n * St::REC, &bsign);
 #pragma unroll
       for (int f = 0; f < NF; f++) {
 #pragma unroll
@@ -696,7 +710,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
       qcount += (int)__popcll(pm);
     } else if (emit) {
       // ---- the pending sample, in place
-      const TaskDev<T>& tk = A.tasks[task];
+      const TaskDev<T>& tk = side_of(A).tasks[task];
       Prep<T> sp;
       const int status = sample_prepare<M>(A, tk, px, pl, pk, pf, pdl, leaf_next, sp);
       if (status) np++;
@@ -779,7 +793,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
               for (int q = 0; q < 3; q++) xl[1 + q] = e64[q];
             }
             const int etask = (int)in[20], owner = (int)in[21];
-            const TaskDev<T>& tk = A.tasks[etask];
+            const TaskDev<T>& tk = side_of(A).tasks[etask];
             // the candidate record overwrites the slot; everything of the queued sample is read first
             const T sdl = in[12];
             const int sleaf = (int)in[13];
@@ -828,7 +842,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
           if (mine) {
             ctask = (int)rec[22];
             status = sample_frame_stage<M>(A, rec, sp);
-            const TaskDev<T>& tk = A.tasks[ctask];
+            const TaskDev<T>& tk = side_of(A).tasks[ctask];
             enf = tk.nf;
             if (status != 0 && tk.stats) s_np++;
             if (status == 2) s_bad += enf;
@@ -842,7 +856,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
               int flag = 0;  // 1: fold in; 0: nothing to fold (no emission, or dropped)
               if (status == 1) {
                 T OA[16], OS[4], dtau, dtauF;
-                if (sample_op(A.tasks[ctask], sp, f, OA, OS, &dtau, &dtauF)) {
+                if (sample_op(side_of(A).tasks[ctask], sp, f, OA, OS, &dtau, &dtauF)) {
 #pragma unroll
                   for (int q = 0; q < 16; q++) rec[q] = OA[q];
 #pragma unroll
@@ -912,7 +926,8 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
 
     if (!alive) {
       // ---- ray done: write it out and free the lane (pixel.hpp:213-223)
-      phx_freqout* o0 = A.tasks[task].out + (size_t)pix * A.tasks[task].nf_total;
+      const TaskDev<T>& tkw = side_of(A).tasks[task];
+      phx_freqout* o0 = tkw.out + (size_t)pix * tkw.nf_total;
 #pragma unroll
       for (int f = 0; f < NF; f++) {
         if (f >= nf) break;
@@ -947,10 +962,11 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
     if (M == MIX) w_fp64 += __shfl_xor(w_fp64, off);
   }
   if (lane == 0) {
-    atomicAdd(&A.tot[0], w_steps);
-    atomicAdd(&A.tot[1], w_np);
-    atomicAdd(&A.tot[2], w_bad);
-    if (M == MIX) atomicAdd(&A.tot[3], w_fp64);
+    unsigned long long* tot = side_of(A).tot;
+    atomicAdd(&tot[0], w_steps);
+    atomicAdd(&tot[1], w_np);
+    atomicAdd(&tot[2], w_bad);
+    if (M == MIX) atomicAdd(&tot[3], w_fp64);
   }
 }
 
@@ -1449,9 +1465,6 @@ static void launch_march(phx_snapshot* s, const LaunchInfo& li, int j0, int j1, 
   A.P.te_max = T(rq->thetae_max);
   A.P.rt_outer = T(rq->rt_outer);
   side.P = A.P;
-  // from pageable host memory: the call returns once the source has been staged
-  A.side = static_cast<const SideArgs<M>*>(ensure(s->side, sizeof side));
-  HIPCHECK(hipMemcpyAsync(const_cast<SideArgs<M>*>(A.side), &side, sizeof side, hipMemcpyHostToDevice, s->stream));
   A.start = s->start.p;
   A.stepsize = T(rq->stepsize);
   A.r_out = T(rq->r_out);
@@ -1471,6 +1484,16 @@ static void launch_march(phx_snapshot* s, const LaunchInfo& li, int j0, int j1, 
   A.j1 = j1;
   A.counter = s->d_counter;
   A.tot = s->d_tot;
+  side.tasks = A.tasks;
+  side.jobs = A.jobs;
+  side.start = A.start;
+  side.counter = A.counter;
+  side.tot = A.tot;
+  side.j0 = A.j0;
+  side.j1 = A.j1;
+  // from pageable host memory: the call returns once the source has been staged
+  A.side = static_cast<const SideArgs<M>*>(ensure(s->side, sizeof side));
+  HIPCHECK(hipMemcpyAsync(const_cast<SideArgs<M>*>(A.side), &side, sizeof side, hipMemcpyHostToDevice, s->stream));
   const void* kfn = reinterpret_cast<const void*>(&k_march<M, NF, Q>);
   int cus = 0, per_cu = 0;
   HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));

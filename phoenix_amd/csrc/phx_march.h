@@ -119,75 +119,77 @@ PHX_DEV void theta_trig(double x2, double hs, double* s2x, double* c2x, double* 
   sincos(2. * x2, s2x, c2x);
   sincos(x2 + 0.5 * hs * *s2x, s, c);
 }
+// Taylor coefficients of sin r / r and cos r in z = r^2 (Horner order): sin to r^19, cos to r^20.
+// Kept in constant memory and loaded where a polynomial is evaluated (taylor_coef): as FP64
+// instruction operands they would otherwise be materialized in scalar registers and held across
+// the whole march loop, which spills the march's other uniform values.
+__constant__ double kTaylorSin[9] = {-1. / 121645100408832000., 1. / 355687428096000., -1. / 1307674368000.,
+                                     1. / 6227020800.,          -1. / 39916800.,       1. / 362880.,
+                                     -1. / 5040.,               1. / 120.,             -1. / 6.};
+__constant__ double kTaylorCos[10] = {1. / 2432902008176640000., -1. / 6402373705728000., 1. / 20922789888000.,
+                                      -1. / 87178291200.,        1. / 479001600.,         -1. / 3628800.,
+                                      1. / 40320.,               -1. / 720.,              1. / 24.,
+                                      -0.5};
+struct TaylorSC {
+  double s[9], c[10];
+};
+// the coefficients, loaded (scalar loads) at the call: the empty asm makes the table address a new
+// value there, so the loads are not hoisted out of an enclosing loop
+PHX_DEV TaylorSC taylor_coef() {
+  const double* ps = kTaylorSin;
+  const double* pc = kTaylorCos;
+  asm volatile("" : "+s"(ps), "+s"(pc));
+  using W = const __attribute__((address_space(4))) double*;
+  TaylorSC t;
+#pragma unroll
+  for (int i = 0; i < 9; i++) t.s[i] = ((W)ps)[i];
+#pragma unroll
+  for (int i = 0; i < 10; i++) t.c[i] = ((W)pc)[i];
+  return t;
+}
+// sin(r) and cos(r) by the Taylor polynomials above (|r| <= pi/4: truncation below 1e-19)
+PHX_DEV void taylor_sc(const TaylorSC& t, double r, double* sr, double* cr) {
+  const double z = r * r;
+  double ps = t.s[0];
+#pragma unroll
+  for (int i = 1; i < 9; i++) ps = fma(ps, z, t.s[i]);
+  *sr = fma(ps * z, r, r);
+  double pc = t.c[0];
+#pragma unroll
+  for (int i = 1; i < 10; i++) pc = fma(pc, z, t.c[i]);
+  *cr = fma(pc, z, 1.);
+}
 // FP64 sin and cos of a small |x| (< ~1e3): Cody-Waite reduction by pi/2 with a two-part
-// constant (fma), Taylor polynomials on [-pi/4, pi/4] (sin to r^19, cos to r^20: truncation
-// below 1e-19), quadrant from the low bits of the quotient.  Compact, so that the FP64 steps of
-// the mixed march keep few registers (the library's sincos carries a large-argument path).
-PHX_DEV void sincos_small(double x, double* s, double* c) {
+// constant (fma), the Taylor polynomials on [-pi/4, pi/4], quadrant from the low bits of the
+// quotient.  Compact, so that the FP64 steps of the mixed march keep few registers (the library's
+// sincos carries a large-argument path).
+PHX_DEV void sincos_small(const TaylorSC& t, double x, double* s, double* c) {
   const double q = rint(x * 0.63661977236758134308);
   const int qi = (int)q;
   double r = fma(-q, 1.5707963267948966192, x);
   r = fma(-q, 6.123233995736766036e-17, r);
-  const double z = r * r;
-  double ps = -1. / 121645100408832000.;
-  ps = fma(ps, z, 1. / 355687428096000.);
-  ps = fma(ps, z, -1. / 1307674368000.);
-  ps = fma(ps, z, 1. / 6227020800.);
-  ps = fma(ps, z, -1. / 39916800.);
-  ps = fma(ps, z, 1. / 362880.);
-  ps = fma(ps, z, -1. / 5040.);
-  ps = fma(ps, z, 1. / 120.);
-  ps = fma(ps, z, -1. / 6.);
-  const double sr = fma(ps * z, r, r);
-  double pc = 1. / 2432902008176640000.;
-  pc = fma(pc, z, -1. / 6402373705728000.);
-  pc = fma(pc, z, 1. / 20922789888000.);
-  pc = fma(pc, z, -1. / 87178291200.);
-  pc = fma(pc, z, 1. / 479001600.);
-  pc = fma(pc, z, -1. / 3628800.);
-  pc = fma(pc, z, 1. / 40320.);
-  pc = fma(pc, z, -1. / 720.);
-  pc = fma(pc, z, 1. / 24.);
-  pc = fma(pc, z, -0.5);
-  const double cr = fma(pc, z, 1.);
+  double sr, cr;
+  taylor_sc(t, r, &sr, &cr);
   double sn = (qi & 1) ? cr : sr, cs = (qi & 1) ? sr : cr;
   if (qi & 2) sn = -sn;
   if ((qi + 1) & 2) cs = -cs;
   *s = sn;
   *c = cs;
 }
+PHX_DEV void sincos_small(double x, double* s, double* c) { sincos_small(taylor_coef(), x, s, c); }
 
 // FP64 with one compact sincos: sin/cos of x2, the double-angle identities, and theta = x2 +
-// delta by angle addition with sin/cos of |delta| <= h/2 <= 0.5 in Taylor series to delta^19 /
-// delta^20 (truncation below 2e-23).  Used by the FP64 steps of the mixed march.
+// delta by angle addition with sin/cos of |delta| <= h/2 <= 0.5 by the same Taylor polynomials
+// (truncation below 2e-23).  Used by the FP64 steps of the mixed march.
 PHX_DEV void theta_trig_1(double x2, double hs, double* s2x, double* c2x, double* s, double* c) {
+  const TaylorSC t = taylor_coef();
   double sx, cx;
-  sincos_small(x2, &sx, &cx);
+  sincos_small(t, x2, &sx, &cx);
   *s2x = 2. * sx * cx;
   *c2x = (cx - sx) * (cx + sx);
-  const double d = 0.5 * hs * *s2x, z = d * d;
-  // sin d = d (1 - z/3! + z^2/5! - ...), cos d = 1 - z/2! + z^2/4! - ... (Horner in z)
-  double ps = -1. / 121645100408832000.;  // -1/19!
-  ps = fma(ps, z, 1. / 355687428096000.);
-  ps = fma(ps, z, -1. / 1307674368000.);
-  ps = fma(ps, z, 1. / 6227020800.);
-  ps = fma(ps, z, -1. / 39916800.);
-  ps = fma(ps, z, 1. / 362880.);
-  ps = fma(ps, z, -1. / 5040.);
-  ps = fma(ps, z, 1. / 120.);
-  ps = fma(ps, z, -1. / 6.);
-  const double sd = fma(ps * z, d, d);
-  double pc = 1. / 2432902008176640000.;  // 1/20!
-  pc = fma(pc, z, -1. / 6402373705728000.);
-  pc = fma(pc, z, 1. / 20922789888000.);
-  pc = fma(pc, z, -1. / 87178291200.);
-  pc = fma(pc, z, 1. / 479001600.);
-  pc = fma(pc, z, -1. / 3628800.);
-  pc = fma(pc, z, 1. / 40320.);
-  pc = fma(pc, z, -1. / 720.);
-  pc = fma(pc, z, 1. / 24.);
-  pc = fma(pc, z, -0.5);
-  const double cd = fma(pc, z, 1.);
+  const double d = 0.5 * hs * *s2x;
+  double sd, cd;
+  taylor_sc(t, d, &sd, &cd);
   *s = sx * cd + cx * sd;
   *c = cx * cd - sx * sd;
 }
