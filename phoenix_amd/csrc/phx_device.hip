@@ -146,6 +146,7 @@ struct StHam {
   double pt, pp;
   float kap1, kap2;
   double d1[5];  // first stage of the coming step: FP64, or the (mirrored) FP32 derivative widened
+  Pt32 tb;       // FP32 step: the point quantities of its first stage, the base of the others
   bool f64 = false;
   PHX_DEV void load(const double* st, float* bsign) {
 #pragma unroll
@@ -197,7 +198,8 @@ struct StHam {
     } else {
       float b[5], d[5], k0;
       const bool south = base32(b);
-      ham_rhs<float>(args.P.a, args.P.hs, float(pt), float(pp), b, d, &k0);
+      tb = pt32_at(b[0], b[1], args.P.hs);
+      ham_core<float>(args.P.a, args.P.hs, float(pt), float(pp), tb.r, tb.s2x, tb.c2x, tb.s, tb.c, b[3], b[4], d, &k0);
 #pragma unroll
       for (int q = 0; q < 5; q++) d1[q] = d[q];
       ck[0] = k0;
@@ -242,16 +244,22 @@ struct StHam {
         sum[i] = d;
         y[i] = b[i] + 0.5f * dl * d;
       }
+      // stages 2-4 at the offsets h d of stage 1's point (|h d| <= 0.05; else evaluated afresh)
+      float e1 = 0.5f * dl * float(d1[0]), e2 = 0.5f * dl * float(d1[1]);
 #pragma unroll
       for (int q = 1; q < 4; q++) {
         float d[5], k0;
-        ham_rhs<float>(args.P.a, args.P.hs, pt32, pp32, y, d, &k0);
+        const Pt32 t = (fabsf(e1) <= 0.05f && fabsf(e2) <= 0.05f) ? pt32_offset(tb, e1, e2, args.P.hs)
+                                                                   : pt32_at(y[0], y[1], args.P.hs);
+        ham_core<float>(args.P.a, args.P.hs, pt32, pp32, t.r, t.s2x, t.c2x, t.s, t.c, y[3], y[4], d, &k0);
         const float w = q == 3 ? 1.f : 2.f, h = q == 2 ? dl : 0.5f * dl;
 #pragma unroll
         for (int i = 0; i < 5; i++) {
           sum[i] += w * d[i];
           if (q < 3) y[i] = b[i] + h * d[i];
         }
+        e1 = h * d[0];
+        e2 = h * d[1];
       }
       const float dt6 = dl / 6.f;
       float inc[5];

@@ -407,11 +407,70 @@ PHX_DEV void rk4(T a, T hs, Ray<T>& g, T dt) {
 // the explicit theta'(x2) of the p_2 term), *k0 = k^t = g^{tt} p_t + g^{t1} p_1.
 // RECIP: S, 1/r, 1/sin theta and 1/theta' from one reciprocal of their product (FP64, where a
 // division is a long sequence); otherwise four reciprocals (FP32, hardware rcp).
+// ham_core: the same from r, sin/cos 2x2 and sin/cos theta at the point.
+template <class T, bool RECIP = false>
+PHX_DEV void ham_core(T a, T hs, T pt, T pp, T r, T s2x, T c2x, T s, T c, T p1, T p2, T d[5], T* k0);
 template <class T, bool ONE_SINCOS = false, bool RECIP = false>
 PHX_DEV void ham_rhs(T a, T hs, T pt, T pp, const T y[5], T d[5], T* k0) {
   const T r = exp_r(y[0]);
   T s2x, c2x, s, c;
   theta_trig_sel<ONE_SINCOS>(y[1], hs, &s2x, &c2x, &s, &c);
+  ham_core<T, RECIP>(a, hs, pt, pp, r, s2x, c2x, s, c, y[3], y[4], d, k0);
+}
+// The point quantities an FP32 RK4 stage needs: r, sin/cos x2, sin/cos 2x2, sin/cos theta.
+struct Pt32 {
+  float r, sx, cx, s2x, c2x, s, c;
+};
+PHX_DEV Pt32 pt32_at(float x1, float x2, float hs) {
+  Pt32 t;
+  t.r = exp_r(x1);
+  sincos_small(x2, &t.sx, &t.cx);
+  t.s2x = 2.0f * t.sx * t.cx;
+  t.c2x = (t.cx - t.sx) * (t.cx + t.sx);
+  const float d = 0.5f * hs * t.s2x, z = d * d;
+  float sd = fmaf(z, 2.7557319223985893e-6f, -1.9841269841269841e-4f);
+  sd = fmaf(sd, z, 8.3333333333333333e-3f);
+  sd = fmaf(sd, z, -1.6666666666666667e-1f);
+  sd = fmaf(sd * z, d, d);
+  float cd = fmaf(z, -2.7557319223985888e-7f, 2.4801587301587302e-5f);
+  cd = fmaf(cd, z, -1.3888888888888889e-3f);
+  cd = fmaf(cd, z, 4.1666666666666667e-2f);
+  cd = fmaf(cd, z, -0.5f);
+  cd = fmaf(cd, z, 1.0f);
+  t.s = t.sx * cd + t.cx * sd;
+  t.c = t.cx * cd - t.sx * sd;
+  return t;
+}
+// sin and cos of a small |e| <= 0.05 (Taylor to e^5 / e^6: truncation below 2e-11)
+PHX_DEV void sincos_tiny(float e, float* s, float* c) {
+  const float z = e * e;
+  *s = fmaf(e * z, fmaf(z, 1.f / 120.f, -1.f / 6.f), e);
+  *c = fmaf(z, fmaf(z, fmaf(z, -1.f / 720.f, 1.f / 24.f), -0.5f), 1.f);
+}
+// The point quantities at (x1 + e1, x2 + e2) from those at (x1, x2), for offsets |e1|, |e2| <=
+// 0.05 (an RK4 stage of a step, whose rule keeps |dl k^1| and |dl k^2| near the 0.01 stepsize):
+// exp by its Taylor polynomial to e1^5, sin/cos by angle addition with sincos_tiny, theta'
+// offset = e2 + (h/2)(sin 2(x2 + e2) - sin 2x2).  Errors stay at FP32 rounding; the offsets are
+// applied to the base point exactly, not to a rounded stage position.
+PHX_DEV Pt32 pt32_offset(const Pt32& b, float e1, float e2, float hs) {
+  Pt32 t;
+  const float ex = e1 * fmaf(e1, fmaf(e1, fmaf(e1, fmaf(e1, 1.f / 120.f, 1.f / 24.f), 1.f / 6.f), 0.5f), 1.f);
+  t.r = fmaf(b.r, ex, b.r);
+  float se, ce;
+  sincos_tiny(e2, &se, &ce);
+  t.sx = b.sx * ce + b.cx * se;
+  t.cx = b.cx * ce - b.sx * se;
+  t.s2x = 2.0f * t.sx * t.cx;
+  t.c2x = (t.cx - t.sx) * (t.cx + t.sx);
+  const float dth = fmaf(0.5f * hs, t.s2x - b.s2x, e2);
+  float st, ct;
+  sincos_tiny(dth, &st, &ct);
+  t.s = b.s * ct + b.c * st;
+  t.c = b.c * ct - b.s * st;
+  return t;
+}
+template <class T, bool RECIP>
+PHX_DEV void ham_core(T a, T hs, T pt, T pp, T r, T s2x, T c2x, T s, T c, T p1, T p2, T d[5], T* k0) {
   const T tp = T(1) + hs * c2x, tpp = T(-2) * hs * s2x;
   const T a2 = a * a;
   const T Sig = r * r + a2 * c * c;
@@ -430,7 +489,6 @@ PHX_DEV void ham_rhs(T a, T hs, T pt, T pp, const T y[5], T d[5], T* k0) {
     itp = T(1) / tp;
   }
   const T is2 = is * is, itp2 = itp * itp;
-  const T p1 = y[3], p2 = y[4];
   const T dr2 = T(1) - T(2) * ir + a2 * ir * ir;  // Delta / r^2
   const T ptt = pt * pt, q22 = p2 * p2 * itp2, q33 = pp * pp * is2;
   const T Q = T(-2) * r * ptt + T(4) * pt * p1 + dr2 * p1 * p1 + T(2) * a * ir * p1 * pp + q22 + q33;
