@@ -340,6 +340,7 @@ struct Prep {
 // [6..9] U^mu, [10..13] b^mu, [14..17] the ray's polarization vector (or its Walker-Penrose
 // constant), [18] Theta_e, [19] n_e, [20] |B|, [21] the step length, [22] the task.
 constexpr int CREC = 23;
+static_assert(CREC < 24, "a candidate record fits a queue slot (QW scalars) with the owner in slot 23");
 // First half of a sample (src/core/render/pixel.hpp:52-79): null re-solve of k, fluid at the
 // sample (wrap, cached leaf / lookup, cell, trilinear gather, derive_fluid, cuts).
 // 0: no emission; 2: emits nothing finite (b^mu = 0, a field-free cell: the pitch angle is pi/2,
@@ -1450,7 +1451,6 @@ struct LaunchInfo {
 template <int M, int NF, bool Q>
 static void launch_march(phx_snapshot* s, const LaunchInfo& li, int j0, int j1, const Tune& tn, phx_info* info) {
   using T = typename ModeOf<M>::T;
-  using Lt = typename ModeOf<M>::L;
   MarchArgs<M> A;
   std::memset(&A, 0, sizeof A);
   A.G = grid_of<T>(s, tables_of<T>(s));
