@@ -1041,6 +1041,35 @@ __global__ __launch_bounds__(BLOCK) void k_rays(const Cam* __restrict__ cams, co
   }
 }
 
+// Per leaf: the largest rho if every cell is field-free (B^i = 0), has rho >= 0 and finite
+// primitives, else -1.  One workgroup per leaf.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_leafmeta(const T* __restrict__ prim, int cells, T* __restrict__ meta) {
+  __shared__ T smax[BLOCK];
+  __shared__ int sbad[BLOCK];
+  const int q = blockIdx.x;
+  const T* p = prim + (size_t)q * cells * 8;
+  T mx = T(0);
+  int bad = 0;
+  for (int c = threadIdx.x; c < cells; c += BLOCK) {
+    const T* v = p + (size_t)c * 8;
+    for (int a = 0; a < 8; a++) bad |= !isfinite(v[a]);
+    bad |= v[5] != T(0) || v[6] != T(0) || v[7] != T(0) || v[0] < T(0);
+    mx = fmax(mx, v[0]);
+  }
+  smax[threadIdx.x] = mx;
+  sbad[threadIdx.x] = bad;
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+      sbad[threadIdx.x] |= sbad[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) meta[q] = sbad[0] ? T(-1) : smax[0];
+}
+
 // cons -> prim of one cell in FP64 (src/core/model/bhac.hpp:178-346): metric at the
 // sqrt(gamma)-weighted 27-point Simpson centroid, Valencia inversion, entropy fallback.
 template <class T>
@@ -1117,6 +1146,7 @@ template <class T>
 struct Tables {
   T* blk = nullptr;   // [nleafs][12] block records
   T* prim = nullptr;  // [nleafs * cells][8] primitives
+  T* meta = nullptr;  // [nleafs] field-free leaves' largest rho, else -1 (k_leafmeta)
 };
 
 // Launch knobs (picked by the autotuner or from the environment)
@@ -1370,6 +1400,17 @@ static phx_snapshot* snapshot_open(const phx_model* m, const char* dump, int tab
   HIPCHECK(hipStreamSynchronize(s->stream));
   s->upload_s = now_s() - t0;
   if (!m->prim) decode_dump(s.get(), m, dump);
+  // the field-free leaves of each resident cell table
+  if (tables & PHX_TABLE_FP32) {
+    s->t32.meta = dalloc<float>(s.get(), (size_t)m->nleafs * sizeof(float));
+    k_leafmeta<float><<<m->nleafs, BLOCK, 0, s->stream>>>(s->t32.prim, m->cells, s->t32.meta);
+  }
+  if (tables & PHX_TABLE_FP64) {
+    s->t64.meta = dalloc<double>(s.get(), (size_t)m->nleafs * sizeof(double));
+    k_leafmeta<double><<<m->nleafs, BLOCK, 0, s->stream>>>(s->t64.prim, m->cells, s->t64.meta);
+  }
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(s->stream));
   return s.release();
 }
 
@@ -1378,6 +1419,7 @@ static Grid<T> grid_of(const phx_snapshot* s, const Tables<T>& tb) {
   const phx_model& m = s->geo;
   Grid<T> G;
   G.blk = tb.blk;
+  G.meta = tb.meta;
   G.prim = tb.prim;
   G.map = s->map;
   G.nleafs = m.nleafs;

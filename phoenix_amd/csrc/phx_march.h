@@ -570,6 +570,8 @@ PHX_DEV T step_len(T stepsize, T x2, const T k[4], bool have_dxc, const T dxc[3]
 template <class T>
 struct Grid {
   const T* blk;    // [nleafs][12]: lo[3], hi[3], dxc[3], 1/dxc[3]
+  const T* meta;   // [nleafs]: the largest rho of a field-free leaf (B^i = 0, rho >= 0 and finite
+                   // primitives in every cell), else -1
   const T* prim;   // [nleafs * cells][8]
   const int* map;  // [fd0][fd1][fd2]
   int nleafs, cells, nx0, nx1, nx2, fd0, fd1, fd2;
@@ -758,6 +760,14 @@ PHX_DEV void load8(const float* p, float v[8]) {
   v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
   v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
 }
+PHX_DEV void load2(const float* p, float v[2]) {
+  const float2 a = *reinterpret_cast<const float2*>(p);
+  v[0] = a.x; v[1] = a.y;
+}
+PHX_DEV void load2(const double* p, double v[2]) {
+  const double2 a = *reinterpret_cast<const double2*>(p);
+  v[0] = a.x; v[1] = a.y;
+}
 PHX_DEV void load8(const double* p, double v[8]) {
   const double4 a = *reinterpret_cast<const double4*>(p), b = *reinterpret_cast<const double4*>(p + 4);
   v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
@@ -789,6 +799,17 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
 #pragma unroll
   for (int i = 0; i < 4; i++) fl.x[i] = T(x[i]);
   if (q == -1) return 0;
+  // A field-free leaf (B^i = 0, rho >= 0 and finite primitives in every cell): every sample in it has b = 0
+  // exactly, and needs only rho and u.  Where its largest rho is so low that even that rho fails
+  // the sigma cut (sigma = (0 + 1e-6) / (rho + 1e-20) > sigma_cut; the interpolated rho is a convex
+  // combination of the corners, so it lies in [0, max]; the margin covers its rounding), every
+  // sample in it is cut: nothing emits and nothing is counted, as the full derivation would find.
+  const T rho_ff = G.meta[q];
+  const bool ffleaf = rho_ff >= T(0);
+  if (ffleaf && T(1e-6) > P.sigma_cut * (rho_ff + T(1e-20)) * T(1.00001)) {
+    fl.ne = T(0);
+    return 0;
+  }
   const L* b = GL.blk + 12 * q;
   const int nxs[3] = {G.nx0, G.nx1, G.nx2};
   int c[3];
@@ -819,22 +840,40 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
   T qv[8];
 #pragma unroll
   for (int a = 0; a < 8; a++) qv[a] = T(0);
+  if (ffleaf) {
+    // a field-free leaf: only rho and u of the 8 corners, interpolated as below
 #pragma unroll
-  for (int z = 0; z < 2; z++) {
-    const int ck = z ? kp : c[2];
-    const T wz = z ? del[2] : w3;
+    for (int z = 0; z < 2; z++) {
+      const int ck = z ? kp : c[2];
+      const T wz = z ? del[2] : w3;
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-      T v[8];
-      load8(base + (size_t)(ci[s] + cj[s] * G.nx0 + ck * G.nx0 * G.nx1) * 8, v);
-      const T w = wq[s] * wz;
+      for (int s = 0; s < 4; s++) {
+        T v[2];
+        load2(base + (size_t)(ci[s] + cj[s] * G.nx0 + ck * G.nx0 * G.nx1) * 8, v);
+        const T w = wq[s] * wz;
 #pragma unroll
-      for (int a = 0; a < 8; a++) qv[a] += v[a] * w;
+        for (int a = 0; a < 2; a++) qv[a] += v[a] * w;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int z = 0; z < 2; z++) {
+      const int ck = z ? kp : c[2];
+      const T wz = z ? del[2] : w3;
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        T v[8];
+        load8(base + (size_t)(ci[s] + cj[s] * G.nx0 + ck * G.nx0 * G.nx1) * 8, v);
+        const T w = wq[s] * wz;
+#pragma unroll
+        for (int a = 0; a < 8; a++) qv[a] += v[a] * w;
+      }
     }
   }
   // derive_fluid
   const T tiny = T(1.e-6);
-  if (qv[5] == T(0) && qv[6] == T(0) && qv[7] == T(0) && isfinite(qv[2]) && isfinite(qv[3]) && isfinite(qv[4])) {
+  if (ffleaf ||
+      (qv[5] == T(0) && qv[6] == T(0) && qv[7] == T(0) && isfinite(qv[2]) && isfinite(qv[3]) && isfinite(qv[4]))) {
     // a field-free sample (B^i = 0 at all 8 corners): b^mu = 0 exactly, whatever the velocity and the
     // metric (alpha and the Lorentz factor are finite: g^tt < 0 and the Kerr-Schild spatial metric is
     // positive definite), so |B|, beta, Theta_e and the cuts follow from rho and u alone, with the
