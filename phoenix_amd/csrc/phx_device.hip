@@ -1400,14 +1400,29 @@ static phx_snapshot* snapshot_open(const phx_model* m, const char* dump, int tab
   HIPCHECK(hipStreamSynchronize(s->stream));
   s->upload_s = now_s() - t0;
   if (!m->prim) decode_dump(s.get(), m, dump);
-  // the field-free leaves of each resident cell table
+  // the field-free leaves of each resident cell table (PHX_FIELD_FREE_LEAVES=0: none is marked, every
+  // sample takes the full derivation)
+  const char* ffe = std::getenv("PHX_FIELD_FREE_LEAVES");
+  const bool mark = !(ffe && std::atoi(ffe) == 0);
   if (tables & PHX_TABLE_FP32) {
     s->t32.meta = dalloc<float>(s.get(), (size_t)m->nleafs * sizeof(float));
-    k_leafmeta<float><<<m->nleafs, BLOCK, 0, s->stream>>>(s->t32.prim, m->cells, s->t32.meta);
+    if (mark) {
+      k_leafmeta<float><<<m->nleafs, BLOCK, 0, s->stream>>>(s->t32.prim, m->cells, s->t32.meta);
+    } else {
+      const std::vector<float> none(m->nleafs, -1.f);
+      HIPCHECK(hipMemcpyAsync(s->t32.meta, none.data(), none.size() * sizeof(float), hipMemcpyHostToDevice, s->stream));
+      HIPCHECK(hipStreamSynchronize(s->stream));
+    }
   }
   if (tables & PHX_TABLE_FP64) {
     s->t64.meta = dalloc<double>(s.get(), (size_t)m->nleafs * sizeof(double));
-    k_leafmeta<double><<<m->nleafs, BLOCK, 0, s->stream>>>(s->t64.prim, m->cells, s->t64.meta);
+    if (mark) {
+      k_leafmeta<double><<<m->nleafs, BLOCK, 0, s->stream>>>(s->t64.prim, m->cells, s->t64.meta);
+    } else {
+      const std::vector<double> none(m->nleafs, -1.);
+      HIPCHECK(hipMemcpyAsync(s->t64.meta, none.data(), none.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+      HIPCHECK(hipStreamSynchronize(s->stream));
+    }
   }
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(s->stream));

@@ -186,6 +186,32 @@ def test_escaping_rays_end_at_rt_outer_with_the_same_results(c1_run, precision, 
     snap.close()
 
 
+@pytest.mark.parametrize("preset", ["c1", "c2"])
+def test_field_free_leaves_change_nothing(preset, snapdir, tmp_path, monkeypatch):
+    """Samples in field-free leaves (B = 0 in every cell) gather only rho and u, and those whose leaf's
+    largest rho already fails the sigma cut are cut without a gather: bit-identical results and
+    counts against the full derivation of every sample (PHX_FIELD_FREE_LEAVES=0), in every
+    precision, with the default and a stricter sigma cut."""
+    dump, grid, _ = S.ensure_snapshot(preset, snapdir)
+    cfg = configs.write_config(str(tmp_path / "ff.toml"), dump, grid)
+    run = capi.Run(cfg)
+    snap = capi.Snapshot(run, capi.TABLE_FP32 | capi.TABLE_FP64)
+    monkeypatch.setenv("PHX_FIELD_FREE_LEAVES", "0")
+    full = capi.Snapshot(run, capi.TABLE_FP32 | capi.TABLE_FP64)
+    monkeypatch.delenv("PHX_FIELD_FREE_LEAVES")
+    for sigma_cut in (1.0, 0.01):
+        rq = run.request(img_width=48, img_height=48, inclination=60.0, sigma_cut=sigma_cut)
+        for precision in (capi.FP32, capi.FP64, capi.MIXED):
+            a, ia = snap.render(rq, precision)
+            b, ib = full.render(rq, precision)
+            print(f"\n{preset} sigma_cut {sigma_cut} fp{precision}: kernel {ia['kernel_s'] * 1e3:.2f} vs "
+                  f"{ib['kernel_s'] * 1e3:.2f} ms, segments {ia['nplasma']}, dropped {ia['bad']}")
+            assert np.array_equal(a, b)
+            assert (ia["steps"], ia["nplasma"], ia["bad"]) == (ib["steps"], ib["nplasma"], ib["bad"])
+    snap.close()
+    full.close()
+
+
 def test_errors_are_reported(c1_run):
     snap = capi.Snapshot(c1_run, capi.TABLE_FP32)
     with pytest.raises(capi.PhxError) as e:
