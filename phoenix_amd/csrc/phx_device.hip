@@ -80,8 +80,8 @@ template <int M>
 struct MarchArgs {
   using T = typename ModeOf<M>::T;
   using L = typename ModeOf<M>::L;
-  Grid<T> G;                // FP32 / FP64: cell table and the lookup's block table and map
-  const SideArgs<M>* side;  // MIX: the grid and sample parameters in device memory (else G and P)
+  Grid<T> G;                // cell table and the lookup's block table and map (FP32 reads it here)
+  const SideArgs<M>* side;  // MIX, FP64: the grid and sample parameters in device memory (FP32: G and P)
   Plasma<T> P;              // a, hs and rt_outer are read by every march from here
   const void* start;  // per-job start records, in job order
   T stepsize, r_out, cut_in;
@@ -280,13 +280,13 @@ using StateOf = typename std::conditional<M == MIX, StHam, StRef<typename ModeOf
 template <int M>
 using RecOf = typename std::conditional<M == M32, float, double>::type;  // start record scalar
 
-// the grid and sample parameters.  The mixed march loads them (scalar loads) where a lookup miss or
-// a sample needs them: the empty asm makes the pointer a new value at each use, so the loads are not
-// hoisted out of the march loop.  The FP32 and FP64 marches read the launch arguments, which
-// measured faster there (FP32 162 vs 172 ms).
+// the grid and sample parameters.  The mixed and FP64 marches load them (scalar loads) where a
+// lookup miss or a sample needs them: the empty asm makes the pointer a new value at each use, so
+// the loads are not hoisted out of the march loop.  The FP32 march reads the launch arguments,
+// which measured faster there (162 vs 172 ms; FP64 349 -> 346 ms, mixed about -7%).
 template <int M>
 PHX_DEV SideArgs<M> side_of(const MarchArgs<M>& A) {
-  if constexpr (M != MIX) {
+  if constexpr (M == M32) {
     SideArgs<M> out;
     out.G = A.G;
     out.P = A.P;
