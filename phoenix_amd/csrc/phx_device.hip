@@ -575,6 +575,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
   T AA[NF][16], SS[NF][4], SC[NF][4], tau[NF], tauF[NF];
   int steps = 0, turns = 0, stop = 0, np = 0, bad = 0, pix = 0, task = 0, nf = 0;
   T thdot = T(0), dlp = T(0);
+  T r_prev = T(0);  // r of the previous pass's position, the pending sample's (read from the third pass on)
   bool bneg = false, bpos = false, have = false, stats = false;
   unsigned s_steps = 0, s_np = 0, s_bad = 0;  // per-lane sums (a lane marches well under 2^32 steps)
   unsigned s_fp64 = 0;                        // MIX: steps taken in FP64
@@ -638,7 +639,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
     const T r = exp_r(cx[1]);
     // ln r of the pending sample (the state before the last step; MIX keeps it in FP64 only)
     const T lrs = M == MIX ? T(pl[1]) : px[1];
-    const T rs = exp_r(lrs);  // radius of the pending sample
+    const T rs = r_prev;  // radius of the pending sample: the previous pass's r (exp_r of the same lrs)
     // escape: a ray moving outward at r >= rt_outer, whose pending sample lies there too, can
     // never emit again (a null geodesic in Kerr has at most one radial turning point outside the
     // photon orbits, so it does not return), and the rest of its march changes none of its
@@ -752,6 +753,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
       }
     }
     st.lpos(cx, pl);
+    r_prev = r;
     pdl = dlp;
     if (alive) {
       // ---- turning points of theta (pixel.hpp:193-197), grid-aware step and RK4 (:200-210)
