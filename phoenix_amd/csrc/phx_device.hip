@@ -678,10 +678,11 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
         int Lf = lcache.leaf;
         if (!cached) {
           const Grid<T> G = side_of(A).G;
-          Lf = find_leaf_fill(G, wx, lcache, &cached);
+          constexpr int DXC = M == MIX ? 9 : 6;  // the mixed march's step rule takes 1/dxc
+          Lf = find_leaf_fill<T, DXC>(G, wx, lcache, &cached);
           if (need_dxc && Lf >= 0 && !cached) {  // a leaf of the fallback scan (not cached)
 #pragma unroll
-            for (int d = 0; d < 3; d++) lcache.dxc[d] = G.blk[12 * Lf + 6 + d];
+            for (int d = 0; d < 3; d++) lcache.dxc[d] = G.blk[12 * Lf + DXC + d];
             lcache.leaf = -1;
           }
         }
@@ -760,7 +761,11 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
       if (thdot * ck[2] < T(0) && steps > 2) turns++;
       thdot = ck[2];
       if ((bneg && turns > A.max_order) || (bpos && turns > A.max_order + 1)) stop = 1;
-      const T dl = step_len(A.stepsize, st.x2p(), ck, have_dxc, dxc);
+      T dl;
+      if constexpr (M == MIX)
+        dl = step_len_mixed(A.stepsize, 1.f / A.stepsize, st.x2p(), ck, have_dxc, dxc);  // dxc: 1/cell size
+      else
+        dl = step_len(A.stepsize, st.x2p(), ck, have_dxc, dxc);
       if (st.step(A, dl) && stats) s_fp64++;
       dlp = fabs(dl);
       steps++;

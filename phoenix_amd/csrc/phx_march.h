@@ -560,6 +560,20 @@ PHX_DEV T step_len(T stepsize, T x2, const T k[4], bool have_dxc, const T dxc[3]
   }
   return -fmin(T(1) / (i1 + i2 + i3), grid);
 }
+// The same rule for the mixed march, with the stepsize's and the cell sizes' reciprocals
+// (inv_dxc) in place of the divisions by them: 1/|stepsize/k| = |k|/stepsize and 1/|dxc/k| =
+// |k|/dxc, exact in real arithmetic and equal to FP32 rounding (the guard against a zero x2 term
+// at the pole is kept as it is)
+PHX_DEV float step_len_mixed(float stepsize, float inv_step, float x2, const float k[4], bool have_dxc,
+                             const float inv_dxc[3]) {
+  const float S2 = Lim<float>::tiny;
+  const float ak1 = fabsf(k[1]) + S2, ak2 = fabsf(k[2]) + S2, ak3 = fabsf(k[3]) + S2;
+  const float d2 = stepsize * fminf(x2, float(PI_D) - x2) / ak2;
+  const float i1 = ak1 * inv_step, i2 = 1.f / (fabsf(d2) + S2), i3 = ak3 * inv_step;
+  float grid = Lim<float>::big;
+  if (have_dxc) grid = 0.25f / (ak1 * inv_dxc[0] + ak2 * inv_dxc[1] + ak3 * inv_dxc[2]);
+  return -fminf(1.f / (i1 + i2 + i3), grid);
+}
 
 // ------------------------------------------------------------------ AMR lookup
 // The snapshot's device tables: one 12-scalar record per leaf (lower corner, upper
@@ -694,7 +708,7 @@ PHX_DEV int find_leaf(const Grid<T>& G, const T x[4]) {
 template <class T>
 struct LeafCache {
   int leaf = -1;
-  T box[6], dxc[3];
+  T box[6], dxc[3];  // dxc: the cell size (the mixed march keeps its reciprocal)
 };
 // whether the lane's cached leaf holds x (half-open box with find_leaf's nudge)
 template <class T>
@@ -705,7 +719,8 @@ PHX_DEV bool leaf_cache_hit(const LeafCache<T>& c, const T x[4]) {
 }
 // find_leaf on a miss of the lane's cache: the map candidate, once validated, refills the cache
 // (*cached then tells that c.dxc holds the leaf's cell size); the fallbacks are find_leaf's
-template <class T>
+// DXC: the block-record slot the cache takes the cell size from (6: dxc; 9: 1/dxc, the mixed march)
+template <class T, int DXC = 6>
 PHX_DEV int find_leaf_fill(const Grid<T>& G, const T x[4], LeafCache<T>& c, bool* cached) {
   const T small = x[2] > T(PI_D / 2.) ? T(-1e-9) : T(1e-9);
   *cached = false;
@@ -730,7 +745,7 @@ PHX_DEV int find_leaf_fill(const Grid<T>& G, const T x[4], LeafCache<T>& c, bool
 #pragma unroll
       for (int q = 0; q < 6; q++) c.box[q] = box[q];
 #pragma unroll
-      for (int q = 0; q < 3; q++) c.dxc[q] = b[6 + q];
+      for (int q = 0; q < 3; q++) c.dxc[q] = b[DXC + q];
       *cached = true;
       return cand;
     }
