@@ -600,6 +600,11 @@ PHX_DEV double wrap_mod(double x, double y) { return fabs(x) < y ? x : fmod(x, y
 template <class T>
 PHX_DEV void wrap(T x[4]) {  // sampler.hpp:29-38
   const T TP = T(2. * PI_D), PI = T(PI_D);
+  // the common case, in which both fmods return their argument and no reflection follows
+  if (x[3] >= T(0) && x[3] < TP && x[2] >= T(1e-6) && x[2] < PI) {
+    x[2] = x[2] - T(1e-6);
+    return;
+  }
   x[3] = wrap_mod(x[3], TP);
   x[2] = wrap_mod(x[2], PI) - T(1e-6);
   if (x[3] < T(0)) x[3] = TP + x[3];
