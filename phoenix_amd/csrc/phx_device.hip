@@ -607,8 +607,11 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
       const int2 jb = S.jobs[n];
       task = jb.x;
       pix = jb.y;
-      nf = S.tasks[task].nf;
-      stats = S.tasks[task].stats != 0;
+      {
+        const TaskDev<T> tr = load_global(S.tasks + task);
+        nf = tr.nf;
+        stats = tr.stats != 0;
+      }
       T bsign;
       st.load(static_cast<const R*>(S.start) + (size_t)n * St::REC, &bsign);
 #pragma unroll
@@ -648,7 +651,6 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
     const bool escaped = A.escape && steps >= 2 && r >= A.P.rt_outer && rs >= A.P.rt_outer && cx[1] > lrs;
     const bool alive = r < A.r_out && r > A.cut_in && steps < A.max_steps && !stop && !escaped;
     if constexpr (M == MIX) {
-      st.begin(A, alive, ck);
       // the pending sample's FP32 position and polarization constant follow from its FP64
       // position and the ray's constant: not carried from pass to pass
 #pragma unroll
@@ -682,7 +684,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
           Lf = find_leaf_fill<T, DXC>(G, wx, lcache, &cached);
           if (need_dxc && Lf >= 0 && !cached) {  // a leaf of the fallback scan (not cached)
 #pragma unroll
-            for (int d = 0; d < 3; d++) lcache.dxc[d] = G.blk[12 * Lf + DXC + d];
+            for (int d = 0; d < 3; d++) lcache.dxc[d] = glb(G.blk)[12 * Lf + DXC + d];
             lcache.leaf = -1;
           }
         }
@@ -721,7 +723,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
       qcount += (int)__popcll(pm);
     } else if (emit) {
       // ---- the pending sample, in place
-      const TaskDev<T>& tk = side_of(A).tasks[task];
+      const TaskDev<T> tk = load_global(side_of(A).tasks + task);
       Prep<T> sp;
       const int status = sample_prepare<M>(A, tk, px, pl, pk, pf, pdl, leaf_next, sp);
       if (status) np++;
@@ -745,6 +747,10 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
       }
     }
 
+    // the coming step's precision and first stage, and the wavevector at the current point (not
+    // needed before: the pending sample carries its own); evaluated after the sample, so the stage
+    // is not held in registers across it
+    if constexpr (M == MIX) st.begin(A, alive, ck);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       pk[q] = ck[q];
@@ -809,7 +815,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
               for (int q = 0; q < 3; q++) xl[1 + q] = e64[q];
             }
             const int etask = (int)in[20], owner = (int)in[21];
-            const TaskDev<T>& tk = side_of(A).tasks[etask];
+            const TaskDev<T> tk = load_global(side_of(A).tasks + etask);
             // the candidate record overwrites the slot; everything of the queued sample is read first
             const T sdl = in[12];
             const int sleaf = (int)in[13];
@@ -858,7 +864,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
           if (mine) {
             ctask = (int)rec[22];
             status = sample_frame_stage<M>(A, rec, sp);
-            const TaskDev<T>& tk = side_of(A).tasks[ctask];
+            const TaskDev<T> tk = load_global(side_of(A).tasks + ctask);
             enf = tk.nf;
             if (status != 0 && tk.stats) s_np++;
             if (status == 2) s_bad += enf;
@@ -872,7 +878,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
               int flag = 0;  // 1: fold in; 0: nothing to fold (no emission, or dropped)
               if (status == 1) {
                 T OA[16], OS[4], dtau, dtauF;
-                if (sample_op(side_of(A).tasks[ctask], sp, f, OA, OS, &dtau, &dtauF)) {
+                if (sample_op(load_global(side_of(A).tasks + ctask), sp, f, OA, OS, &dtau, &dtauF)) {
 #pragma unroll
                   for (int q = 0; q < 16; q++) rec[q] = OA[q];
 #pragma unroll
@@ -942,20 +948,21 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
 
     if (!alive) {
       // ---- ray done: write it out and free the lane (pixel.hpp:213-223)
-      const TaskDev<T>& tkw = side_of(A).tasks[task];
+      const TaskDev<T> tkw = load_global(side_of(A).tasks + task);
       phx_freqout* o0 = tkw.out + (size_t)pix * tkw.nf_total;
 #pragma unroll
       for (int f = 0; f < NF; f++) {
         if (f >= nf) break;
-        double2* o = reinterpret_cast<double2*>(o0 + f);
+        using GD2 = __attribute__((address_space(1))) d2v*;  // global stores (not flat)
+        const GD2 o = (GD2)(o0 + f);
         // with the queue, np is counted where the samples are computed; a ray with no plasma
         // segment has folded nothing in, so its sums are the zeros written otherwise
         if (QUEUE || np > 0) {
-          o[0] = make_double2((double)SS[f][0], (double)SS[f][1]);
-          o[1] = make_double2((double)SS[f][2], (double)SS[f][3]);
-          o[2] = make_double2((double)tau[f], (double)tauF[f]);
+          o[0] = d2v{(double)SS[f][0], (double)SS[f][1]};
+          o[1] = d2v{(double)SS[f][2], (double)SS[f][3]};
+          o[2] = d2v{(double)tau[f], (double)tauF[f]};
         } else {
-          o[0] = o[1] = o[2] = make_double2(0., 0.);
+          o[0] = o[1] = o[2] = d2v{0., 0.};
         }
       }
       if (stats) {

@@ -16,6 +16,29 @@ namespace phxd {
 
 #define PHX_DEV __device__ __forceinline__
 
+// The global-memory view of a pointer.  The march reaches its tables through pointers that it
+// loads from memory, whose address space the compiler cannot infer: loads through them are flat
+// instructions, which also count against the scalar-load counter, so that every wait for a scalar
+// load waits for them as well.  Loads through this view are global loads.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+template <class T>
+PHX_DEV gptr<T> glb(const T* p) {
+  return (gptr<T>)p;
+}
+// a copy of a struct in global memory, loaded dword by dword through the global view
+typedef unsigned __attribute__((may_alias)) dword_alias;
+template <class S>
+PHX_DEV S load_global(const S* p) {
+  static_assert(sizeof(S) % 4 == 0, "whole dwords");
+  const gptr<unsigned> w = (gptr<unsigned>)p;
+  S out;
+  dword_alias* d = reinterpret_cast<dword_alias*>(&out);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(S) / 4); i++) d[i] = w[i];
+  return out;
+}
+
 // ------------------------------------------------------------------ constants
 // CGS constants of the reference (src/core/numerics/constants.hpp:22-81)
 constexpr double MP = 1.6726219e-24, ME = 9.1093829e-28, QE = 4.80320425e-10;
@@ -619,13 +642,13 @@ PHX_DEV bool off_domain(const Grid<T>& G, const T x[4]) {  // sampler.hpp:43-50
 }
 template <class T>
 PHX_DEV bool in_leaf(const Grid<T>& G, int q, const T x[4], T small) {  // half-open (sampler.hpp:87-101)
-  const T* b = G.blk + 12 * q;
+  const gptr<T> b = glb(G.blk) + 12 * q;
   return !(x[1] + small < b[0] || x[1] + small >= b[3] || x[2] + small < b[1] || x[2] + small >= b[4] ||
            x[3] + small < b[2] || x[3] + small >= b[5]);
 }
 template <class T>
 PHX_DEV bool in_leaf_closed(const Grid<T>& G, int q, const T x[4]) {  // inclusive (sampler.hpp:109-122)
-  const T* b = G.blk + 12 * q;
+  const gptr<T> b = glb(G.blk) + 12 * q;
   return !(x[1] < b[0] || x[1] > b[3] || x[2] < b[1] || x[2] > b[4] || x[3] < b[2] || x[3] > b[5]);
 }
 // The table a sample's cell is located in: the block records and the domain bounds, in the
@@ -645,7 +668,7 @@ PHX_DEV bool off_domain(const Loc<L>& G, const L x[4]) {  // sampler.hpp:43-50
 }
 template <class L>
 PHX_DEV bool in_leaf_closed(const Loc<L>& G, int q, const L x[4]) {  // inclusive (sampler.hpp:109-122)
-  const L* b = G.blk + 12 * q;
+  const gptr<L> b = glb(G.blk) + 12 * q;
   return !(x[1] < b[0] || x[1] > b[3] || x[2] < b[1] || x[2] > b[4] || x[3] < b[2] || x[3] > b[5]);
 }
 // find_leaf (sampler.hpp:143-180) on an already wrapped position
@@ -666,11 +689,11 @@ PHX_DEV int find_leaf_located(const Grid<float>& G, const Loc<L>& GL, const L x[
     const float fk = float(x[3] + small - GL.lo3) / G.w2 * G.up;
     if (!(fi < 0.f || fj < 0.f || fk < 0.f)) {
       const int i = (int)fi, j = (int)fj, k = (int)fk;
-      if (i < G.fd0 && j < G.fd1 && k < G.fd2) cand = G.map[(i * G.fd1 + j) * G.fd2 + k];
+      if (i < G.fd0 && j < G.fd1 && k < G.fd2) cand = glb(G.map)[(i * G.fd1 + j) * G.fd2 + k];
     }
   }
   auto inside = [&](int q) {
-    const L* b = GL.blk + 12 * q;
+    const gptr<L> b = glb(GL.blk) + 12 * q;
     return !(x[1] + small < b[0] || x[1] + small >= b[3] || x[2] + small < b[1] || x[2] + small >= b[4] ||
              x[3] + small < b[2] || x[3] + small >= b[5]);
   };
@@ -693,7 +716,7 @@ PHX_DEV int find_leaf(const Grid<T>& G, const T x[4]) {
     const T fk = (x[3] + small - G.lo3) / G.w2 * G.up;
     if (!(fi < T(0) || fj < T(0) || fk < T(0))) {
       const int i = (int)fi, j = (int)fj, k = (int)fk;
-      if (i < G.fd0 && j < G.fd1 && k < G.fd2) cand = G.map[((size_t)i * G.fd1 + j) * G.fd2 + k];
+      if (i < G.fd0 && j < G.fd1 && k < G.fd2) cand = glb(G.map)[((size_t)i * G.fd1 + j) * G.fd2 + k];
     }
   }
   if (cand >= 0 && in_leaf(G, cand, x, small)) return cand;
@@ -736,11 +759,11 @@ PHX_DEV int find_leaf_fill(const Grid<T>& G, const T x[4], LeafCache<T>& c, bool
     const T fk = (x[3] + small - G.lo3) / G.w2 * G.up;
     if (!(fi < T(0) || fj < T(0) || fk < T(0))) {
       const int i = (int)fi, j = (int)fj, k = (int)fk;
-      if (i < G.fd0 && j < G.fd1 && k < G.fd2) cand = G.map[(i * G.fd1 + j) * G.fd2 + k];
+      if (i < G.fd0 && j < G.fd1 && k < G.fd2) cand = glb(G.map)[(i * G.fd1 + j) * G.fd2 + k];
     }
   }
   if (cand >= 0) {
-    const T* b = G.blk + 12 * cand;
+    const gptr<T> b = glb(G.blk) + 12 * cand;
     T box[6];
 #pragma unroll
     for (int q = 0; q < 6; q++) box[q] = b[q];
@@ -775,23 +798,28 @@ template <class T>
 struct Plasma {  // plasma parameters shared by every image of a launch (src/core/model/bhac.hpp:150-151)
   T a, hs, gam, rlow, ibc, sigma_cut, te_min, te_max, rt_outer;
 };
-PHX_DEV void load8(const float* p, float v[8]) {
-  const float4 a = *reinterpret_cast<const float4*>(p), b = *reinterpret_cast<const float4*>(p + 4);
+// clang vector types: loaded through a global pointer as one wide load each
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+PHX_DEV void load8(gptr<float> p, float v[8]) {
+  const f4v a = *reinterpret_cast<gptr<f4v>>(p), b = *reinterpret_cast<gptr<f4v>>(p + 4);
   v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
   v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
 }
-PHX_DEV void load2(const float* p, float v[2]) {
-  const float2 a = *reinterpret_cast<const float2*>(p);
+PHX_DEV void load2(gptr<float> p, float v[2]) {
+  const f2v a = *reinterpret_cast<gptr<f2v>>(p);
   v[0] = a.x; v[1] = a.y;
 }
-PHX_DEV void load2(const double* p, double v[2]) {
-  const double2 a = *reinterpret_cast<const double2*>(p);
+PHX_DEV void load2(gptr<double> p, double v[2]) {
+  const d2v a = *reinterpret_cast<gptr<d2v>>(p);
   v[0] = a.x; v[1] = a.y;
 }
-PHX_DEV void load8(const double* p, double v[8]) {
-  const double4 a = *reinterpret_cast<const double4*>(p), b = *reinterpret_cast<const double4*>(p + 4);
-  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+PHX_DEV void load8(gptr<double> p, double v[8]) {
+  const d2v a = *reinterpret_cast<gptr<d2v>>(p), b = *reinterpret_cast<gptr<d2v>>(p + 2);
+  const d2v c = *reinterpret_cast<gptr<d2v>>(p + 4), d = *reinterpret_cast<gptr<d2v>>(p + 6);
+  v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+  v[4] = c.x; v[5] = c.y; v[6] = d.x; v[7] = d.y;
 }
 // get_fluid_params + derive_fluid (src/core/model/sampler.hpp:305-427): wrap, cached-leaf test,
 // re-wrap and lookup on a miss, cell by truncation, weights from the cell's lower corner,
@@ -824,13 +852,13 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
   // the sigma cut (sigma = (0 + 1e-6) / (rho + 1e-20) > sigma_cut; the interpolated rho is a convex
   // combination of the corners, so it lies in [0, max]; the margin covers its rounding), every
   // sample in it is cut: nothing emits and nothing is counted, as the full derivation would find.
-  const T rho_ff = G.meta[q];
+  const T rho_ff = glb(G.meta)[q];
   const bool ffleaf = rho_ff >= T(0);
   if (ffleaf && T(1e-6) > P.sigma_cut * (rho_ff + T(1e-20)) * T(1.00001)) {
     fl.ne = T(0);
     return 0;
   }
-  const L* b = GL.blk + 12 * q;
+  const gptr<L> b = glb(GL.blk) + 12 * q;
   const int nxs[3] = {G.nx0, G.nx1, G.nx2};
   int c[3];
   T del[3];
@@ -856,7 +884,7 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
   const T w1 = T(1) - del[0], w2 = T(1) - del[1], w3 = T(1) - del[2];
   const int ci[4] = {c[0], c[0], ip, ip}, cj[4] = {c[1], jp, c[1], jp};
   const T wq[4] = {w1 * w2, w1 * del[1], del[0] * w2, del[0] * del[1]};
-  const T* base = G.prim + (size_t)q * G.cells * 8;
+  const gptr<T> base = glb(G.prim) + (size_t)q * G.cells * 8;
   T qv[8];
 #pragma unroll
   for (int a = 0; a < 8; a++) qv[a] = T(0);
