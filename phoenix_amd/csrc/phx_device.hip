@@ -261,7 +261,7 @@ struct StHam {
         e1 = h * d[0];
         e2 = h * d[1];
       }
-      const float dt6 = dl / 6.f;
+      const float dt6 = div_const(dl, 6.);
       float inc[5];
 #pragma unroll
       for (int i = 0; i < 5; i++) inc[i] = sum[i] * dt6;

@@ -60,6 +60,22 @@ template <> struct Lim<double> {  // precision.hpp:53-58
   static constexpr bool fp32 = false;
 };
 
+// Divisions the optimizer rewrites (a negation folded into the divisor) or speculates out of a
+// branch lose their fast-math marking, and an FP32 division then expands to the correctly rounded
+// sequence (a dozen instructions and two mode switches).  These helpers state the FP32 form
+// directly; FP64 divides.
+// x / c for a constant c: FP32 multiplies by the rounded reciprocal (within an ulp of the quotient)
+template <class T>
+PHX_DEV T div_const(T x, double c) {
+  if constexpr (Lim<T>::fp32)
+    return x * float(1. / c);
+  else
+    return x / T(c);
+}
+// a / b: FP32 by the hardware reciprocal (1 ulp) and a multiply
+PHX_DEV float div_rcp(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+PHX_DEV double div_rcp(double a, double b) { return a / b; }
+
 PHX_DEV void scs(float x, float* s, float* c) { sincosf(x, s, c); }
 PHX_DEV void scs(double x, double* s, double* c) { sincos(x, s, c); }
 
@@ -407,7 +423,7 @@ PHX_DEV void rk4(T a, T hs, Ray<T>& g, T dt) {
 #pragma unroll
     for (int i = 0; i < 12; i++) inc[i] = wdt * d[i];
   }
-  const T dt6 = dt / T(6);
+  const T dt6 = div_const(dt, 6.);
 #pragma unroll
   for (int i = 0; i < 12; i++) {
     const T incr = sum[i] * dt6;
@@ -1017,7 +1033,7 @@ PHX_DEV T pitch_angle(const Met<T>& g, const T k[4], const T b[4], const T U[4])
   if (bn == T(0)) return T(PI_D / 2.);
   const T kk = fabs(dot(g, k, U));
   T mu = dot(g, k, b) / (kk * bn);
-  if (fabs(mu) > T(1)) mu = mu / fabs(mu);
+  if (fabs(mu) > T(1)) mu = copysign(T(1), mu);  // mu / |mu|
   return acos(mu);
 }
 
@@ -1104,8 +1120,8 @@ PHX_DEV void bessel_ratios(T x, T* r10, T* r02) {  // K1/K2 and K0/K2 (bessel.hp
 template <class T>
 PHX_DEV T fm_conv(T X) {  // f_m(X) (coeffs.hpp:63-70)
   const T lx = log(X);
-  const T ex = exp(-X / T(47.2));
-  return T(2.011) * exp(-exp(T(1.035) * lx) / T(4.7)) - cos(X * T(0.5)) * exp(-exp(T(1.2) * lx) / T(2.73)) -
+  const T ex = exp(-div_const(X, 47.2));
+  return T(2.011) * exp(-div_const(exp(T(1.035) * lx), 4.7)) - cos(X * T(0.5)) * exp(-div_const(exp(T(1.2) * lx), 2.73)) -
          T(0.011) * ex +
          (T(0.011) * ex - T(369.69342869090815) * exp(T(-8. / 3.) * lx)) * T(0.5) *
              (T(1) + tanh(T(10) * (lx - T(4.787491742782046))));
@@ -1115,14 +1131,14 @@ template <class T>
 PHX_DEV Coef<T> thermal(T te, T ne, T nu, T B, T sth, T cth, T kr1, T kr0) {
   Coef<T> r;
   r.jU = r.aU = r.rU = T(0);
-  const T ite = T(1) / te;
+  const T ite = div_rcp(T(1), te);
   const T nuc = T(NU_CRIT) * B * sth * te * te;
   const T x = nu / nuc;
   const T pref = T(JNU) * ne * nu * ite * ite;
   const T cx = cbrt(x), ic = T(1) / cx, ex = exp(T(-1.8899) * cx);
   r.jI = pref * (T(2.5651) * (T(1) + T(1.92) * ic + T(0.9977) * ic * ic) * ex);
   r.jQ = pref * (T(2.5651) * (T(1) + T(0.93193) * ic + T(0.499873) * ic * ic) * ex);
-  r.jV = T(4. / 3.) * pref * ite * (cth / sth) *
+  r.jV = T(4. / 3.) * pref * ite * div_rcp(cth, sth) *
          ((T(1.81348) * ic * ic * ic + T(3.42319) * ic * ic + T(0.0292545) / sqrt(x) + T(2.03773) * ic) * ex);
   const T pre = T(2. * HP) * nu * nu * nu / T(CL * CL);
   const T xx = T(H_OVER_MEC2) * nu / te;
@@ -1147,9 +1163,9 @@ template <class T>
 PHX_DEV void kratios(T te, T* kr1, T* kr0) {
   const T ite = T(1) / te;
   if (ite < T(1. / 5.)) {
-    const T k0 = -log(ite / T(2)) - T(0.5772), k1 = T(1) / ite, k2 = T(2) / ite / ite;
-    *kr1 = k1 / k2;
-    *kr0 = (k2 > T(0)) ? k0 / k2 : T(1);
+    const T k0 = -log(ite / T(2)) - T(0.5772), k1 = div_rcp(T(1), ite), k2 = div_rcp(div_rcp(T(2), ite), ite);
+    *kr1 = div_rcp(k1, k2);
+    *kr0 = (k2 > T(0)) ? div_rcp(k0, k2) : T(1);
   } else {
     bessel_ratios(ite, kr1, kr0);
   }
@@ -1169,9 +1185,9 @@ PHX_DEV Coef<T> invariant_rotated(Coef<T> c, T nup, T jscale, T c2, T s2) {
   c.rV = c.rV * nup;
   const T pf = sqrt(c.jQ * c.jQ + c.jV * c.jV) / c.jI;
   if (pf > T(1)) {
-    c.jQ = c.jQ / (pf + T(0.005));
-    c.jU = c.jU / (pf + T(0.005));
-    c.jV = c.jV / (pf + T(0.005));
+    c.jQ = div_rcp(c.jQ, pf + T(0.005));
+    c.jU = div_rcp(c.jU, pf + T(0.005));
+    c.jV = div_rcp(c.jV, pf + T(0.005));
   }
   Coef<T> r = c;
   r.jQ = c2 * c.jQ + s2 * c.jU;
