@@ -868,24 +868,31 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
   // the sigma cut (sigma = (0 + 1e-6) / (rho + 1e-20) > sigma_cut; the interpolated rho is a convex
   // combination of the corners, so it lies in [0, max]; the margin covers its rounding), every
   // sample in it is cut: nothing emits and nothing is counted, as the full derivation would find.
+  // the leaf's record is loaded together with its field-free mark (one memory round trip)
+  constexpr bool MIXED = !std::is_same<T, L>::value;
+  const gptr<L> b = glb(GL.blk) + 12 * q;
+  L blo[3], bdx[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    blo[d] = b[d];
+    bdx[d] = MIXED ? b[9 + d] : b[6 + d];  // the mixed march: 1/dx
+  }
   const T rho_ff = glb(G.meta)[q];
   const bool ffleaf = rho_ff >= T(0);
   if (ffleaf && T(1e-6) > P.sigma_cut * (rho_ff + T(1e-20)) * T(1.00001)) {
     fl.ne = T(0);
     return 0;
   }
-  const gptr<L> b = glb(GL.blk) + 12 * q;
   const int nxs[3] = {G.nx0, G.nx1, G.nx2};
   int c[3];
   T del[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    const L lb = b[d], dx = b[6 + d];
+    const L lb = blo[d], dx = bdx[d];
     // the mixed march (FP64 location, FP32 fluid) multiplies by the reciprocal cell width: a
     // last-ulp change of u only moves a sample across a cell face, where the block-local trilinear
     // weights are continuous; FP32 and FP64 divide as the reference does
-    constexpr bool MIXED = !std::is_same<T, L>::value;
-    const L u = MIXED ? (x[1 + d] - lb) * b[9 + d] : (x[1 + d] - lb) / dx;
+    const L u = MIXED ? (x[1 + d] - lb) * dx : (x[1 + d] - lb) / dx;
     c[d] = (int)u;
     if (c[d] >= nxs[d]) c[d] = nxs[d] - 1;
     const int ix = F32 ? (int)floor(u) : (int)(u + L(1000)) - 1000;
@@ -904,34 +911,77 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
   T qv[8];
 #pragma unroll
   for (int a = 0; a < 8; a++) qv[a] = T(0);
-  if (ffleaf) {
+  // FP32 cells: the corner loads of a layer (4 corners; the field-free gather: all 8) are issued
+  // before the first is used: the scheduling barriers keep the compiler from interleaving them with
+  // the sums, which under register pressure waits for each corner in turn (8 memory round trips a
+  // sample).  FP64 cells (the FP64 march) keep the interleaved form.
+  if constexpr (!Lim<T>::fp32) {
+    if (ffleaf) {
+#pragma unroll
+      for (int z = 0; z < 2; z++) {
+        const int ck = z ? kp : c[2];
+        const T wz = z ? del[2] : w3;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+          T v[2];
+          load2(base + (size_t)(ci[s] + cj[s] * G.nx0 + ck * G.nx0 * G.nx1) * 8, v);
+          const T w = wq[s] * wz;
+#pragma unroll
+          for (int a = 0; a < 2; a++) qv[a] += v[a] * w;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int z = 0; z < 2; z++) {
+        const int ck = z ? kp : c[2];
+        const T wz = z ? del[2] : w3;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+          T v[8];
+          load8(base + (size_t)(ci[s] + cj[s] * G.nx0 + ck * G.nx0 * G.nx1) * 8, v);
+          const T w = wq[s] * wz;
+#pragma unroll
+          for (int a = 0; a < 8; a++) qv[a] += v[a] * w;
+        }
+      }
+    }
+  } else if (ffleaf) {
     // a field-free leaf: only rho and u of the 8 corners, interpolated as below
+    T v[8][2];
 #pragma unroll
     for (int z = 0; z < 2; z++) {
       const int ck = z ? kp : c[2];
+#pragma unroll
+      for (int s = 0; s < 4; s++) load2(base + (size_t)(ci[s] + cj[s] * G.nx0 + ck * G.nx0 * G.nx1) * 8, v[4 * z + s]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int z = 0; z < 2; z++) {
       const T wz = z ? del[2] : w3;
 #pragma unroll
       for (int s = 0; s < 4; s++) {
-        T v[2];
-        load2(base + (size_t)(ci[s] + cj[s] * G.nx0 + ck * G.nx0 * G.nx1) * 8, v);
         const T w = wq[s] * wz;
 #pragma unroll
-        for (int a = 0; a < 2; a++) qv[a] += v[a] * w;
+        for (int a = 0; a < 2; a++) qv[a] += v[4 * z + s][a] * w;
       }
     }
   } else {
 #pragma unroll
     for (int z = 0; z < 2; z++) {
       const int ck = z ? kp : c[2];
+      T v8[4][8];
+#pragma unroll
+      for (int s = 0; s < 4; s++) load8(base + (size_t)(ci[s] + cj[s] * G.nx0 + ck * G.nx0 * G.nx1) * 8, v8[s]);
+      __builtin_amdgcn_sched_barrier(0);
       const T wz = z ? del[2] : w3;
 #pragma unroll
       for (int s = 0; s < 4; s++) {
-        T v[8];
-        load8(base + (size_t)(ci[s] + cj[s] * G.nx0 + ck * G.nx0 * G.nx1) * 8, v);
+        const T* v = v8[s];
         const T w = wq[s] * wz;
 #pragma unroll
         for (int a = 0; a < 8; a++) qv[a] += v[a] * w;
       }
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
   // derive_fluid
