@@ -656,16 +656,49 @@ template <class T>
 PHX_DEV bool off_domain(const Grid<T>& G, const T x[4]) {  // sampler.hpp:43-50
   return x[1] > G.hi1 || x[1] < G.lo1 || x[2] < G.lo2 || x[2] > G.hi2 || x[3] < G.lo3 || x[3] > G.hi3;
 }
+// The box tests of the FP32-cell marches (FP32, mixed) load the six bounds first and combine the
+// comparisons without short circuit: a short-circuit chain makes each load conditional on the
+// previous comparison, and the compiler then waits for each bound in turn (six memory round trips
+// instead of one).  The FP64 march keeps the short-circuit form, with which its build reproduces
+// the reference's FP64 results (the batched form changed them there; see the design notes).
+// the leaf's box lo[3], hi[3]
+template <class T>
+PHX_DEV void leaf_box(gptr<T> blk, int q, T bx[6]) {
+  const gptr<T> b = blk + 12 * q;
+#pragma unroll
+  for (int i = 0; i < 6; i++) bx[i] = b[i];
+}
+template <class T>
+PHX_DEV bool in_box_half_open(const T bx[6], const T x[4], T small) {
+  return !((x[1] + small < bx[0]) | (x[1] + small >= bx[3]) | (x[2] + small < bx[1]) | (x[2] + small >= bx[4]) |
+           (x[3] + small < bx[2]) | (x[3] + small >= bx[5]));
+}
+template <class T>
+PHX_DEV bool in_box_closed(const T bx[6], const T x[4]) {
+  return !((x[1] < bx[0]) | (x[1] > bx[3]) | (x[2] < bx[1]) | (x[2] > bx[4]) | (x[3] < bx[2]) | (x[3] > bx[5]));
+}
 template <class T>
 PHX_DEV bool in_leaf(const Grid<T>& G, int q, const T x[4], T small) {  // half-open (sampler.hpp:87-101)
-  const gptr<T> b = glb(G.blk) + 12 * q;
-  return !(x[1] + small < b[0] || x[1] + small >= b[3] || x[2] + small < b[1] || x[2] + small >= b[4] ||
-           x[3] + small < b[2] || x[3] + small >= b[5]);
+  if constexpr (Lim<T>::fp32) {
+    T bx[6];
+    leaf_box(glb(G.blk), q, bx);
+    return in_box_half_open(bx, x, small);
+  } else {
+    const gptr<T> b = glb(G.blk) + 12 * q;
+    return !(x[1] + small < b[0] || x[1] + small >= b[3] || x[2] + small < b[1] || x[2] + small >= b[4] ||
+             x[3] + small < b[2] || x[3] + small >= b[5]);
+  }
 }
 template <class T>
 PHX_DEV bool in_leaf_closed(const Grid<T>& G, int q, const T x[4]) {  // inclusive (sampler.hpp:109-122)
-  const gptr<T> b = glb(G.blk) + 12 * q;
-  return !(x[1] < b[0] || x[1] > b[3] || x[2] < b[1] || x[2] > b[4] || x[3] < b[2] || x[3] > b[5]);
+  if constexpr (Lim<T>::fp32) {
+    T bx[6];
+    leaf_box(glb(G.blk), q, bx);
+    return in_box_closed(bx, x);
+  } else {
+    const gptr<T> b = glb(G.blk) + 12 * q;
+    return !(x[1] < b[0] || x[1] > b[3] || x[2] < b[1] || x[2] > b[4] || x[3] < b[2] || x[3] > b[5]);
+  }
 }
 // The table a sample's cell is located in: the block records and the domain bounds, in the
 // locating precision L (FP64 in the mixed-precision march, else the grid's own table)
@@ -682,10 +715,17 @@ template <class L>
 PHX_DEV bool off_domain(const Loc<L>& G, const L x[4]) {  // sampler.hpp:43-50
   return x[1] > G.hi1 || x[1] < G.lo1 || x[2] < G.lo2 || x[2] > G.hi2 || x[3] < G.lo3 || x[3] > G.hi3;
 }
-template <class L>
+// BATCH: the batched form (the marches with FP32 cells, whose locating table may be FP64)
+template <bool BATCH, class L>
 PHX_DEV bool in_leaf_closed(const Loc<L>& G, int q, const L x[4]) {  // inclusive (sampler.hpp:109-122)
-  const gptr<L> b = glb(G.blk) + 12 * q;
-  return !(x[1] < b[0] || x[1] > b[3] || x[2] < b[1] || x[2] > b[4] || x[3] < b[2] || x[3] > b[5]);
+  if constexpr (BATCH) {
+    L bx[6];
+    leaf_box(glb(G.blk), q, bx);
+    return in_box_closed(bx, x);
+  } else {
+    const gptr<L> b = glb(G.blk) + 12 * q;
+    return !(x[1] < b[0] || x[1] > b[3] || x[2] < b[1] || x[2] > b[4] || x[3] < b[2] || x[3] > b[5]);
+  }
 }
 // find_leaf (sampler.hpp:143-180) on an already wrapped position
 template <class T>
@@ -709,9 +749,9 @@ PHX_DEV int find_leaf_located(const Grid<float>& G, const Loc<L>& GL, const L x[
     }
   }
   auto inside = [&](int q) {
-    const gptr<L> b = glb(GL.blk) + 12 * q;
-    return !(x[1] + small < b[0] || x[1] + small >= b[3] || x[2] + small < b[1] || x[2] + small >= b[4] ||
-             x[3] + small < b[2] || x[3] + small >= b[5]);
+    L bx[6];
+    leaf_box(glb(GL.blk), q, bx);
+    return in_box_half_open(bx, x, small);
   };
   if (cand >= 0 && inside(cand)) return cand;
   const L eps = L(1e-6);
@@ -783,8 +823,10 @@ PHX_DEV int find_leaf_fill(const Grid<T>& G, const T x[4], LeafCache<T>& c, bool
     T box[6];
 #pragma unroll
     for (int q = 0; q < 6; q++) box[q] = b[q];
-    if (!(x[1] + small < box[0] || x[1] + small >= box[3] || x[2] + small < box[1] || x[2] + small >= box[4] ||
-          x[3] + small < box[2] || x[3] + small >= box[5])) {
+    const bool inside = Lim<T>::fp32 ? in_box_half_open(box, x, small)
+                                     : !(x[1] + small < box[0] || x[1] + small >= box[3] || x[2] + small < box[1] ||
+                                         x[2] + small >= box[4] || x[3] + small < box[2] || x[3] + small >= box[5]);
+    if (inside) {
       c.leaf = cand;
 #pragma unroll
       for (int q = 0; q < 6; q++) c.box[q] = box[q];
@@ -853,7 +895,7 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
   const bool inside_one = std::is_same<T, L>::value ? (exp(x[1]) < L(1)) : (x[1] < L(0));
   if (inside_one || off_domain(GL, x)) return 0;
   int q = cache;
-  if (q == -1 || !in_leaf_closed(GL, q, x)) {
+  if (q == -1 || !in_leaf_closed<Lim<T>::fp32>(GL, q, x)) {
     wrap(x);
     if constexpr (std::is_same<T, L>::value)
       q = find_leaf(G, x);
@@ -1156,13 +1198,40 @@ PHX_DEV void bessel_ratios(T x, T* r10, T* r02) {  // K1/K2 and K0/K2 (bessel.hp
   const T la = bes_ln<T>(p, 0), lb = bes_ln<T>(p, 1);
   const T z = (T(2) * log(x) - (lb + la)) / (lb - la);
   T b1a = T(0), b2a = T(0), b1b = T(0), b2b = T(0);
-  for (int j = PHX_BES_NTERM[p] - 1; j >= 1; j--) {  // Clenshaw, both fits at once
-    const T ta = T(2) * z * b1a - b2a + bes_c<T>(0, p, j);
-    b2a = b1a;
-    b1a = ta;
-    const T tb = T(2) * z * b1b - b2b + bes_c<T>(1, p, j);
-    b2b = b1b;
-    b1b = tb;
+  if constexpr (Lim<T>::fp32) {
+    // FP32: all 24 table entries of the piece (zero past the fit's length, where the recurrence
+    // stays exactly 0 from its zero start, so the result is the same), in 3 batches of 8 per fit
+    // whose loads are issued together (a per-term load would wait for each coefficient in turn)
+#pragma unroll
+    for (int j0 = 23; j0 >= 1; j0 -= 8) {
+      T ca[8], cb[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        ca[i] = j0 - i >= 0 ? bes_c<T>(0, p, j0 - i) : T(0);
+        cb[i] = j0 - i >= 0 ? bes_c<T>(1, p, j0 - i) : T(0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        if (j0 - i < 1) break;
+        const T ta = T(2) * z * b1a - b2a + ca[i];
+        b2a = b1a;
+        b1a = ta;
+        const T tb = T(2) * z * b1b - b2b + cb[i];
+        b2b = b1b;
+        b1b = tb;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+    for (int j = PHX_BES_NTERM[p] - 1; j >= 1; j--) {  // Clenshaw, both fits at once
+      const T ta = T(2) * z * b1a - b2a + bes_c<T>(0, p, j);
+      b2a = b1a;
+      b1a = ta;
+      const T tb = T(2) * z * b1b - b2b + bes_c<T>(1, p, j);
+      b2b = b1b;
+      b1b = tb;
+    }
   }
   *r10 = bes_c<T>(0, p, 0) + z * b1a - b2a;
   *r02 = bes_c<T>(1, p, 0) + z * b1b - b2b;
