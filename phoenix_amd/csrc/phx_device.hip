@@ -1301,6 +1301,9 @@ static void upload_blocks(phx_snapshot* s, const phx_model* m, Tables<T>& tb) {
 template <class T>
 static void upload_prim(phx_snapshot* s, const phx_model* m, Tables<T>& tb) {
   const size_t nprim = (size_t)m->nleafs * m->cells * 8;
+  // the FP32 marches address the FP32 cell table by 32-bit byte offsets
+  if (sizeof(T) == 4 && nprim * sizeof(T) >= (size_t(1) << 32))
+    throw Error(PHX_E_ARG, "the FP32 cell table of this snapshot exceeds 4 GiB");
   tb.prim = dalloc<T>(s, nprim * sizeof(T));
   if (m->prim) {  // host primitives given: convert and upload; otherwise the dump is decoded on the device
     std::vector<T> p(nprim);

@@ -26,6 +26,14 @@ template <class T>
 PHX_DEV gptr<T> glb(const T* p) {
   return (gptr<T>)p;
 }
+// Element i of a table at a uniform base, by a 32-bit byte offset: the load then takes the base
+// from scalar registers and the offset from one vector register, with no 64-bit address arithmetic
+// per lane.  The table must be smaller than 4 GiB (the snapshot loader checks the cell table).
+template <class T>
+PHX_DEV gptr<T> at32(gptr<T> base, unsigned i) {
+  using C = const __attribute__((address_space(1))) char*;
+  return (gptr<T>)((C)base + i * (unsigned)sizeof(T));
+}
 // a copy of a struct in global memory, loaded dword by dword through the global view
 typedef unsigned __attribute__((may_alias)) dword_alias;
 template <class S>
@@ -889,18 +897,43 @@ template <class T, class L>
 PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P, T Ne, T Bu, T rhigh, const L X[4],
                          int cache, Fluid<T>& fl) {
   constexpr bool F32 = Lim<L>::fp32;
+  constexpr bool MIXED = !std::is_same<T, L>::value;
+  // FP32 cells (the FP32 and mixed marches): the hinted leaf's whole record (box, cell size and
+  // its reciprocal) and its field-free mark are loaded in one memory round trip, before the domain
+  // tests; they serve the box test and the cell location (a failed test reloads them for the leaf
+  // the lookup finds).  The FP64 march loads the box for the test and the rest after it.
+  constexpr bool BATCH = Lim<T>::fp32;
   L x[4] = {X[0], X[1], X[2], X[3]};
   wrap(x);
+  int q = cache;
+  L rec[12];
+  T rho_ff = T(-1);
+  auto load_record = [&](int leaf) {
+    const gptr<L> b = glb(GL.blk) + 12 * leaf;
+#pragma unroll
+    for (int i = 0; i < 12; i++) rec[i] = b[i];
+    rho_ff = glb(G.meta)[leaf];
+  };
+  if constexpr (BATCH) {
+    if (q != -1) load_record(q);
+  }
   // r < 1: exp(x1) < 1 (the FP64 locate of the mixed march tests x1 < 0, the same cut)
   const bool inside_one = std::is_same<T, L>::value ? (exp(x[1]) < L(1)) : (x[1] < L(0));
   if (inside_one || off_domain(GL, x)) return 0;
-  int q = cache;
-  if (q == -1 || !in_leaf_closed<Lim<T>::fp32>(GL, q, x)) {
+  bool hint_ok;
+  if constexpr (BATCH)
+    hint_ok = q != -1 && in_box_closed(rec, x);
+  else
+    hint_ok = q != -1 && in_leaf_closed<false>(GL, q, x);
+  if (!hint_ok) {
     wrap(x);
     if constexpr (std::is_same<T, L>::value)
       q = find_leaf(G, x);
     else
       q = find_leaf_located(G, GL, x);
+    if constexpr (BATCH) {
+      if (q != -1) load_record(q);
+    }
   }
 #pragma unroll
   for (int i = 0; i < 4; i++) fl.x[i] = T(x[i]);
@@ -910,16 +943,13 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
   // the sigma cut (sigma = (0 + 1e-6) / (rho + 1e-20) > sigma_cut; the interpolated rho is a convex
   // combination of the corners, so it lies in [0, max]; the margin covers its rounding), every
   // sample in it is cut: nothing emits and nothing is counted, as the full derivation would find.
-  // the leaf's record is loaded together with its field-free mark (one memory round trip)
-  constexpr bool MIXED = !std::is_same<T, L>::value;
-  const gptr<L> b = glb(GL.blk) + 12 * q;
+  if constexpr (!BATCH) load_record(q);  // the FP64 march: the record and the mark after the test
   L blo[3], bdx[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    blo[d] = b[d];
-    bdx[d] = MIXED ? b[9 + d] : b[6 + d];  // the mixed march: 1/dx
+    blo[d] = rec[d];
+    bdx[d] = MIXED ? rec[9 + d] : rec[6 + d];  // the mixed march: 1/dx
   }
-  const T rho_ff = glb(G.meta)[q];
   const bool ffleaf = rho_ff >= T(0);
   if (ffleaf && T(1e-6) > P.sigma_cut * (rho_ff + T(1e-20)) * T(1.00001)) {
     fl.ne = T(0);
@@ -950,6 +980,8 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
   const int ci[4] = {c[0], c[0], ip, ip}, cj[4] = {c[1], jp, c[1], jp};
   const T wq[4] = {w1 * w2, w1 * del[1], del[0] * w2, del[0] * del[1]};
   const gptr<T> base = glb(G.prim) + (size_t)q * G.cells * 8;
+  const gptr<T> prim = glb(G.prim);               // FP32 cells: 32-bit offsets from the table base
+  const unsigned qc = (unsigned)q * (unsigned)G.cells;  // the leaf's first cell
   T qv[8];
 #pragma unroll
   for (int a = 0; a < 8; a++) qv[a] = T(0);
@@ -994,7 +1026,7 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
     for (int z = 0; z < 2; z++) {
       const int ck = z ? kp : c[2];
 #pragma unroll
-      for (int s = 0; s < 4; s++) load2(base + (size_t)(ci[s] + cj[s] * G.nx0 + ck * G.nx0 * G.nx1) * 8, v[4 * z + s]);
+      for (int s = 0; s < 4; s++) load2(at32(prim, (qc + (unsigned)(ci[s] + cj[s] * G.nx0 + ck * G.nx0 * G.nx1)) * 8u), v[4 * z + s]);
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -1013,7 +1045,7 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
       const int ck = z ? kp : c[2];
       T v8[4][8];
 #pragma unroll
-      for (int s = 0; s < 4; s++) load8(base + (size_t)(ci[s] + cj[s] * G.nx0 + ck * G.nx0 * G.nx1) * 8, v8[s]);
+      for (int s = 0; s < 4; s++) load8(at32(prim, (qc + (unsigned)(ci[s] + cj[s] * G.nx0 + ck * G.nx0 * G.nx1)) * 8u), v8[s]);
       __builtin_amdgcn_sched_barrier(0);
       const T wz = z ? del[2] : w3;
 #pragma unroll
