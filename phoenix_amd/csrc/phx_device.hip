@@ -244,12 +244,12 @@ struct StHam {
         sum[i] = d;
         y[i] = b[i] + 0.5f * dl * d;
       }
-      // stages 2-4 at the offsets h d of stage 1's point (|h d| <= 0.05; else evaluated afresh)
+      // stages 2-4 at the offsets h d of stage 1's point (|h d| <= 0.02; else evaluated afresh)
       float e1 = 0.5f * dl * float(d1[0]), e2 = 0.5f * dl * float(d1[1]);
 #pragma unroll
       for (int q = 1; q < 4; q++) {
         float d[5], k0;
-        const Pt32 t = (fabsf(e1) <= 0.05f && fabsf(e2) <= 0.05f) ? pt32_offset(tb, e1, e2, args.P.hs)
+        const Pt32 t = (fabsf(e1) <= 0.02f && fabsf(e2) <= 0.02f) ? pt32_offset(tb, e1, e2, args.P.hs)
                                                                    : pt32_at(y[0], y[1], args.P.hs);
         ham_core<float>(args.P.a, args.P.hs, pt32, pp32, t.r, t.s2x, t.c2x, t.s, t.c, y[3], y[4], d, &k0);
         const float w = q == 3 ? 1.f : 2.f, h = q == 2 ? dl : 0.5f * dl;

@@ -488,20 +488,22 @@ PHX_DEV Pt32 pt32_at(float x1, float x2, float hs) {
   t.c = t.cx * cd - t.sx * sd;
   return t;
 }
-// sin and cos of a small |e| <= 0.05 (Taylor to e^5 / e^6: truncation below 2e-11)
+// sin and cos of a small |e| <= 0.03 (Taylor to e^3 / e^4: truncation below 7e-9 of sin e and
+// 1e-12 of cos e, under FP32 rounding)
 PHX_DEV void sincos_tiny(float e, float* s, float* c) {
   const float z = e * e;
-  *s = fmaf(e * z, fmaf(z, 1.f / 120.f, -1.f / 6.f), e);
-  *c = fmaf(z, fmaf(z, fmaf(z, -1.f / 720.f, 1.f / 24.f), -0.5f), 1.f);
+  *s = fmaf(e * z, -1.f / 6.f, e);
+  *c = fmaf(z, fmaf(z, 1.f / 24.f, -0.5f), 1.f);
 }
 // The point quantities at (x1 + e1, x2 + e2) from those at (x1, x2), for offsets |e1|, |e2| <=
-// 0.05 (an RK4 stage of a step, whose rule keeps |dl k^1| and |dl k^2| near the 0.01 stepsize):
-// exp by its Taylor polynomial to e1^5, sin/cos by angle addition with sincos_tiny, theta'
-// offset = e2 + (h/2)(sin 2(x2 + e2) - sin 2x2).  Errors stay at FP32 rounding; the offsets are
-// applied to the base point exactly, not to a rounded stage position.
+// 0.02 (an RK4 stage of a step, whose rule keeps |dl k^1| and |dl k^2| at or below the 0.01
+// stepsize; the theta offset is at most (1 + h) |e2|): exp by its Taylor polynomial to e1^4
+// (truncation below 3e-11), sin/cos by angle addition with sincos_tiny, theta' offset =
+// e2 + (h/2)(sin 2(x2 + e2) - sin 2x2).  Errors stay at FP32 rounding; the offsets are applied to
+// the base point exactly, not to a rounded stage position.
 PHX_DEV Pt32 pt32_offset(const Pt32& b, float e1, float e2, float hs) {
   Pt32 t;
-  const float ex = e1 * fmaf(e1, fmaf(e1, fmaf(e1, fmaf(e1, 1.f / 120.f, 1.f / 24.f), 1.f / 6.f), 0.5f), 1.f);
+  const float ex = e1 * fmaf(e1, fmaf(e1, fmaf(e1, 1.f / 24.f, 1.f / 6.f), 0.5f), 1.f);
   t.r = fmaf(b.r, ex, b.r);
   float se, ce;
   sincos_tiny(e2, &se, &ce);

@@ -186,6 +186,22 @@ def test_escaping_rays_end_at_rt_outer_with_the_same_results(c1_run, precision, 
     snap.close()
 
 
+def test_distant_camera_mixed_matches_fp64(snapdir, tmp_path):
+    """A camera at r = 1e11 (r^2 terms of order 1e22 in the FP32 stages of the mixed march): the
+    mixed image is finite and passes the parity gates against the FP64 march of the same camera."""
+    dump, grid, _ = S.ensure_snapshot("c1", snapdir)
+    cfg = configs.write_config(str(tmp_path / "far.toml"), dump, grid, camera__img_width="24",
+                               camera__img_height="24", camera__r_cam="1e11")
+    run = capi.Run(cfg)
+    ref, i64 = run.dev_render(capi.FP64)
+    res, imx = run.dev_render(capi.MIXED)
+    figs = parity.compare(res, ref)
+    print(f"\nr_cam 1e11: mixed steps {imx['steps']} (FP64 {imx['fp64_steps']}), FP64 march steps {i64['steps']}; "
+          f"{parity.summary(figs)}")
+    assert np.all(np.isfinite(res))
+    assert parity.passes(figs), parity.summary(figs)
+
+
 @pytest.mark.parametrize("preset", ["c1", "c2"])
 def test_field_free_leaves_change_nothing(preset, snapdir, tmp_path, monkeypatch):
     """Samples in field-free leaves (B = 0 in every cell) gather only rho and u, and those whose leaf's
