@@ -323,9 +323,9 @@ template <class T>
 PHX_DEV void null_resolve(const Met<T>& g, T k[4], T r) {
   if (fabs(dot(g, k, k)) > T(1e-6) && r > T(2)) {
     const T aa = g.d00;
-    const T bc = -(g.d01 * k[1]) / aa - (g.d03 * k[3]) / aa;
-    const T gz = -(g.d11 * k[1] * k[1]) / aa - (T(2) * g.d13 * k[1] * k[3]) / aa - (g.d22 * k[2] * k[2]) / aa -
-                 (g.d33 * k[3] * k[3]) / aa;
+    const T bc = div_rcp(-(g.d01 * k[1]), aa) - div_rcp(g.d03 * k[3], aa);
+    const T gz = div_rcp(-(g.d11 * k[1] * k[1]), aa) - div_rcp(T(2) * g.d13 * k[1] * k[3], aa) -
+                 div_rcp(g.d22 * k[2] * k[2], aa) - div_rcp(g.d33 * k[3] * k[3], aa);
     k[0] = bc + sqrt(bc * bc + gz);
   }
 }

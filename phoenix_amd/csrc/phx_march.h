@@ -80,8 +80,13 @@ PHX_DEV T div_const(T x, double c) {
   else
     return x / T(c);
 }
-// a / b: FP32 by the hardware reciprocal (1 ulp) and a multiply
-PHX_DEV float div_rcp(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+// a / b: FP32 by the hardware reciprocal (1 ulp) and a multiply that is not fused with what
+// follows.  For |b| < 2^96 this is the value of the compiler's own fast FP32 division (which adds a
+// range scaling of b: four more instructions); every denominator it is used for stays far below.
+PHX_DEV float div_rcp(float a, float b) {
+#pragma clang fp contract(off)
+  return a * __builtin_amdgcn_rcpf(b);
+}
 PHX_DEV double div_rcp(double a, double b) { return a / b; }
 
 PHX_DEV void scs(float x, float* s, float* c) { sincosf(x, s, c); }
@@ -582,8 +587,8 @@ PHX_DEV void wp_angle(T a, const Met<T>& g, const T k[4], const T perp[4], const
   wp_kappa(a, g.r, g.s, g.c, g.hf, k, para, &e2r, &e2i);
   const T f1 = kap1 * e2i - e2r * kap2, f2 = e1r * kap2 - kap1 * e1i;
   const T nn = f1 * f1 + f2 * f2;
-  *c2 = (f1 * f1 - f2 * f2) / nn;
-  *s2 = T(2) * f1 * f2 / nn;
+  *c2 = div_rcp(f1 * f1 - f2 * f2, nn);
+  *s2 = div_rcp(T(2) * f1 * f2, nn);
 }
 template <class T>
 PHX_DEV T pole_x2(const Ray<T>& g) {  // geodesic.hpp:200-205
@@ -1074,13 +1079,13 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
     fl.ne = rho * Ne + tiny;
     const T Bsq = tiny;
     fl.B = sqrt(Bsq) * Bu;
-    const T beta = uu * (P.gam - T(1)) / (T(0.5) * (Bsq + tiny));
+    const T beta = div_rcp(uu * (P.gam - T(1)), T(0.5) * (Bsq + tiny));
     const T bb = beta * P.ibc;
     const T b2 = bb * bb;
-    const T trat = rhigh * b2 / (T(1) + b2) + P.rlow / (T(1) + b2);
-    const T te_unit = T(1. / 3. * (MP / ME)) / (trat + T(1));
-    fl.te = (uu / rho) * te_unit;
-    if ((Bsq / (rho + T(1e-20)) > P.sigma_cut) || exp_r(fl.x[1]) > P.rt_outer || fl.te > P.te_max ||
+    const T trat = div_rcp(rhigh * b2, T(1) + b2) + div_rcp(P.rlow, T(1) + b2);
+    const T te_unit = div_rcp(T(1. / 3. * (MP / ME)), trat + T(1));
+    fl.te = div_rcp(uu, rho) * te_unit;
+    if ((div_rcp(Bsq, rho + T(1e-20)) > P.sigma_cut) || exp_r(fl.x[1]) > P.rt_outer || fl.te > P.te_max ||
         fl.te < P.te_min) {
       fl.ne = T(0);
       return 0;
@@ -1094,30 +1099,30 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
   const T gV[4] = {T(0), qv[2], qv[3], qv[4]};
   const T Bp[4] = {T(0), qv[5], qv[6], qv[7]};
   fl.ne = rho * Ne + tiny;
-  const T shift1 = u.u01 / (-u.u00);  // shift^2 = shift^3 = 0 in Kerr-Schild
+  const T shift1 = div_rcp(u.u01, -u.u00);  // shift^2 = shift^3 = 0 in Kerr-Schild
   const T alpha = T(1) / sqrt(-u.u00);
   const T vv = g.d11 * gV[1] * gV[1] + T(2) * g.d13 * gV[1] * gV[3] + g.d22 * gV[2] * gV[2] + g.d33 * gV[3] * gV[3];
   const T lf = sqrt(vv + T(1));
-  const T V1 = gV[1] / lf, V2 = gV[2] / lf, V3 = gV[3] / lf;
-  fl.U[0] = lf / alpha;
-  fl.U[1] = V1 * lf - shift1 * lf / alpha;
+  const T V1 = div_rcp(gV[1], lf), V2 = div_rcp(gV[2], lf), V3 = div_rcp(gV[3], lf);
+  fl.U[0] = div_rcp(lf, alpha);
+  fl.U[1] = V1 * lf - div_rcp(shift1 * lf, alpha);
   fl.U[2] = V2 * lf;
   fl.U[3] = V3 * lf;
   const T Vl1 = g.d11 * V1 + g.d13 * V3, Vl2 = g.d22 * V2, Vl3 = g.d13 * V1 + g.d33 * V3;
-  fl.b[0] = lf * (Bp[1] * Vl1 + Bp[2] * Vl2 + Bp[3] * Vl3) / alpha;
+  fl.b[0] = div_rcp(lf * (Bp[1] * Vl1 + Bp[2] * Vl2 + Bp[3] * Vl3), alpha);
 #pragma unroll
-  for (int i = 1; i < 4; i++) fl.b[i] = (Bp[i] + alpha * fl.b[0] * fl.U[i]) / lf;
+  for (int i = 1; i < 4; i++) fl.b[i] = div_rcp(Bp[i] + alpha * fl.b[0] * fl.U[i], lf);
   T bd[4];
   lower(g, fl.b, bd);
   const T Bsq = fabs(fl.b[0] * bd[0] + fl.b[1] * bd[1] + fl.b[2] * bd[2] + fl.b[3] * bd[3]) + tiny;
   fl.B = sqrt(Bsq) * Bu;
-  const T beta = uu * (P.gam - T(1)) / (T(0.5) * (Bsq + tiny));
+  const T beta = div_rcp(uu * (P.gam - T(1)), T(0.5) * (Bsq + tiny));
   const T bb = beta * P.ibc;
   const T b2 = bb * bb;
-  const T trat = rhigh * b2 / (T(1) + b2) + P.rlow / (T(1) + b2);
-  const T te_unit = T(1. / 3. * (MP / ME)) / (trat + T(1));
-  fl.te = (uu / rho) * te_unit;
-  if ((Bsq / (rho + T(1e-20)) > P.sigma_cut) || g.r > P.rt_outer || fl.te > P.te_max || fl.te < P.te_min) {
+  const T trat = div_rcp(rhigh * b2, T(1) + b2) + div_rcp(P.rlow, T(1) + b2);
+  const T te_unit = div_rcp(T(1. / 3. * (MP / ME)), trat + T(1));
+  fl.te = div_rcp(uu, rho) * te_unit;
+  if ((div_rcp(Bsq, rho + T(1e-20)) > P.sigma_cut) || g.r > P.rt_outer || fl.te > P.te_max || fl.te < P.te_min) {
     fl.ne = T(0);
     return 0;
   }
@@ -1132,19 +1137,19 @@ PHX_DEV void frame(const Met<T>& g, T a, const T k[4], const T U[4], const T b[4
   const T om = -dot(g, k, U);
   T eK[4];
 #pragma unroll
-  for (int i = 0; i < 4; i++) eK[i] = k[i] / om - U[i];
+  for (int i = 0; i < 4; i++) eK[i] = div_rcp(k[i], om) - U[i];
   const T Be = dot(g, U, b);
-  const T Cc = dot(g, k, b) / om - Be;
+  const T Cc = div_rcp(dot(g, k, b), om) - Be;
   const T b2 = dot(g, b, b);
   const T Nc = sqrt(b2 + Be * Be - Cc * Cc);
 #pragma unroll
-  for (int i = 0; i < 4; i++) para[i] = (b[i] + Be * U[i] - Cc * eK[i]) / Nc;
+  for (int i = 0; i < 4; i++) para[i] = div_rcp(b[i] + Be * U[i] - Cc * eK[i], Nc);
   T Ud[4], kd[4], bd[4];
   lower(g, U, Ud);
   lower(g, k, kd);
   lower(g, b, bd);
   const T sq = (g.r * g.r + a * a * g.c * g.c) * fabs(g.s) * g.r * g.hf;
-  const T s = T(-1) / (sq * om * Nc);
+  const T s = div_rcp(T(-1), sq * om * Nc);
   // Levi-Civita contraction eps^{i..} U_a k_c b_d (tetrad.hpp:70-80)
   const T m01 = kd[0] * bd[1] - kd[1] * bd[0], m02 = kd[0] * bd[2] - kd[2] * bd[0], m03 = kd[0] * bd[3] - kd[3] * bd[0];
   const T m12 = kd[1] * bd[2] - kd[2] * bd[1], m13 = kd[1] * bd[3] - kd[3] * bd[1], m23 = kd[2] * bd[3] - kd[3] * bd[2];
@@ -1158,7 +1163,7 @@ PHX_DEV T pitch_angle(const Met<T>& g, const T k[4], const T b[4], const T U[4])
   const T bn = sqrt(fabs(dot(g, b, b)));
   if (bn == T(0)) return T(PI_D / 2.);
   const T kk = fabs(dot(g, k, U));
-  T mu = dot(g, k, b) / (kk * bn);
+  T mu = div_rcp(dot(g, k, b), kk * bn);
   if (fabs(mu) > T(1)) mu = copysign(T(1), mu);  // mu / |mu|
   return acos(mu);
 }
@@ -1286,24 +1291,24 @@ PHX_DEV Coef<T> thermal(T te, T ne, T nu, T B, T sth, T cth, T kr1, T kr0) {
   r.jU = r.aU = r.rU = T(0);
   const T ite = div_rcp(T(1), te);
   const T nuc = T(NU_CRIT) * B * sth * te * te;
-  const T x = nu / nuc;
+  const T x = div_rcp(nu, nuc);
   const T pref = T(JNU) * ne * nu * ite * ite;
-  const T cx = cbrt(x), ic = T(1) / cx, ex = exp(T(-1.8899) * cx);
+  const T cx = cbrt(x), ic = div_rcp(T(1), cx), ex = exp(T(-1.8899) * cx);
   r.jI = pref * (T(2.5651) * (T(1) + T(1.92) * ic + T(0.9977) * ic * ic) * ex);
   r.jQ = pref * (T(2.5651) * (T(1) + T(0.93193) * ic + T(0.499873) * ic * ic) * ex);
   r.jV = T(4. / 3.) * pref * ite * div_rcp(cth, sth) *
-         ((T(1.81348) * ic * ic * ic + T(3.42319) * ic * ic + T(0.0292545) / sqrt(x) + T(2.03773) * ic) * ex);
+         ((T(1.81348) * ic * ic * ic + T(3.42319) * ic * ic + div_rcp(T(0.0292545), sqrt(x)) + T(2.03773) * ic) * ex);
   const T pre = T(2. * HP) * nu * nu * nu / T(CL * CL);
-  const T xx = T(H_OVER_MEC2) * nu / te;
-  const T Bnu = Lim<T>::fp32 ? pre / expm1(xx) : pre / (exp(xx) - T(1));
-  const T iB = T(1) / Bnu;
+  const T xx = div_rcp(T(H_OVER_MEC2) * nu, te);
+  const T Bnu = Lim<T>::fp32 ? div_rcp(pre, expm1(xx)) : pre / (exp(xx) - T(1));
+  const T iB = div_rcp(T(1), Bnu);
   r.aI = r.jI * iB;
   r.aQ = r.jQ * iB;
   r.aV = r.jV * iB;
-  const T iw = T(1) / (T(2 * PI_D) * nu);
+  const T iw = div_rcp(T(1), T(2 * PI_D) * nu);
   const T wp2 = T(WP2) * ne;
   const T om0 = T(GYRO) * B;
-  const T Xe = te * sqrt(T(1.4142135623730951) * sth * (T(1.e3 / (2. * PI_D)) * om0 / nu));
+  const T Xe = te * sqrt(T(1.4142135623730951) * sth * div_rcp(T(1.e3 / (2. * PI_D)) * om0, nu));
   const T q = om0 * iw;
   const T wpw = wp2 * iw;
   r.rQ = (T(0.5) / T(CL)) * wpw * q * q * fm_conv(Xe) * (kr1 + T(6) * te) * sth * sth;
@@ -1327,7 +1332,7 @@ PHX_DEV void kratios(T te, T* kr1, T* kr0) {
 // (coeffs.hpp:147-182)
 template <class T>
 PHX_DEV Coef<T> invariant_rotated(Coef<T> c, T nup, T jscale, T c2, T s2) {
-  const T js = jscale / nup / nup;
+  const T js = div_rcp(div_rcp(jscale, nup), nup);
   c.jI = c.jI * js;
   c.jQ = c.jQ * js;
   c.jV = c.jV * js;
@@ -1336,7 +1341,7 @@ PHX_DEV Coef<T> invariant_rotated(Coef<T> c, T nup, T jscale, T c2, T s2) {
   c.aV = c.aV * nup;
   c.rQ = c.rQ * nup;
   c.rV = c.rV * nup;
-  const T pf = sqrt(c.jQ * c.jQ + c.jV * c.jV) / c.jI;
+  const T pf = div_rcp(sqrt(c.jQ * c.jQ + c.jV * c.jV), c.jI);
   if (pf > T(1)) {
     c.jQ = div_rcp(c.jQ, pf + T(0.005));
     c.jU = div_rcp(c.jU, pf + T(0.005));
