@@ -251,3 +251,37 @@ def test_writer_falls_back_to_text_when_hdf5_layout_is_invalid(tmp_path, snapdir
     rows = np.loadtxt(tmp_path / "o" / "image.dat")
     assert rows.shape == (100, 8)
     run.close()
+
+
+# ------------------------------------------------------------------ device tables
+def _float_table(text, name):
+    body = re.search(name + r"\[3\]\[24\][^=]*=\s*\{(.*?)\};", text, re.S).group(1)
+    rows = re.findall(r"\{([^{}]*)\}", body)
+    return np.array([[float(v) for v in r.split(",") if v.strip()] for r in rows], dtype=np.float32)
+
+
+def test_bessel_tables_padded_with_zeros_give_the_same_clenshaw_sums():
+    """The FP32 march evaluates the Bessel-ratio Chebyshev fits over all 24 table entries of a piece
+    instead of the fit's own length: the entries past the length are 0, the recurrence stays exactly
+    at 0 through them from its zero start, so the sums are bit-identical (float32 arithmetic here)."""
+    with open(os.path.join(ROOT, "phoenix_amd", "csrc", "phx_bessel_table.h")) as f:
+        text = f.read()
+    nterm = [int(v) for v in re.search(r"PHX_BES_NTERM\[3\] = \{([^}]*)\}", text).group(1).split(",")]
+    z = np.linspace(-1.0, 1.0, 4001, dtype=np.float32)
+    two = np.float32(2)
+    for name in ("PHX_BES_K1K2_F", "PHX_BES_K0K2_F"):
+        tab = _float_table(text, name)
+        assert tab.shape == (3, 24)
+        for p in range(3):
+            assert np.all(tab[p, nterm[p]:] == 0), (name, p)
+
+            def clenshaw(top):
+                b1 = np.zeros_like(z)
+                b2 = np.zeros_like(z)
+                for j in range(top, 0, -1):
+                    t = two * z * b1 - b2 + tab[p, j]
+                    b2, b1 = b1, t
+                return tab[p, 0] + z * b1 - b2
+
+            short, padded = clenshaw(nterm[p] - 1), clenshaw(23)
+            assert np.array_equal(short.view(np.uint32), padded.view(np.uint32)), (name, p)
