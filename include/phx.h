@@ -227,15 +227,17 @@ int phx_snapshot_set_tune(phx_snapshot* snap, const phx_tune* tune);
  * in arrival order while their result buffers fit hbm_budget; every launch marches the next
  * quantum_jobs jobs (pixels x frequency chunks) across all admitted requests of a snapshot, so
  * requests arriving while others are on the GPU join the next launch; snapshots with work run
- * on their own streams.  Latency = submit -> result copied to the caller's buffer.
+ * on their own streams, each with two launches queued (one runs while the next is built).
+ * Result buffers come from one pool of hbm_budget bytes allocated at start (no allocation per
+ * request).  Latency = submit -> result copied to the caller's buffer.
  * The requests of one snapshot must share the launch-wide parameters of a batch render. */
 typedef struct phx_server phx_server;
 typedef struct phx_server_opts {
   int32_t precision;    /* PHX_FP32 (default), PHX_FP64 or PHX_MIXED */
   int32_t quantum_jobs; /* jobs per launch; 0 = 262144 */
-  int64_t hbm_budget;   /* bytes of admitted result buffers; 0 = 8 GiB */
+  int64_t hbm_budget;   /* bytes of admitted result buffers (the result pool); 0 = 8 GiB */
   int32_t max_active;   /* requests in flight at most; 0 = no limit but the budget */
-  int32_t pad;
+  int32_t order;        /* jobs per launch taken 0: oldest request first, 1: least attained service first */
 } phx_server_opts;
 typedef struct phx_serve_stats {
   int64_t submitted, completed, failed, active, waiting;
@@ -255,7 +257,9 @@ int phx_server_submit(phx_server* srv, int32_t snap, const phx_model* request, p
 int phx_server_wait(phx_server* srv, int64_t ticket, double timeout_s, int32_t* done, double* latency_s);
 int phx_server_stats(phx_server* srv, phx_serve_stats* stats);
 int phx_server_reset_stats(phx_server* srv);
-/* stops the worker after the launch in progress; requests not completed are dropped */
+/* stops the worker once its queued launches have finished; requests not completed fail with
+ * "server stopped" (a phx_server_wait on them returns PHX_E_DEVICE), and the call returns after
+ * every thread waiting in phx_server_wait has left it */
 int phx_server_stop(phx_server* srv);
 
 /* decoded primitives of a snapshot, [nleafs][cells][8] as double (for checks) */

@@ -59,7 +59,7 @@ def make_params(**kw) -> Params:
 def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
     if os.path.isdir("/root/reference/src"):
-        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+        subprocess.run(["make", "-s", "-C", HERE, "ref", "ref_output"], check=True)
 
 
 _lib = None

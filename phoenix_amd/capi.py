@@ -84,7 +84,7 @@ class Tune(ctypes.Structure):
 
 class ServerOpts(ctypes.Structure):
     _fields_ = [("precision", c_int), ("quantum_jobs", c_int), ("hbm_budget", c_int64), ("max_active", c_int),
-                ("pad", c_int)]
+                ("order", c_int)]
 
 
 class ServeStats(ctypes.Structure):
@@ -370,9 +370,9 @@ class Server:
     """phx_server_*: continuous-batching render service over resident snapshots."""
 
     def __init__(self, snapshots, precision: int = FP32, quantum_jobs: int = 0, hbm_budget: int = 0,
-                 max_active: int = 0):
+                 max_active: int = 0, order: int = 0):
         arr = (ctypes.c_void_p * len(snapshots))(*[sn.h.value for sn in snapshots])
-        opts = ServerOpts(precision, quantum_jobs, hbm_budget, max_active, 0)
+        opts = ServerOpts(precision, quantum_jobs, hbm_budget, max_active, order)
         self.h = ctypes.c_void_p()
         check(lib().phx_server_start(arr, len(snapshots), ctypes.byref(opts), ctypes.byref(self.h)))
         self._keep = {}

@@ -6,34 +6,13 @@
 // records gathered with 16-byte loads, and a fully unrolled Gauss-Jordan
 // inverse whose partial pivoting is done with selects (no scratch memory).
 #pragma once
-#include <hip/hip_runtime.h>
-
 #include <type_traits>
 
+#include "phx_target.h"
 #include "phx_bessel_table.h"
 
 namespace phxd {
 
-#define PHX_DEV __device__ __forceinline__
-
-// The global-memory view of a pointer.  The march reaches its tables through pointers that it
-// loads from memory, whose address space the compiler cannot infer: loads through them are flat
-// instructions, which also count against the scalar-load counter, so that every wait for a scalar
-// load waits for them as well.  Loads through this view are global loads.
-template <class T>
-using gptr = const __attribute__((address_space(1))) T*;
-template <class T>
-PHX_DEV gptr<T> glb(const T* p) {
-  return (gptr<T>)p;
-}
-// Element i of a table at a uniform base, by a 32-bit byte offset: the load then takes the base
-// from scalar registers and the offset from one vector register, with no 64-bit address arithmetic
-// per lane.  The table must be smaller than 4 GiB (the snapshot loader checks the cell table).
-template <class T>
-PHX_DEV gptr<T> at32(gptr<T> base, unsigned i) {
-  using C = const __attribute__((address_space(1))) char*;
-  return (gptr<T>)((C)base + i * (unsigned)sizeof(T));
-}
 // a copy of a struct in global memory, loaded dword by dword through the global view
 typedef unsigned __attribute__((may_alias)) dword_alias;
 template <class S>
@@ -85,7 +64,7 @@ PHX_DEV T div_const(T x, double c) {
 // range scaling of b: four more instructions); every denominator it is used for stays far below.
 PHX_DEV float div_rcp(float a, float b) {
 #pragma clang fp contract(off)
-  return a * __builtin_amdgcn_rcpf(b);
+  return a * hw_rcp(b);
 }
 PHX_DEV double div_rcp(double a, double b) { return a / b; }
 
@@ -175,28 +154,26 @@ PHX_DEV void theta_trig(double x2, double hs, double* s2x, double* c2x, double* 
 // Kept in constant memory and loaded where a polynomial is evaluated (taylor_coef): as FP64
 // instruction operands they would otherwise be materialized in scalar registers and held across
 // the whole march loop, which spills the march's other uniform values.
-__constant__ double kTaylorSin[9] = {-1. / 121645100408832000., 1. / 355687428096000., -1. / 1307674368000.,
+PHX_CONSTANT double kTaylorSin[9] = {-1. / 121645100408832000., 1. / 355687428096000., -1. / 1307674368000.,
                                      1. / 6227020800.,          -1. / 39916800.,       1. / 362880.,
                                      -1. / 5040.,               1. / 120.,             -1. / 6.};
-__constant__ double kTaylorCos[10] = {1. / 2432902008176640000., -1. / 6402373705728000., 1. / 20922789888000.,
+PHX_CONSTANT double kTaylorCos[10] = {1. / 2432902008176640000., -1. / 6402373705728000., 1. / 20922789888000.,
                                       -1. / 87178291200.,        1. / 479001600.,         -1. / 3628800.,
                                       1. / 40320.,               -1. / 720.,              1. / 24.,
                                       -0.5};
 struct TaylorSC {
   double s[9], c[10];
 };
-// the coefficients, loaded (scalar loads) at the call: the empty asm makes the table address a new
-// value there, so the loads are not hoisted out of an enclosing loop
+// the coefficients, loaded (scalar loads) at the call through an opaque copy of the table address,
+// so that the loads are not hoisted out of an enclosing loop
 PHX_DEV TaylorSC taylor_coef() {
-  const double* ps = kTaylorSin;
-  const double* pc = kTaylorCos;
-  asm volatile("" : "+s"(ps), "+s"(pc));
-  using W = const __attribute__((address_space(4))) double*;
+  const double* ps = opaque_scalar<const double*>(kTaylorSin);
+  const double* pc = opaque_scalar<const double*>(kTaylorCos);
   TaylorSC t;
 #pragma unroll
-  for (int i = 0; i < 9; i++) t.s[i] = ((W)ps)[i];
+  for (int i = 0; i < 9; i++) t.s[i] = load_constant(ps, i);
 #pragma unroll
-  for (int i = 0; i < 10; i++) t.c[i] = ((W)pc)[i];
+  for (int i = 0; i < 10; i++) t.c[i] = load_constant(pc, i);
   return t;
 }
 // sin(r) and cos(r) by the Taylor polynomials above (|r| <= pi/4: truncation below 1e-19)
@@ -317,68 +294,113 @@ PHX_DEV void lower(const Met<T>& g, const T V[4], T out[4]) {
 }
 
 // ------------------------------------------------------------------ geodesic
-// Per-point quantities of the RHS (src/core/geometry/geodesic.hpp:81-109)
+// The connection of the MKS-BHAC Kerr-Schild metric, from its derivatives.  The metric depends on
+// x1 = ln r and x2 only, so with A = d g / d x1 and B = d g / d x2 (both with the sparsity of g)
+//   Gamma_nu(U, V) = 1/2 [U^1 (A V)_nu + U^2 (B V)_nu + V^1 (A U)_nu + V^2 (B U)_nu]
+//                    - 1/2 delta_nu^1 U.A.V - 1/2 delta_nu^2 U.B.V
+// (Gamma_{nu ab} U^a V^b with Gamma_{nu ab} = (g_{nu a, b} + g_{nu b, a} - g_{ab, nu}) / 2), and the
+// geodesic / parallel-transport right-hand side is -g^{mu nu} Gamma_nu.  With r = e^{x1},
+// theta = x2 + h/2 sin 2x2, theta' = 1 + h cos 2x2, theta'' = -2 h sin 2x2, s = sin theta,
+// c = cos theta, Sigma = r^2 + a^2 c^2 and z = 2 r / Sigma, the covariant metric is
+//   g00 = z - 1, g01 = z r, g03 = -a z s^2, g11 = (1 + z) r^2, g13 = -a (1 + z) r s^2,
+//   g22 = Sigma theta'^2, g33 = s^2 Sigma + a^2 s^4 (1 + z),
+// and, writing z1 = dz/dx1 = z (a^2 c^2 - r^2) / Sigma (so that z + z1 = 2 z a^2 c^2 / Sigma exactly,
+// kept in that form: the two terms cancel at large r), z2 = dz/dx2 = 2 a^2 s c theta' z / Sigma,
+// S2 = dSigma/dx2 = -2 a^2 s c theta' and q2 = d(s^2)/dx2 = 2 s c theta':
+//   A00 = z1, A01 = r (z + z1), A03 = -a s^2 z1, A11 = r^2 (2 + z + (z + z1)), A13 = -a r s^2 (1 + (z + z1)),
+//   A22 = 2 r^2 theta'^2, A33 = s^2 (2 r^2 + a^2 s^2 z1);
+//   B00 = z2, B01 = r z2, B03 = -a (s^2 z2 + z q2), B11 = r^2 z2, B13 = -a r (q2 (1 + z) + s^2 z2),
+//   B22 = theta' (S2 theta' + 2 Sigma theta''), B33 = q2 (Sigma + 2 a^2 s^2 (1 + z)) + s^2 (S2 + a^2 s^2 z2).
+// The contravariant metric: g^00 = -1 - z, g^01 = 2 / Sigma, g^11 = (r^2 - 2 r + a^2) / (Sigma r^2),
+// g^13 = a / (Sigma r), g^22 = 1 / (Sigma theta'^2), g^33 = 1 / (Sigma s^2) (metric.hpp:64-91).
+// This is the same connection as the reference's closed form (geodesic.hpp:81-169), written from
+// the metric's derivatives; tests/test_cpu_lane.py holds the march to the reference's FP64 build.
+template <class T>
+struct SymD {  // a symmetric 4x4 with the sparsity of the Kerr-Schild metric
+  T m00, m01, m03, m11, m13, m22, m33;
+};
+template <class T>
+PHX_DEV void symd_vec(const SymD<T>& D, const T v[4], T out[4]) {
+  out[0] = D.m00 * v[0] + D.m01 * v[1] + D.m03 * v[3];
+  out[1] = D.m01 * v[0] + D.m11 * v[1] + D.m13 * v[3];
+  out[2] = D.m22 * v[2];
+  out[3] = D.m03 * v[0] + D.m13 * v[1] + D.m33 * v[3];
+}
 template <class T>
 struct Geo {
-  T a, a2, r, r2, tp, tp2, tpp, s, c, s2, s2th, cot, Sig, Sb, Del, A, B, C;
+  SymD<T> A, B;                          // d g / d x1, d g / d x2
+  T u00, u01, u11, u13, u22, u33;        // g^{mu nu}
 };
 template <class T, bool ONE_SINCOS = false>
 PHX_DEV Geo<T> geo_at(T a, T hs, T x1, T x2) {
   Geo<T> G;
-  G.a = a;
-  G.a2 = a * a;
-  G.r = exp_r(x1);
-  G.r2 = G.r * G.r;
-  T s2x, c2x;
-  theta_trig_sel<ONE_SINCOS>(x2, hs, &s2x, &c2x, &G.s, &G.c);
-  G.tp = T(1) + hs * c2x;
-  G.tp2 = G.tp * G.tp;
-  G.tpp = -T(2) * hs * s2x;
-  G.s2 = G.s * G.s;
-  G.s2th = T(2) * G.s * G.c;
-  G.cot = G.c / G.s;
-  G.Sig = G.r2 + G.a2 * G.c * G.c;
-  G.Sb = T(1) / G.Sig;
-  G.Del = G.r2 - T(2) * G.r + G.a2;
-  G.A = G.Sig * G.Del + T(2) * G.r * (G.r2 + G.a2);
-  G.B = (T(2) * G.r2 - G.Sig) * G.Sb * G.Sb * G.Sb;
-  G.C = G.r * G.Sb - G.a2 * G.B * G.s2;
+  const T r = exp_r(x1), r2 = r * r, a2 = a * a;
+  T s2x, c2x, s, c;
+  theta_trig_sel<ONE_SINCOS>(x2, hs, &s2x, &c2x, &s, &c);
+  const T tp = T(1) + hs * c2x, tpp = T(-2) * hs * s2x;
+  const T ss = s * s, sc = s * c, a2c2 = a2 * c * c;
+  const T Sig = r2 + a2c2, iS = T(1) / Sig;
+  const T z = T(2) * r * iS;
+  const T z1 = z * (a2c2 - r2) * iS, zz1 = T(2) * z * a2c2 * iS;  // dz/dx1 and z + dz/dx1
+  const T S2 = T(-2) * a2 * sc * tp, q2 = T(2) * sc * tp;
+  const T z2 = -z * S2 * iS;
+  const T opz = T(1) + z;
+  G.A.m00 = z1;
+  G.A.m01 = r * zz1;
+  G.A.m03 = -a * ss * z1;
+  G.A.m11 = r2 * (T(2) + z + zz1);
+  G.A.m13 = -a * r * ss * (T(1) + zz1);
+  G.A.m22 = T(2) * r2 * tp * tp;
+  G.A.m33 = ss * (T(2) * r2 + a2 * ss * z1);
+  G.B.m00 = z2;
+  G.B.m01 = r * z2;
+  G.B.m03 = -a * (ss * z2 + z * q2);
+  G.B.m11 = r2 * z2;
+  G.B.m13 = -a * r * (q2 * opz + ss * z2);
+  G.B.m22 = tp * (S2 * tp + T(2) * Sig * tpp);
+  G.B.m33 = q2 * (Sig + T(2) * a2 * ss * opz) + ss * (S2 + a2 * ss * z2);
+  const T ir = T(1) / r, itp = T(1) / tp, is = T(1) / s;
+  G.u00 = -opz;
+  G.u01 = T(2) * iS;
+  G.u11 = iS * (r2 - T(2) * r + a2) * ir * ir;
+  G.u13 = a * iS * ir;
+  G.u22 = iS * itp * itp;
+  G.u33 = iS * is * is;
   return G;
 }
-// -Gamma^i_jk U^j V^k in closed form (geodesic.hpp:114-169)
+// -g^{mu nu} Gamma_nu for the covariant connection contraction w (Gamma_nu above)
 template <class T>
-PHX_DEV void chris(const Geo<T>& G, const T U[4], const T V[4], T out[4]) {
-  const T h = T(0.5), two = T(2), four = T(4);
-  const T P00 = U[0] * V[0];
-  const T P01 = h * (U[0] * V[1] + U[1] * V[0]);
-  const T P02 = h * (U[0] * V[2] + U[2] * V[0]);
-  const T P03 = h * (U[0] * V[3] + U[3] * V[0]);
-  const T P11 = U[1] * V[1];
-  const T P12 = h * (U[1] * V[2] + U[2] * V[1]);
-  const T P13 = h * (U[1] * V[3] + U[3] * V[1]);
-  const T P22 = U[2] * V[2];
-  const T P23 = h * (U[2] * V[3] + U[3] * V[2]);
-  const T P33 = U[3] * V[3];
-  const T a = G.a, a2 = G.a2, r = G.r, r2 = G.r2, tp = G.tp, tp2 = G.tp2, s = G.s, c = G.c, s2 = G.s2;
-  const T s2th = G.s2th, Sig = G.Sig, Sb = G.Sb, Sb2 = Sb * Sb, Sb3 = Sb2 * Sb;
-  const T Del = G.Del, A = G.A, B = G.B, C = G.C, cot = G.cot;
-  const T K = a2 * r * s2th * Sb2 * tp;
-  out[0] = -two * (-r2 * Sb * tp2 * P22 - r * C * s2 * P33 + K * (-P02 - r * P12 + a * s2 * P23) +
-                   B * ((Sig + two * r) * r * P01 + r * P00 + (Sig + r) * r2 * P11 - two * a * r * s2 * P03 -
-                        a * s2 * (Sig + two * r) * r * P13));
-  out[1] = -((T(1) - (two * Sig - Del) * B * r) * P11 +
-             (Del / r) * (B * P00 - two * a * B * s2 * P03 - r * Sb * tp2 * P22 - C * s2 * P33) +
-             two * ((Del - Sig) * B * P01 - a2 * s * c * Sb * tp * P12 + a * (r * Sb + (Sig - Del) * B) * s2 * P13));
-  out[2] = -((G.tpp / tp - a2 * s * c * Sb * tp) * P22 +
-             Sb3 *
-                 ((-a2 * r * s2th) * (P00 + r2 * P11) - s * c * (Del * Sig * Sig + two * r * (r2 + a2) * (r2 + a2)) * P33 -
-                  two * a2 * r * s2th * r * P01 + two * a * s * c * (A + Sig * (Sig - Del)) * r * P13 +
-                  two * a * r * (r2 + a2) * s2th * P03) /
-                 tp +
-             two * r * Sb * r * P12);
-  out[3] = -(a * (B * P00 + B * r2 * P11 - r * Sb * tp2 * P22 - C * s2 * P33) + two * a * B * r * P01 -
-             four * a * r * cot * Sb2 * tp * P02 - two * a * (Sig + two * r) * cot * Sb2 * r * tp * P12 -
-             two * a2 * B * s2 * P03 + two * (cot + a2 * r * s2th * Sb2) * tp * P23 + two * C * r * P13);
+PHX_DEV void raise_neg(const Geo<T>& G, const T w[4], T out[4]) {
+  out[0] = -(G.u00 * w[0] + G.u01 * w[1]);
+  out[1] = -(G.u01 * w[0] + G.u11 * w[1] + G.u13 * w[3]);
+  out[2] = -(G.u22 * w[2]);
+  out[3] = -(G.u13 * w[1] + G.u33 * w[3]);
+}
+// the geodesic acceleration -Gamma^mu(k, k) and the parallel transport -Gamma^mu(k, f) at one point
+template <class T>
+PHX_DEV void connection(const Geo<T>& G, const T k[4], const T f[4], T acc_k[4], T acc_f[4]) {
+  T Ak[4], Bk[4], Af[4], Bf[4];
+  symd_vec(G.A, k, Ak);
+  symd_vec(G.B, k, Bk);
+  symd_vec(G.A, f, Af);
+  symd_vec(G.B, f, Bf);
+  const T kAk = k[0] * Ak[0] + k[1] * Ak[1] + k[2] * Ak[2] + k[3] * Ak[3];
+  const T kBk = k[0] * Bk[0] + k[1] * Bk[1] + k[2] * Bk[2] + k[3] * Bk[3];
+  const T kAf = k[0] * Af[0] + k[1] * Af[1] + k[2] * Af[2] + k[3] * Af[3];
+  const T kBf = k[0] * Bf[0] + k[1] * Bf[1] + k[2] * Bf[2] + k[3] * Bf[3];
+  T wk[4], wf[4];
+  const T h = T(0.5);
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    wk[n] = k[1] * Ak[n] + k[2] * Bk[n];
+    wf[n] = h * (k[1] * Af[n] + k[2] * Bf[n] + f[1] * Ak[n] + f[2] * Bk[n]);
+  }
+  wk[1] -= h * kAk;
+  wk[2] -= h * kBk;
+  wf[1] -= h * kAf;
+  wf[2] -= h * kBf;
+  raise_neg(G, wk, acc_k);
+  raise_neg(G, wf, acc_f);
 }
 
 // Ray state: (hi, lo) pairs of x, k, F1; lo is only carried in FP32 (geodesic.hpp:11-14)
@@ -397,7 +419,7 @@ PHX_DEV void two_sum_add(T& hi, T& lo, T d) {  // TwoSum + Fast2Sum (precision.h
   hi = s2;
 }
 // RK4 of x, k (geodesic) and F1 (parallel transport) with the FP32 south-hemisphere
-// fold (geodesic.hpp:222-256).  The weighted stage sum k1 + 2 k2 + 2 k3 + k4 is
+// fold (geodesic.hpp:222-256), the right-hand side from connection().  The weighted stage sum k1 + 2 k2 + 2 k3 + k4 is
 // accumulated as the stages are formed, so only it and the next stage's offset are kept.
 template <class T, bool ONE_SINCOS = false>
 PHX_DEV void rk4(T a, T hs, Ray<T>& g, T dt) {
@@ -421,8 +443,7 @@ PHX_DEV void rk4(T a, T hs, Ray<T>& g, T dt) {
     const Geo<T> G = geo_at<T, ONE_SINCOS>(a, hs, y[1], y[2]);
 #pragma unroll
     for (int i = 0; i < 4; i++) d[i] = y[4 + i];
-    chris(G, y + 4, y + 4, d + 4);
-    chris(G, y + 4, y + 8, d + 8);
+    connection(G, y + 4, y + 8, d + 4, d + 8);
     if (south) {
       d[2] = -d[2];
       d[6] = -d[6];
@@ -671,11 +692,20 @@ template <class T>
 PHX_DEV bool off_domain(const Grid<T>& G, const T x[4]) {  // sampler.hpp:43-50
   return x[1] > G.hi1 || x[1] < G.lo1 || x[2] < G.lo2 || x[2] > G.hi2 || x[3] < G.lo3 || x[3] > G.hi3;
 }
-// The box tests of the FP32-cell marches (FP32, mixed) load the six bounds first and combine the
-// comparisons without short circuit: a short-circuit chain makes each load conditional on the
-// previous comparison, and the compiler then waits for each bound in turn (six memory round trips
-// instead of one).  The FP64 march keeps the short-circuit form, with which its build reproduces
-// the reference's FP64 results (the batched form changed them there; see the design notes).
+// The batched memory forms: box tests that load the six bounds first and combine the comparisons
+// without short circuit (a short-circuit chain makes each load conditional on the previous
+// comparison, and the compiler then waits for each bound in turn: six memory round trips instead
+// of one), a sample's leaf record loaded in one round trip before its domain tests, and corner
+// gathers whose loads are issued a layer at a time.  The FP32-cell marches (FP32, mixed) always
+// use them; the FP64 march uses them when PHX_FP64_BATCHED is 1, and the short-circuit,
+// interleaved forms otherwise.  Both forms compute the same values in the same order.
+#ifndef PHX_FP64_BATCHED
+#define PHX_FP64_BATCHED 1
+#endif
+template <class T>
+constexpr bool batched_loads() {
+  return Lim<T>::fp32 || PHX_FP64_BATCHED;
+}
 // the leaf's box lo[3], hi[3]
 template <class T>
 PHX_DEV void leaf_box(gptr<T> blk, int q, T bx[6]) {
@@ -694,7 +724,7 @@ PHX_DEV bool in_box_closed(const T bx[6], const T x[4]) {
 }
 template <class T>
 PHX_DEV bool in_leaf(const Grid<T>& G, int q, const T x[4], T small) {  // half-open (sampler.hpp:87-101)
-  if constexpr (Lim<T>::fp32) {
+  if constexpr (batched_loads<T>()) {
     T bx[6];
     leaf_box(glb(G.blk), q, bx);
     return in_box_half_open(bx, x, small);
@@ -706,7 +736,7 @@ PHX_DEV bool in_leaf(const Grid<T>& G, int q, const T x[4], T small) {  // half-
 }
 template <class T>
 PHX_DEV bool in_leaf_closed(const Grid<T>& G, int q, const T x[4]) {  // inclusive (sampler.hpp:109-122)
-  if constexpr (Lim<T>::fp32) {
+  if constexpr (batched_loads<T>()) {
     T bx[6];
     leaf_box(glb(G.blk), q, bx);
     return in_box_closed(bx, x);
@@ -838,7 +868,7 @@ PHX_DEV int find_leaf_fill(const Grid<T>& G, const T x[4], LeafCache<T>& c, bool
     T box[6];
 #pragma unroll
     for (int q = 0; q < 6; q++) box[q] = b[q];
-    const bool inside = Lim<T>::fp32 ? in_box_half_open(box, x, small)
+    const bool inside = batched_loads<T>() ? in_box_half_open(box, x, small)
                                      : !(x[1] + small < box[0] || x[1] + small >= box[3] || x[2] + small < box[1] ||
                                          x[2] + small >= box[4] || x[3] + small < box[2] || x[3] + small >= box[5]);
     if (inside) {
@@ -905,11 +935,11 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
                          int cache, Fluid<T>& fl) {
   constexpr bool F32 = Lim<L>::fp32;
   constexpr bool MIXED = !std::is_same<T, L>::value;
-  // FP32 cells (the FP32 and mixed marches): the hinted leaf's whole record (box, cell size and
-  // its reciprocal) and its field-free mark are loaded in one memory round trip, before the domain
-  // tests; they serve the box test and the cell location (a failed test reloads them for the leaf
-  // the lookup finds).  The FP64 march loads the box for the test and the rest after it.
-  constexpr bool BATCH = Lim<T>::fp32;
+  // Batched form: the hinted leaf's whole record (box, cell size and its reciprocal) and its
+  // field-free mark are loaded in one memory round trip, before the domain tests; they serve the
+  // box test and the cell location (a failed test reloads them for the leaf the lookup finds).
+  // Otherwise the box is loaded for the test and the rest after it.
+  constexpr bool BATCH = batched_loads<T>();
   L x[4] = {X[0], X[1], X[2], X[3]};
   wrap(x);
   int q = cache;
@@ -950,7 +980,7 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
   // the sigma cut (sigma = (0 + 1e-6) / (rho + 1e-20) > sigma_cut; the interpolated rho is a convex
   // combination of the corners, so it lies in [0, max]; the margin covers its rounding), every
   // sample in it is cut: nothing emits and nothing is counted, as the full derivation would find.
-  if constexpr (!BATCH) load_record(q);  // the FP64 march: the record and the mark after the test
+  if constexpr (!BATCH) load_record(q);  // unbatched: the record and the mark after the test
   L blo[3], bdx[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -992,11 +1022,11 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
   T qv[8];
 #pragma unroll
   for (int a = 0; a < 8; a++) qv[a] = T(0);
-  // FP32 cells: the corner loads of a layer (4 corners; the field-free gather: all 8) are issued
+  // Batched form: the corner loads of a layer (4 corners; the field-free gather: all 8) are issued
   // before the first is used: the scheduling barriers keep the compiler from interleaving them with
   // the sums, which under register pressure waits for each corner in turn (8 memory round trips a
-  // sample).  FP64 cells (the FP64 march) keep the interleaved form.
-  if constexpr (!Lim<T>::fp32) {
+  // sample).  Otherwise the loads and sums interleave.
+  if constexpr (!BATCH) {
     if (ffleaf) {
 #pragma unroll
       for (int z = 0; z < 2; z++) {
@@ -1035,7 +1065,7 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
 #pragma unroll
       for (int s = 0; s < 4; s++) load2(at32(prim, (qc + (unsigned)(ci[s] + cj[s] * G.nx0 + ck * G.nx0 * G.nx1)) * 8u), v[4 * z + s]);
     }
-    __builtin_amdgcn_sched_barrier(0);
+    PHX_SCHED_BARRIER();
 #pragma unroll
     for (int z = 0; z < 2; z++) {
       const T wz = z ? del[2] : w3;
@@ -1053,7 +1083,7 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
       T v8[4][8];
 #pragma unroll
       for (int s = 0; s < 4; s++) load8(at32(prim, (qc + (unsigned)(ci[s] + cj[s] * G.nx0 + ck * G.nx0 * G.nx1)) * 8u), v8[s]);
-      __builtin_amdgcn_sched_barrier(0);
+      PHX_SCHED_BARRIER();
       const T wz = z ? del[2] : w3;
 #pragma unroll
       for (int s = 0; s < 4; s++) {
@@ -1062,7 +1092,7 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
 #pragma unroll
         for (int a = 0; a < 8; a++) qv[a] += v[a] * w;
       }
-      __builtin_amdgcn_sched_barrier(0);
+      PHX_SCHED_BARRIER();
     }
   }
   // derive_fluid
@@ -1199,26 +1229,6 @@ PHX_DEV T hankel(int nu, T x) {  // src/core/numerics/bessel.hpp:99-108
   return sum;
 }
 template <class T>
-PHX_DEV void bessel_scaled(T x, T* K0, T* K1, T* K2) {  // bessel.hpp:65-88
-  const T dt = fmin(T(0.2), T(0.7) / sqrt(x));
-  const T tmax = acosh(T(1) + T(42) / x) + T(2);
-  const int nmax = (int)(tmax / dt) + 1;
-  T s0 = T(0.5), s1 = T(0.5), s2 = T(0.5);
-  for (int k = 1; k <= nmax; k++) {
-    const T t = T(k) * dt;
-    const T sh = sinh(T(0.5) * t);
-    const T ch = T(1) + T(2) * sh * sh;
-    const T e = exp(-T(2) * x * sh * sh);
-    s0 += e;
-    s1 += e * ch;
-    s2 += e * (T(2) * ch * ch - T(1));
-    if (e * (T(2) * ch * ch) < Lim<T>::tol_series * T(0.1) * s2) break;
-  }
-  *K0 = dt * s0;
-  *K1 = dt * s1;
-  *K2 = dt * s2;
-}
-template <class T>
 PHX_DEV void bessel_ratios(T x, T* r10, T* r02) {  // K1/K2 and K0/K2 (bessel.hpp:118-148)
   if (x >= T(30)) {
     const T s2 = hankel(2, x);
@@ -1226,11 +1236,11 @@ PHX_DEV void bessel_ratios(T x, T* r10, T* r02) {  // K1/K2 and K0/K2 (bessel.hp
     *r02 = hankel(0, x) / s2;
     return;
   }
+  // x = 1 / Theta_e >= 0.2 here (kratios takes the asymptotic forms for Theta_e > 5), so the
+  // reference's numerical-integral branch for x < 0.01 (bessel.hpp:65-88, 138-141) is never reached;
+  // below the table's first edge the ratios are NaN, and a segment built from them is dropped
   if (x < T(0.01)) {
-    T k0, k1, k2;
-    bessel_scaled(x, &k0, &k1, &k2);
-    *r10 = k1 / k2;
-    *r02 = k0 / k2;
+    *r10 = *r02 = T(NAN);
     return;
   }
   const int p = (x < T(0.1)) ? 0 : (x < T(1.0)) ? 1 : 2;
@@ -1249,7 +1259,7 @@ PHX_DEV void bessel_ratios(T x, T* r10, T* r02) {  // K1/K2 and K0/K2 (bessel.hp
         ca[i] = j0 - i >= 0 ? bes_c<T>(0, p, j0 - i) : T(0);
         cb[i] = j0 - i >= 0 ? bes_c<T>(1, p, j0 - i) : T(0);
       }
-      __builtin_amdgcn_sched_barrier(0);
+      PHX_SCHED_BARRIER();
 #pragma unroll
       for (int i = 0; i < 8; i++) {
         if (j0 - i < 1) break;
@@ -1260,7 +1270,7 @@ PHX_DEV void bessel_ratios(T x, T* r10, T* r02) {  // K1/K2 and K0/K2 (bessel.hp
         b2b = b1b;
         b1b = tb;
       }
-      __builtin_amdgcn_sched_barrier(0);
+      PHX_SCHED_BARRIER();
     }
   } else {
     for (int j = PHX_BES_NTERM[p] - 1; j >= 1; j--) {  // Clenshaw, both fits at once

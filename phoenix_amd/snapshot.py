@@ -81,6 +81,8 @@ PRESETS = {
     "c5": SnapshotSpec(nxlone=(128, 64, 64), block=(16, 16, 16), refine=(0.1, 0.5)),
     "c5_a05": SnapshotSpec(spin=0.5, seed=2, nxlone=(128, 64, 64), block=(16, 16, 16),
                            refine=(0.1, 0.5)),
+    # base blocks written in k, j, i order (grmhd.sfc = false) on a 6x3x3 base of 8^3 blocks + 1 level
+    "c1_sfc0": SnapshotSpec(seed=3, block=(8, 8, 8), sfc=False),
     # a tiny one for unit tests: 24x12x12 in 6^3 blocks + 1 level
     "tiny": SnapshotSpec(nxlone=(24, 12, 12), block=(6, 6, 6), refine=(0.1,)),
 }

@@ -25,6 +25,7 @@ from phoenix_amd import snapshot as S  # noqa: E402
 from phoenix_amd.configs import write_config  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+FREQS8 = "[" + ", ".join(repr(float(f)) for f in np.geomspace(86e9, 690e9, 8)) + "]"
 
 # name -> (snapshot preset, config overrides as section__key: TOML text)
 CASES = {
@@ -40,6 +41,28 @@ CASES = {
                                  "camera__img_height": "32", "plasma__m_unit": "5e18"}),
     "c1a05_i10_48": ("c1_a05", {"camera__img_width": "48", "camera__img_height": "48"}),
     "c2_i10_64": ("c2", {"camera__img_width": "64", "camera__img_height": "64"}),
+    # the precision workload's shape: 8 frequencies log-spaced over 86-690 GHz, both spins, 10 / 60 / 163 deg
+    "c2_i163_8f_40": ("c2", {"camera__inclination": "163", "camera__img_width": "40", "camera__img_height": "40",
+                             "camera__frequency_list": FREQS8}),
+    "c2a05_i10_8f_40": ("c2_a05", {"camera__img_width": "40", "camera__img_height": "40",
+                                   "camera__frequency_list": FREQS8}),
+    "c2a05_i60_8f_40": ("c2_a05", {"camera__inclination": "60", "camera__img_width": "40",
+                                   "camera__img_height": "40", "camera__frequency_list": FREQS8}),
+    "c2a05_i163_8f_40": ("c2_a05", {"camera__inclination": "163", "camera__img_width": "40",
+                                    "camera__img_height": "40", "camera__frequency_list": FREQS8}),
+    # the serving snapshots: 16^3 blocks, 3 AMR levels, spins 0.9375 and 0.5; request shapes of the mix
+    "c5_i10_48": ("c5", {"camera__img_width": "48", "camera__img_height": "48"}),
+    "c5_i120_32_2f": ("c5", {"camera__inclination": "120", "camera__img_width": "32", "camera__img_height": "32",
+                             "camera__frequency_list": "[150e9, 460e9]", "plasma__m_unit": "3e18",
+                             "plasma__r_high": "25"}),
+    "c5a05_i60_40_4f": ("c5_a05", {"camera__inclination": "60", "camera__img_width": "40",
+                                   "camera__img_height": "40", "camera__frequency_list": "[86e9, 230e9, 345e9, 690e9]",
+                                   "plasma__m_unit": "7e17", "plasma__r_high": "12"}),
+    "c5a05_i170_32": ("c5_a05", {"camera__inclination": "170", "camera__img_width": "32",
+                                 "camera__img_height": "32", "camera__frequency": "345e9"}),
+    # base blocks in k, j, i order (grmhd.sfc = false), 8^3 blocks on a 6x3x3 base
+    "c1sfc0_i30_40": ("c1_sfc0", {"camera__inclination": "30", "camera__img_width": "40", "camera__img_height": "40",
+                                  "grmhd__sfc": "false"}),
 }
 
 

@@ -4,24 +4,80 @@
   |I - I_ref| / |I_ref| <= 1e-3 and |X - X_ref| / |I_ref| <= 1e-3 for X in Q, U, V;
 * total flux: |F_I - F_I,ref| / |F_I,ref| <= 1e-4 and |F_X - F_X,ref| / |F_I,ref| <= 1e-4.
 
-A handful of photon-ring pixels are chaotic (a 1-ulp change flips a theta
-turning point or the step on which a ray ends); the reference's own FP32 build
-misses the per-pixel gate on 0-2 pixels per image against its FP64 build.  The
-allowance for such pixels is max(2, 0.1% of the gated pixels).
+A few photon-ring pixels are chaotic: a last-ulp change flips a theta turning point or the step
+on which a ray ends (pixel.hpp:193-197), and the pixel then differs at the 1e-3 level or more
+whatever the precision.  The reference's own FP32 build misses the per-pixel gate on such pixels
+against its FP64 build; tests/golden/outlier_rates.json holds that rate, measured with
+oracle/_ref at 512 x 512 pixels on each snapshot family (8 frequencies and three inclinations for
+c2 and c2_a05).  An image may miss the per-pixel gate on at most 1.5 times the reference's rate of
+its snapshot family, and never on fewer pixels than the reference's FP32 build misses on the same
+golden case (when one is named).
 """
 from __future__ import annotations
 
+import json
 import math
+import os
 
 import numpy as np
 
 PIX_TOL = 1e-3
 FLUX_TOL = 1e-4
 FLOOR = 1e-6
+RATE_FACTOR = 1.5
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_RATES = None
 
 
-def compare(res: np.ndarray, ref: np.ndarray, pix_tol=PIX_TOL, flux_tol=FLUX_TOL, floor=FLOOR) -> list:
-    """res, ref: [npix, nf, 6].  Returns one dict of error figures per frequency."""
+def outlier_rates() -> dict:
+    """Per snapshot preset: the reference FP32 build's largest fraction of gated pixels outside the
+    per-pixel gate against its FP64 build, over the measured inclinations and frequencies."""
+    global _RATES
+    if _RATES is None:
+        with open(os.path.join(HERE, "golden", "outlier_rates.json")) as f:
+            data = json.load(f)
+        rates = {}
+        for d in data.values():
+            r = max(nb / g for nb, g in zip(d["nbad"], d["gated"]) if g > 0)
+            rates[d["preset"]] = max(rates.get(d["preset"], 0.0), r)
+        _RATES = rates
+    return _RATES
+
+
+# snapshot presets without a measurement of their own take their generator family's
+FAMILY = {"c1_sfc0": "c1", "tiny": "c1"}
+
+
+def rate_for(preset: str | None) -> float:
+    rates = outlier_rates()
+    if preset is None:
+        return max(rates.values())
+    return rates[FAMILY.get(preset, preset)]
+
+
+def _case_floor(case: str | None, nf: int) -> list:
+    """Pixels the reference's FP32 build misses the gate on in the named golden case, per frequency."""
+    if case is None:
+        return [0] * nf
+    from tests.golden import make_golden as G
+
+    r32, _, _ = G.load(case, 32)
+    r64, _, _ = G.load(case, 64)
+    return [d["nbad"] for d in compare(r32, r64, allowance=False)]
+
+
+def compare(res: np.ndarray, ref: np.ndarray, pix_tol=PIX_TOL, flux_tol=FLUX_TOL, floor=FLOOR, case=None,
+            preset=None, allowance=True) -> list:
+    """res, ref: [npix, nf, 6].  Returns one dict of error figures per frequency.  case / preset name
+    the golden case / snapshot preset whose measured chaotic-pixel rate sets the allowance."""
+    if allowance:
+        if case is not None and preset is None:
+            from tests.golden import make_golden as G
+
+            preset = G.CASES[case][0]
+        rate = rate_for(preset)
+        cfloor = _case_floor(case, ref.shape[1])
     out = []
     for f in range(ref.shape[1]):
         a, b = res[:, f, :4], ref[:, f, :4]
@@ -37,8 +93,10 @@ def compare(res: np.ndarray, ref: np.ndarray, pix_tol=PIX_TOL, flux_tol=FLUX_TOL
         Fb = b[np.all(np.isfinite(b), axis=1)].sum(0)
         dI = abs(Fa[0] - Fb[0]) / abs(Fb[0]) if Fb[0] != 0 else abs(Fa[0])
         dQUV = float(np.max(np.abs(Fa[1:] - Fb[1:]))) / abs(Fb[0]) if Fb[0] != 0 else float(np.max(np.abs(Fa[1:])))
+        gated = int(m.sum())
+        allowed = max(math.ceil(RATE_FACTOR * rate * gated), cfloor[f]) if allowance else 0
         out.append({
-            "f": f, "gated": int(m.sum()), "nbad": nbad, "allowed": max(2, math.ceil(1e-3 * int(m.sum()))),
+            "f": f, "gated": gated, "nbad": nbad, "allowed": allowed,
             "rel_I_max": float(rel.max()) if rel.size else 0.0,
             "rel_I_p999": float(np.percentile(rel, 99.9)) if rel.size else 0.0,
             "quv_max": float(quv.max()) if quv.size else 0.0,

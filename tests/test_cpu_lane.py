@@ -1,0 +1,97 @@
+"""The march's per-ray code (phoenix_amd/csrc/phx_lane.h, phx_march.h) built for the CPU
+(tests/cpu/lane_host.cpp) and run under the address and undefined-behaviour sanitizers:
+
+* the FP64 march's two memory forms (short-circuit box tests and interleaved gathers, or the
+  batched forms the FP32-cell marches use) give bit-identical images and counts;
+* the FP64 march against the reference's FP64 build (the golden fixtures) per pixel, and against
+  the oracle on the same pixels for the step / segment / drop counts;
+* the FP32 and mixed-precision marches against the reference's FP64 build with the parity gates;
+* escaping rays ended at rt_outer change nothing, and with rt_outer inside the photon orbits the
+  early end is not taken.
+
+The kernel runs the same Lane code; this exercises it without a GPU, one lane at a time.
+"""
+import numpy as np
+import pytest
+
+from phoenix_amd import capi, configs, snapshot as S
+from tests import parity
+from tests.cpu import lane
+from tests.golden import make_golden as G
+from tests.test_oracle_golden import params_from_overrides
+
+CASES = ["c1_i60_48", "c1_i163_48", "c1_nlev1_32", "c1sfc0_i30_40", "c2_i163_8f_40", "c5a05_i60_40_4f"]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def built():
+    lane.build()
+
+
+_cache = {}
+
+
+def _setup(case, snapdir, tmp_path, oracle_lib):
+    if case not in _cache:
+        preset, over = G.CASES[case]
+        dump, grid, _ = S.ensure_snapshot(preset, snapdir)
+        run = capi.Run(configs.write_config(str(tmp_path / f"{case}.toml"), dump, grid, **over))
+        o = oracle_lib.Oracle(dump, grid, 64, **params_from_overrides(over))
+        prim = o.tables()[0]
+        _cache[case] = (run, o, prim)
+    return _cache[case]
+
+
+def _pixels(run, stride):
+    n = run.model.img_width * run.model.img_height
+    return np.arange(n // (2 * stride), n, stride, dtype=np.int32)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_fp64_forms_identical_and_match_reference(case, snapdir, tmp_path, oracle_lib):
+    run, o, prim = _setup(case, snapdir, tmp_path, oracle_lib)
+    pix = _pixels(run, 11)
+    r0, s0 = lane.run(run.model, prim, capi.FP64, pix, batched=0, sanitized=True)
+    r1, s1 = lane.run(run.model, prim, capi.FP64, pix, batched=1, sanitized=True)
+    assert np.array_equal(r0, r1) and np.array_equal(s0, s1), "the FP64 memory forms differ"
+    ref, _, _ = G.load(case, 64)
+    ref = ref[pix]
+    m = ref[..., 0] != 0
+    rel = np.abs(r1[..., 0][m] - ref[..., 0][m]) / np.abs(ref[..., 0][m])
+    quv = np.max(np.abs(r1[..., 1:4] - ref[..., 1:4])[m] / np.abs(ref[..., 0][m])[:, None], axis=1)
+    print(f"\n{case}: {pix.size} pixels, FP64 lane vs reference FP64: max |dI|/I {rel.max():.2e}, "
+          f"|dQUV|/I {quv.max():.2e}; counts {s1[:3].tolist()}")
+    assert np.array_equal(ref[..., 0] == 0, r1[..., 0] == 0)
+    assert rel.max() <= 1e-9 and quv.max() <= 1e-9
+    _, st = o.render(pixels=pix.astype(np.int64), threads=8)
+    assert np.all(np.abs(s1[:3] - st) <= 1), (s1[:3], st)
+
+
+@pytest.mark.parametrize("precision", [capi.FP32, capi.MIXED])
+@pytest.mark.parametrize("case", CASES)
+def test_fp32_and_mixed_lanes_within_gates(case, precision, snapdir, tmp_path, oracle_lib):
+    run, _, prim = _setup(case, snapdir, tmp_path, oracle_lib)
+    pix = _pixels(run, 7)
+    res, st = lane.run(run.model, prim, precision, pix, sanitized=True)
+    ref, _, _ = G.load(case, 64)
+    figs = parity.compare(res, ref[pix], preset=G.CASES[case][0])
+    print(f"\n{case} fp{precision}: {parity.summary(figs[:2])}; counts {st.tolist()}")
+    assert parity.passes(figs), parity.summary(figs)
+    if precision == capi.MIXED:
+        assert st[3] > 0  # steps near the horizon or the poles ran in FP64
+
+
+@pytest.mark.parametrize("precision", [capi.FP32, capi.FP64, capi.MIXED])
+def test_escape_changes_nothing_and_is_off_inside_the_photon_orbits(precision, snapdir, tmp_path, oracle_lib):
+    run, _, prim = _setup("c1_i163_48", snapdir, tmp_path, oracle_lib)
+    pix = _pixels(run, 5)
+    full, s_full = lane.run(run.model, prim, precision, pix, escape=1)
+    cut, s_cut = lane.run(run.model, prim, precision, pix, escape=2)
+    assert np.array_equal(full, cut) and s_cut[0] < 0.9 * s_full[0]
+    assert np.array_equal(s_full[1:3], s_cut[1:3])
+    # rt_outer = 3 lies inside the retrograde photon orbit (3.93 at spin 0.9375): rays moving outward
+    # there may still turn back, so escape 2 must march them on exactly as escape 1 does
+    rq = run.request(rt_outer=3.0)
+    full3, s3 = lane.run(rq, prim, precision, pix, escape=1)
+    cut3, c3 = lane.run(rq, prim, precision, pix, escape=2)
+    assert np.array_equal(full3, cut3) and np.array_equal(s3, c3)

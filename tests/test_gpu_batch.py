@@ -64,7 +64,7 @@ def test_batch_mixed_frequency_counts(c1, precision):
     batch, _ = snap.render_batch(reqs, precision)
     for rq, rb in zip(reqs, batch):
         alone, _ = snap.render(rq, precision)
-        figs = parity.compare(rb, alone)
+        figs = parity.compare(rb, alone, preset="c1")
         exact = np.array_equal(rb, alone)
         print(f"\n{rq.img_width}x{rq.img_height} nf={rq.nfreqs} fp{precision}: exact={exact}; {parity.summary(figs)}")
         assert parity.passes(figs), parity.summary(figs)
@@ -77,7 +77,7 @@ def test_batch_vs_fp64_judge(c1):
     mixed, _ = snap.render_batch(reqs, capi.MIXED)
     judge, _ = snap.render_batch(reqs, capi.FP64)
     for rq, a, b in zip(reqs, mixed, judge):
-        figs = parity.compare(a, b)
+        figs = parity.compare(a, b, preset="c1")
         print(f"\n{rq.img_width}x{rq.img_height} i={rq.inclination} nf={rq.nfreqs}: {parity.summary(figs)}")
         assert parity.passes(figs), parity.summary(figs)
 
@@ -92,6 +92,6 @@ def test_autotune_picks_a_valid_configuration(c1):
     assert rate > 0
     again, _ = snap.render_batch(reqs, capi.FP32)
     for rq, a, b in zip(reqs, again, ref):
-        figs = parity.compare(a, b)
+        figs = parity.compare(a, b, preset="c1")
         assert parity.passes(figs), parity.summary(figs)
     snap.set_tune()  # back to the defaults for later tests

@@ -60,7 +60,7 @@ def test_golden_case(case, precision, snapdir, tmp_path):
     ref64, st64, meta = G.load(case, 64)
     ref_same, st_same, _ = G.load(case, 64 if precision == capi.FP64 else 32)
     res, inf = run.dev_render(precision)
-    figs = parity.compare(res, ref64)
+    figs = parity.compare(res, ref64, case=case)
     print(f"\n{case} fp{precision}: {parity.summary(figs)}; steps {inf['steps']} ref {st_same[0]}; "
           f"segments {inf['nplasma']} ref {st_same[1]}; bad {inf['bad']} ref {st_same[2]}; "
           f"kernel {inf['kernel_s'] * 1e3:.2f} ms, vgprs {inf['vgprs']} scratch {inf['scratch_bytes']}")
@@ -87,7 +87,7 @@ def test_frequency_chunks_vs_oracle(nf, snapdir, tmp_path, oracle_lib):
     o.close()
     for precision in (capi.FP64, capi.FP32, capi.MIXED):
         res, inf = run.dev_render(precision)
-        figs = parity.compare(res, ref)
+        figs = parity.compare(res, ref, preset="c1")
         print(f"\nnf={nf} fp{precision}: steps {inf['steps']} ref {st[0]} segs {inf['nplasma']} ref {st[1]} "
               f"bad {inf['bad']} ref {st[2]} vgprs {inf['vgprs']} scratch {inf['scratch_bytes']} "
               f"kernel {inf['kernel_s'] * 1e3:.2f} ms; {parity.summary(figs)}")
@@ -111,7 +111,7 @@ def test_c2_inclination60_vs_oracle(snapdir, tmp_path, oracle_lib):
     o.close()
     for precision in (capi.FP64, capi.FP32, capi.MIXED):
         res, inf = run.dev_render(precision)
-        figs = parity.compare(res, ref)
+        figs = parity.compare(res, ref, preset="c2")
         print(f"\nc2 i60 5freq fp{precision}: {parity.summary(figs)}")
         assert parity.passes(figs), parity.summary(figs)
 
@@ -178,7 +178,7 @@ def test_escaping_rays_end_at_rt_outer_with_the_same_results(c1_run, precision, 
                   f"kernel {i1['kernel_s'] * 1e3:.2f} -> {i2['kernel_s'] * 1e3:.2f} ms")
             assert i2["steps"] < 0.9 * i1["steps"]
             if fw == 1 and precision == capi.MIXED:
-                figs = parity.compare(cut, full)
+                figs = parity.compare(cut, full, preset="c1")
                 assert parity.passes(figs), parity.summary(figs)
                 continue
             assert np.array_equal(cut, full)
@@ -195,7 +195,7 @@ def test_distant_camera_mixed_matches_fp64(snapdir, tmp_path):
     run = capi.Run(cfg)
     ref, i64 = run.dev_render(capi.FP64)
     res, imx = run.dev_render(capi.MIXED)
-    figs = parity.compare(res, ref)
+    figs = parity.compare(res, ref, preset="c1")
     print(f"\nr_cam 1e11: mixed steps {imx['steps']} (FP64 {imx['fp64_steps']}), FP64 march steps {i64['steps']}; "
           f"{parity.summary(figs)}")
     assert np.all(np.isfinite(res))

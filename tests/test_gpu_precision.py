@@ -26,7 +26,7 @@ def test_precision_form_vs_fp64(snapdir, tmp_path, preset):
         lp0, cp0, ev0 = fractions(F64)
         for name, prec in (("fp32", capi.FP32), ("mixed", capi.MIXED)):
             res, info = snap.render(rq, prec)
-            figs = parity.compare(res, judge)
+            figs = parity.compare(res, judge, preset=preset)
             F = run.flux(res, rq)
             lp, cp, ev = fractions(F)
             dev = np.abs((ev - ev0 + 90.0) % 180.0 - 90.0)

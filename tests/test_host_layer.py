@@ -107,7 +107,7 @@ def _write_case(tmp_path, name, over):
     return str(p)
 
 
-@pytest.mark.skipif(not os.path.exists(REF64), reason="reference harness not built (no /root/reference)")
+@pytest.mark.skipif(not os.path.exists(REF64), reason="oracle/_ref/phx_ref_fp64 not built (make -C oracle ref)")
 @pytest.mark.parametrize("name", sorted(CONFIG_CASES))
 def test_config_matches_reference_parser(tmp_path, name):
     over, ovs = CONFIG_CASES[name]
@@ -210,7 +210,7 @@ def _fake_results(m, seed=3):
     return res
 
 
-@pytest.mark.skipif(not os.path.exists(REF_OUT), reason="reference output harness not built (no /root/reference)")
+@pytest.mark.skipif(not os.path.exists(REF_OUT), reason="oracle/_ref/phx_ref_output_fp64 not built (make -C oracle ref_output)")
 @pytest.mark.parametrize("nf", [1, 3])
 def test_writers_match_reference_writers(tmp_path, snapdir, nf):
     dump, grid, _ = S.ensure_snapshot("tiny", snapdir)

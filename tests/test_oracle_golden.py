@@ -43,6 +43,10 @@ def params_from_overrides(over):
         sec, key = k.split("__", 1)
         if key == "frequency_list":
             p["freqs"] = tuple(float(x) for x in v.strip("[]").split(","))
+        elif key == "frequency":
+            p["freqs"] = (float(v),)
+        elif key == "sfc":
+            p["sfc"] = int(v == "true")
         elif key in ("img_width", "img_height", "nlevels"):
             p[names[key]] = int(float(v))
         else:

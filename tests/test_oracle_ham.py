@@ -124,9 +124,12 @@ def test_twin_matches_reference_fp64(case, snapdir, oracle_lib):
     o = O.Oracle(dump, grid, 64, **params_from_overrides(over))
     res, st = o.render_ham(threads=8)
     o.close()
-    figs = parity.compare(res, ref)
+    figs = parity.compare(res, ref, case=case)
     print(f"\n{case}: {parity.summary(figs)}; stats {st.tolist()} ref {st_ref.tolist()}")
     assert parity.passes(figs), parity.summary(figs)
+    # far inside the path's 1e-4 flux gate: the twin carries the polarization by the exact Walker-Penrose
+    # constant where the reference integrates F1 with RK4, so Q/U/V differ by that integration error
+    # (at most 1.1e-7 of I, on c5_i120_32_2f at 460 GHz)
     for d in figs:
-        assert d["flux_dI"] <= 1e-7 and d["flux_dQUV"] <= 1e-7 and d["rel_I_p999"] <= 2e-5
+        assert d["flux_dI"] <= 1e-7 and d["flux_dQUV"] <= 3e-7 and d["rel_I_p999"] <= 2e-5
     assert np.all(np.abs(st - st_ref) <= 1e-5 * st_ref + 1)
