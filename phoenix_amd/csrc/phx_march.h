@@ -67,6 +67,22 @@ PHX_DEV float div_rcp(float a, float b) {
   return a * hw_rcp(b);
 }
 PHX_DEV double div_rcp(double a, double b) { return a / b; }
+// 1 / x and a / b to about half an ulp in FP32: the hardware reciprocal (1 ulp) refined by one
+// Newton step, and the quotient refined by its residual.  The geodesic's right-hand side and step
+// rule use these: with the library's fast FP32 division (about 2.5 ulp) the errors of the
+// connection's reciprocals reach the ray's trajectory, and photon-ring pixels of the FP32 march
+// then leave the per-pixel gate several times as often as the reference's FP32 build (whose
+// divisions are correctly rounded).  FP64 divides.
+PHX_DEV float recip_nr(float x) {
+  const float r = hw_rcp(x);
+  return fmaf(fmaf(-x, r, 1.f), r, r);
+}
+PHX_DEV double recip_nr(double x) { return 1. / x; }
+PHX_DEV float div_nr(float a, float b) {
+  const float r = recip_nr(b), q = a * r;
+  return fmaf(fmaf(-b, q, a), r, q);
+}
+PHX_DEV double div_nr(double a, double b) { return a / b; }
 
 PHX_DEV void scs(float x, float* s, float* c) { sincosf(x, s, c); }
 PHX_DEV void scs(double x, double* s, double* c) { sincos(x, s, c); }
@@ -339,7 +355,7 @@ PHX_DEV Geo<T> geo_at(T a, T hs, T x1, T x2) {
   theta_trig_sel<ONE_SINCOS>(x2, hs, &s2x, &c2x, &s, &c);
   const T tp = T(1) + hs * c2x, tpp = T(-2) * hs * s2x;
   const T ss = s * s, sc = s * c, a2c2 = a2 * c * c;
-  const T Sig = r2 + a2c2, iS = T(1) / Sig;
+  const T Sig = r2 + a2c2, iS = recip_nr(Sig);
   const T z = T(2) * r * iS;
   const T z1 = z * (a2c2 - r2) * iS, zz1 = T(2) * z * a2c2 * iS;  // dz/dx1 and z + dz/dx1
   const T S2 = T(-2) * a2 * sc * tp, q2 = T(2) * sc * tp;
@@ -359,7 +375,7 @@ PHX_DEV Geo<T> geo_at(T a, T hs, T x1, T x2) {
   G.B.m13 = -a * r * (q2 * opz + ss * z2);
   G.B.m22 = tp * (S2 * tp + T(2) * Sig * tpp);
   G.B.m33 = q2 * (Sig + T(2) * a2 * ss * opz) + ss * (S2 + a2 * ss * z2);
-  const T ir = T(1) / r, itp = T(1) / tp, is = T(1) / s;
+  const T ir = recip_nr(r), itp = recip_nr(tp), is = recip_nr(s);
   G.u00 = -opz;
   G.u01 = T(2) * iS;
   G.u11 = iS * (r2 - T(2) * r + a2) * ir * ir;
@@ -623,17 +639,17 @@ template <class T>
 PHX_DEV T step_len(T stepsize, T x2, const T k[4], bool have_dxc, const T dxc[3]) {
   const T S2 = Lim<T>::tiny;
   const T ak1 = fabs(k[1]) + S2, ak2 = fabs(k[2]) + S2, ak3 = fabs(k[3]) + S2;
-  const T d1 = stepsize / ak1;
-  const T d2 = stepsize * fmin(x2, T(PI_D) - x2) / ak2;
-  const T d3 = stepsize / ak3;
-  const T i1 = T(1) / (fabs(d1) + S2), i2 = T(1) / (fabs(d2) + S2), i3 = T(1) / (fabs(d3) + S2);
+  const T d1 = div_nr(stepsize, ak1);
+  const T d2 = div_nr(stepsize * fmin(x2, T(PI_D) - x2), ak2);
+  const T d3 = div_nr(stepsize, ak3);
+  const T i1 = recip_nr(fabs(d1) + S2), i2 = recip_nr(fabs(d2) + S2), i3 = recip_nr(fabs(d3) + S2);
   T grid = Lim<T>::big;
   if (have_dxc) {
-    const T s1 = dxc[0] / ak1, s2 = dxc[1] / ak2, s3 = dxc[2] / ak3;
-    const T j1 = T(1) / (fabs(s1) + S2), j2 = T(1) / (fabs(s2) + S2), j3 = T(1) / (fabs(s3) + S2);
-    grid = T(1) / (j1 + j2 + j3) / T(4);
+    const T s1 = div_nr(dxc[0], ak1), s2 = div_nr(dxc[1], ak2), s3 = div_nr(dxc[2], ak3);
+    const T j1 = recip_nr(fabs(s1) + S2), j2 = recip_nr(fabs(s2) + S2), j3 = recip_nr(fabs(s3) + S2);
+    grid = div_nr(recip_nr(j1 + j2 + j3), T(4));
   }
-  return -fmin(T(1) / (i1 + i2 + i3), grid);
+  return -fmin(recip_nr(i1 + i2 + i3), grid);
 }
 // The same rule for the mixed march, with the stepsize's and the cell sizes' reciprocals
 // (inv_dxc) in place of the divisions by them: 1/|stepsize/k| = |k|/stepsize and 1/|dxc/k| =

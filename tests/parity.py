@@ -9,9 +9,11 @@ on which a ray ends (pixel.hpp:193-197), and the pixel then differs at the 1e-3 
 whatever the precision.  The reference's own FP32 build misses the per-pixel gate on such pixels
 against its FP64 build; tests/golden/outlier_rates.json holds that rate, measured with
 oracle/_ref at 512 x 512 pixels on each snapshot family (8 frequencies and three inclinations for
-c2 and c2_a05).  An image may miss the per-pixel gate on at most 1.5 times the reference's rate of
-its snapshot family, and never on fewer pixels than the reference's FP32 build misses on the same
-golden case (when one is named).
+c2 and c2_a05).  An image may miss the per-pixel gate on as many pixels as a renderer with 1.5 times
+the reference's per-pixel outlier probability for its snapshot family would exceed in fewer than
+1 of 1000 images (the 99.9% quantile of a Poisson count with mean 1.5 x rate x gated pixels: the
+test images hold a few thousand gated pixels, where the expected count is of order one), and
+never fewer than the reference's FP32 build misses on the same golden case (when one is named).
 """
 from __future__ import annotations
 
@@ -56,6 +58,13 @@ def rate_for(preset: str | None) -> float:
     return rates[FAMILY.get(preset, preset)]
 
 
+def poisson_quantile(mean: float, q: float = 0.999) -> int:
+    """Smallest k with P(X <= k) >= q for X ~ Poisson(mean)."""
+    from scipy.stats import poisson
+
+    return int(poisson.ppf(q, mean)) if mean > 0 else 0
+
+
 def _case_floor(case: str | None, nf: int) -> list:
     """Pixels the reference's FP32 build misses the gate on in the named golden case, per frequency."""
     if case is None:
@@ -94,7 +103,7 @@ def compare(res: np.ndarray, ref: np.ndarray, pix_tol=PIX_TOL, flux_tol=FLUX_TOL
         dI = abs(Fa[0] - Fb[0]) / abs(Fb[0]) if Fb[0] != 0 else abs(Fa[0])
         dQUV = float(np.max(np.abs(Fa[1:] - Fb[1:]))) / abs(Fb[0]) if Fb[0] != 0 else float(np.max(np.abs(Fa[1:])))
         gated = int(m.sum())
-        allowed = max(math.ceil(RATE_FACTOR * rate * gated), cfloor[f]) if allowance else 0
+        allowed = max(poisson_quantile(RATE_FACTOR * rate * gated), cfloor[f]) if allowance else 0
         out.append({
             "f": f, "gated": gated, "nbad": nbad, "allowed": allowed,
             "rel_I_max": float(rel.max()) if rel.size else 0.0,
