@@ -202,6 +202,21 @@ int phx_snapshot_result(phx_snapshot* snap, phx_freqout* res, int64_t nres);
 int phx_snapshot_render_batch(phx_snapshot* snap, const phx_model* const* requests, int32_t n, int32_t precision,
                               phx_freqout* const* res, phx_info* info);
 
+/* Allocation-free render.  No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Marches every pixel of `request` (img_width x img_height x nfreqs) on the resident snapshot and
+ * writes the results to d_res, a device buffer the caller owns (res[(i * img_height + j) * nfreqs
+ * + f], as phx_snapshot_render), then waits for the march.  The per-render buffers (jobs, ray
+ * start records, tasks, cameras, counters) live in a fixed 104 MiB array in device memory that
+ * each GPU the library runs on holds until the process ends; the call allocates nothing.
+ * Limits, refused with PHX_E_RANGE: img_width * img_height * ceil(nfreqs / 4) at most 1,048,576
+ * jobs in PHX_FP32 and PHX_MIXED, 545,259 in PHX_FP64; and a request whose march kernel would use
+ * per-lane scratch memory (which the HIP runtime allocates at launch): at present more than one
+ * frequency in PHX_MIXED or more than two in PHX_FP32 / PHX_FP64.  Calls on one GPU share the
+ * array, so the library serializes them.  Samples are processed in place (phx_tune.queue 0). */
+int phx_snapshot_render_noalloc(phx_snapshot* snap, const phx_model* request, int32_t precision,
+                                phx_freqout* d_res, phx_info* info);
+
 /* launch configuration of batch renders */
 typedef struct phx_tune {
   int32_t queue;         /* -1 default (in place), 0 samples processed in place, 1 per-wave LDS sample queue */

@@ -146,6 +146,8 @@ SIGNATURES = {
     "phx_server_reset_stats": (c_int, [ctypes.c_void_p]),
     "phx_server_stop": (c_int, [ctypes.c_void_p]),
     "phx_snapshot_result": (c_int, [ctypes.c_void_p, ctypes.POINTER(FreqOut), c_int64]),
+    "phx_snapshot_render_noalloc": (c_int, [ctypes.c_void_p, ctypes.POINTER(Model), c_int, ctypes.c_void_p,
+                                            ctypes.POINTER(Info)]),
     "phx_snapshot_prims": (c_int, [ctypes.c_void_p, c_int, P_dbl, c_int64]),
 }
 
@@ -357,6 +359,13 @@ class Snapshot:
         t = Tune(queue, blocks_per_cu, order, fp64_wave, refill, escape)
         check(lib().phx_snapshot_set_tune(self.h, ctypes.byref(t)))
         self._tune = t.as_dict()
+
+    def render_noalloc(self, request: Model, precision: int, d_res: int) -> dict:
+        """phx_snapshot_render_noalloc into the device buffer at address d_res; returns the info."""
+        info = Info()
+        check(lib().phx_snapshot_render_noalloc(self.h, ctypes.byref(request), precision, ctypes.c_void_p(d_res),
+                                                ctypes.byref(info)))
+        return info.as_dict()
 
     def prims(self, table: int) -> np.ndarray:
         m = self.run.model

@@ -389,6 +389,13 @@ __global__ void k_reset(int* counter, unsigned long long* tot) {
   if (tot && threadIdx.x < 4) tot[threadIdx.x] = 0ull;
 }
 
+// one value to device memory (a launch's side arguments): a kernel rather than a copy from host
+// memory, so that no host staging is involved and a launch sequence stays capturable
+template <class V>
+__global__ void k_store(const V v, V* dst) {
+  *dst = v;
+}
+
 // Per leaf: the largest rho if every cell is field-free (B^i = 0), has rho >= 0 and finite
 // primitives, else -1.  One workgroup per leaf.
 template <class T>
@@ -805,8 +812,7 @@ static void launch_march(phx_snapshot* s, Slot& sl, const LaunchInfo& li, int j0
   const LaunchBufs bufs{sl.tasks.p, (const int2*)sl.jobs.p, sl.start.p, sl.counter, sl.tot,
                         ensure(sl.side, sizeof side)};
   fill_args<M>(A, side, grid_of<T>(s, tables_of<T>(s)), GL, s->geo, li.uni, tn, bufs, j0, j1);
-  // from pageable host memory: the call returns once the source has been staged
-  HIPCHECK(hipMemcpyAsync(const_cast<void*>(bufs.side), &side, sizeof side, hipMemcpyHostToDevice, s->stream));
+  k_store<SideArgs<M>><<<1, 1, 0, s->stream>>>(side, static_cast<SideArgs<M>*>(const_cast<void*>(bufs.side)));
   const void* kfn = reinterpret_cast<const void*>(&k_march<M, NF, Q>);
   int cus = 0, per_cu = 0;
   HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
@@ -822,10 +828,11 @@ static void launch_march(phx_snapshot* s, Slot& sl, const LaunchInfo& li, int j0
   // limit to what the kernel needs (the reference sets it explicitly too, kernel.cu:305)
   size_t limit = 0;
   HIPCHECK(hipDeviceGetLimit(&limit, hipLimitStackSize));
+  if (li.no_alloc && fa.localSizeBytes > 0)  // the runtime allocates private-segment memory at launch
+    throw Error(PHX_E_RANGE, "the march kernel of this request (" + std::to_string(li.nfmax) +
+                                 " frequencies per lane) uses " + std::to_string(fa.localSizeBytes) +
+                                 " bytes of per-lane scratch, which the allocation-free render cannot use");
   if (fa.localSizeBytes > limit) {
-    if (li.no_alloc)
-      throw Error(PHX_E_RANGE, "the march kernel needs " + std::to_string(fa.localSizeBytes) +
-                                   " bytes of stack per lane, above the device limit");
     HIPCHECK(hipDeviceSetLimit(hipLimitStackSize, (fa.localSizeBytes + 255) & ~size_t(255)));
   }
   k_reset<<<1, 64, 0, s->stream>>>(sl.counter, nullptr);
@@ -1218,6 +1225,169 @@ static void autotune(phx_snapshot* s, const phx_model* const* rqs, int n, int pr
   if (best_rate) *best_rate = rate_best;
 }
 
+// ============================================================================ allocation-free render
+// For callers that cannot allocate device memory (code under HIP graph capture, tight serving
+// loops, harnesses that forbid it): the per-render buffers (job list, ray start records, tasks,
+// cameras, side arguments, counters) live in `scratch`, a fixed array in device memory that each
+// GPU the library runs on holds from module load until the process ends; the caller passes the
+// output.  Its device address comes from hipGetSymbolAddress, once per device, at the first call
+// on that device (in host code the array's name is not a device address).  Calls on one device
+// share the array; the library serializes them (they would otherwise race on it), and the
+// counters and statistics are cleared by kernels at every call.
+constexpr size_t NOALLOC_HEAD = size_t(64) << 10;       // counters, side arguments, tasks, cameras
+constexpr int NOALLOC_MAX_JOBS = 1 << 20;                // FP32 / mixed; FP64 start records are twice as large
+constexpr size_t NOALLOC_JOB_BYTES = sizeof(int2) + 96;  // a job and its FP32 / mixed start record
+constexpr size_t NOALLOC_BYTES = NOALLOC_HEAD + (size_t)NOALLOC_MAX_JOBS * NOALLOC_JOB_BYTES;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+struct NoAllocDev {
+  std::mutex mu;        // one allocation-free render at a time on this device
+  void* ptr = nullptr;  // device address of `scratch` on this device
+};
+static NoAllocDev g_noalloc[64];
+static std::mutex g_noalloc_init;
+
+static void* scratch_on(int device) {
+  std::lock_guard<std::mutex> lk(g_noalloc_init);
+  NoAllocDev& d = g_noalloc[device];
+  if (!d.ptr) {
+    HIPCHECK(hipSetDevice(device));
+    void* p = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(PHX_E_DEVICE, "no device address for the allocation-free render's array");
+    d.ptr = p;
+  }
+  return d.ptr;
+}
+
+// what the setup kernel writes into the array: the request's tasks and cameras, and the job list
+// (every pixel of the image for every task, in the Morton order of the reference's GPU schedule,
+// jobs.hpp:18-46: pixels sorted by the key with i in the even bits and j in the odd bits)
+template <class T>
+struct NoAllocSetup {
+  TaskDev<T> tasks[PHX_MAX_FREQS / CHUNK];
+  Cam cams[PHX_MAX_FREQS / CHUNK];
+  int ntasks, npix, W, H, levels;  // levels: log2 of the power-of-two square that holds the image
+};
+// the n-th pixel (i, j) of a W x H image in Morton order, by descending the quadtree of the
+// enclosing 2^levels square: at each level the four children in key order (i bit low, j bit high)
+// hold the pixels of the image that fall in them, counted in closed form
+__device__ inline int2 morton_nth(unsigned n, int W, int H, int levels) {
+  int i0 = 0, j0 = 0;
+  for (int l = levels - 1; l >= 0; l--) {
+    const int s = 1 << l;
+    for (int c = 0; c < 4; c++) {
+      const int ii = i0 + (c & 1) * s, jj = j0 + (c >> 1) * s;
+      const unsigned cnt = (unsigned)max(0, min(W, ii + s) - ii) * (unsigned)max(0, min(H, jj + s) - jj);
+      if (n < cnt) {
+        i0 = ii;
+        j0 = jj;
+        break;
+      }
+      n -= cnt;
+    }
+  }
+  return make_int2(i0, j0);
+}
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_noalloc_setup(const NoAllocSetup<T> a, TaskDev<T>* __restrict__ tasks,
+                                                         Cam* __restrict__ cams, int2* __restrict__ jobs) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < a.ntasks) {
+    tasks[n] = a.tasks[n];
+    cams[n] = a.cams[n];
+  }
+  const int njobs = a.npix * a.ntasks;
+  for (int j = n; j < njobs; j += gridDim.x * blockDim.x) {
+    const int t = j / a.npix, k = j - t * a.npix;
+    const int2 ij = morton_nth((unsigned)k, a.W, a.H, a.levels);
+    jobs[j] = make_int2(t, ij.x * a.H + ij.y);
+  }
+}
+
+template <class T>
+static void render_noalloc_t(phx_snapshot* s, const phx_model* rq, int precision, phx_freqout* d_res,
+                             phx_info* info) {
+  const int64_t npix = (int64_t)rq->img_width * rq->img_height;
+  const int ntasks = (rq->nfreqs + CHUNK - 1) / CHUNK;
+  const int64_t njobs = npix * ntasks;
+  const size_t rec = precision == PHX_FP64 ? StRef<double>::REC * sizeof(double)
+                     : precision == PHX_MIXED ? StHam::REC * sizeof(double)
+                                              : StRef<float>::REC * sizeof(float);
+  const int64_t max_jobs = (int64_t)((NOALLOC_BYTES - NOALLOC_HEAD) / (sizeof(int2) + rec));
+  if (njobs > max_jobs)
+    throw Error(PHX_E_RANGE, "allocation-free render: " + std::to_string(njobs) + " jobs (pixels x frequency "
+                                 "chunks of 4) above its maximum of " + std::to_string(max_jobs) + " for this precision");
+  char* base = static_cast<char*>(scratch_on(s->device));
+  // the array's layout: counter and statistics, side arguments, tasks, cameras; then jobs and start records
+  Slot sl;
+  sl.counter = reinterpret_cast<int*>(base);
+  sl.tot = reinterpret_cast<unsigned long long*>(base + 256);
+  auto window = [](DevBuf& b, char* p, size_t bytes) {
+    b.p = p;
+    b.cap = bytes;
+    b.fixed = true;
+  };
+  window(sl.side, base + 1024, 8192);
+  window(sl.tasks, base + 9216, 16384);
+  window(sl.cams, base + 25600, 16384);
+  window(sl.jobs, base + NOALLOC_HEAD, (size_t)max_jobs * sizeof(int2));
+  window(sl.start, base + NOALLOC_HEAD + (size_t)max_jobs * sizeof(int2), (size_t)max_jobs * rec);
+  sl.e0 = s->slot.e0;
+  sl.e1 = s->slot.e1;
+  static_assert(sizeof(TaskDev<double>) * (PHX_MAX_FREQS / CHUNK) <= 16384, "tasks window");
+  static_assert(sizeof(Cam) * (PHX_MAX_FREQS / CHUNK) <= 16384, "cameras window");
+  NoAllocSetup<T> a;
+  std::memset(&a, 0, sizeof a);
+  std::vector<TaskDev<T>> tasks;
+  add_tasks<T>(rq, d_res, tasks);
+  const Cam cam = cam_of(s->geo, rq);
+  for (int t = 0; t < ntasks; t++) {
+    a.tasks[t] = tasks[t];
+    a.cams[t] = cam;
+  }
+  a.ntasks = ntasks;
+  a.npix = (int)npix;
+  a.W = rq->img_width;
+  a.H = rq->img_height;
+  a.levels = 0;
+  while ((1 << a.levels) < std::max(a.W, a.H)) a.levels++;
+  const int grid = (int)std::min<int64_t>(4096, (njobs + BLOCK - 1) / BLOCK);
+  k_noalloc_setup<T><<<std::max(grid, 1), BLOCK, 0, s->stream>>>(a, static_cast<TaskDev<T>*>(sl.tasks.p),
+                                                                  static_cast<Cam*>(sl.cams.p),
+                                                                  static_cast<int2*>(sl.jobs.p));
+  HIPCHECK(hipGetLastError());
+  LaunchInfo li;
+  li.njobs = (int)njobs;
+  li.ntasks = ntasks;
+  li.nfmax = std::min(CHUNK, rq->nfreqs) <= 1 ? 1 : std::min(CHUNK, rq->nfreqs) <= 2 ? 2 : 4;
+  li.uni = rq;
+  li.no_alloc = true;
+  Tune tn = s->tune;
+  tn.queue = 0;  // samples in place: the LDS-queue kernels carry per-lane scratch with several frequencies
+  phx_info local;
+  if (!info) info = &local;
+  std::memset(info, 0, sizeof *info);
+  fill_info_device(s, info);
+  enqueue_precision(s, sl, precision, li, tn, info, nullptr, nullptr);
+  finish_march(s, sl, li.njobs, info);
+  info->rays = npix * rq->nfreqs;
+}
+
+static void render_noalloc(phx_snapshot* s, const phx_model* rq, int precision, phx_freqout* d_res, phx_info* info) {
+  check_precision(s, precision);
+  check_request(rq);
+  if (!d_res) throw Error(PHX_E_ARG, "d_res must be a device buffer of img_width * img_height * nfreqs results");
+  if (s->device < 0 || s->device >= 64) throw Error(PHX_E_RANGE, "device index above 63");
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::lock_guard<std::mutex> dk(g_noalloc[s->device].mu);
+  HIPCHECK(hipSetDevice(s->device));
+  if (precision == PHX_FP64)
+    render_noalloc_t<double>(s, rq, precision, d_res, info);
+  else
+    render_noalloc_t<float>(s, rq, precision, d_res, info);
+}
+
 // ============================================================================ serving loop
 // A worker thread keeps the snapshots resident and serves render requests with continuous
 // batching.  A request is admitted when its result block fits the HBM budget (a block of the
@@ -1598,6 +1768,14 @@ int phx_autotune(phx_snapshot* s, const phx_model* const* requests, int32_t n, i
   return guarded([&] {
     if (!s || !requests) throw Error(PHX_E_ARG, "null snapshot or request list");
     autotune(s, requests, n, precision, sample_stride, best, rate);
+  });
+}
+
+int phx_snapshot_render_noalloc(phx_snapshot* s, const phx_model* rq, int32_t precision, phx_freqout* d_res,
+                                phx_info* info) {
+  return guarded([&] {
+    if (!s) throw Error(PHX_E_ARG, "null snapshot");
+    render_noalloc(s, rq, precision, d_res, info);
   });
 }
 
