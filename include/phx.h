@@ -50,7 +50,8 @@ typedef struct phx_info {
   int32_t vgprs, sgprs, scratch_bytes; /* per-lane registers and private-segment bytes of the march kernel */
   int32_t lds_bytes, waves_per_cu;     /* static LDS per workgroup; occupancy the launch assumed */
   double upload_s, kernel_s, decode_s; /* table upload, march kernels (HIP events), device decode */
-  int64_t steps, nplasma, bad;         /* sums over all rays (src/core/render/pixel.hpp:34-37) */
+  int64_t steps, nplasma, bad;         /* sums over all rays (src/core/render/pixel.hpp:34-37); PHX_MIXED
+                                          leaves out the samples in field-free leaves (all cut or dropped) */
   int64_t rays;                        /* rays x frequencies rendered */
   int64_t fp64_steps;                  /* PHX_MIXED: geodesic steps taken in FP64 (near the horizon / poles) */
 } phx_info;
@@ -192,6 +193,12 @@ int phx_snapshot_render(phx_snapshot* snap, const phx_model* request, int32_t pr
                         int32_t njobs, phx_freqout* res, phx_info* info);
 /* copies the device result of the last phx_snapshot_render into res */
 int phx_snapshot_result(phx_snapshot* snap, phx_freqout* res, int64_t nres);
+/* The flux of every frequency of the last phx_snapshot_render, reduced on the device:
+ * spec[f][4] = I, Q, U, V in Jy, the reference's compute_spec (src/host/io/output.hpp:43-53:
+ * 1e23 dOm times the sum of IQUV over the pixels whose I + Q + U + V is finite) summed in a fixed
+ * order on the device (equal to the host's sum to double rounding), so that a caller who needs
+ * only the spectrum copies 4 doubles per frequency instead of the image. */
+int phx_snapshot_flux(phx_snapshot* snap, double* spec, int32_t nfreqs);
 
 /* Batch render: the n requests (complete models, e.g. phx_model_derive'd copies of one run) are
  * marched by one persistent launch over a single job queue holding every pixel of every request
@@ -227,10 +234,17 @@ typedef struct phx_tune {
                             64 (or 0, the default) = the whole wave, 1 = each lane at once */
   int32_t escape;        /* 0 default by precision (PHX_MIXED 2, PHX_FP32 / PHX_FP64 1); 1 every ray marches to
                             r_out / cut_in / max_steps as the reference does; 2 a ray moving outward beyond
-                            rt_outer ends there (it cannot emit again: the same results, fewer steps) */
+                            rt_outer ends there (it cannot emit again: the same results, fewer steps; taken
+                            only when rt_outer lies outside the photon orbits) */
+  int32_t block;         /* workgroup size of the in-place march: 64 or 256 (0 = 256); the queue uses 256
+                            (the reference's compile-time GPU_BLOCK, CMakeLists.txt:25-28) */
+  int32_t fetch;         /* steps per fetch: a wave with rays in flight takes jobs for its idle lanes only every
+                            `fetch` passes (0 or 1 = every pass) */
 } phx_tune;
-/* Autotuner: times every configuration on every sample_stride-th wave of the batch's jobs,
- * keeps the fastest for the snapshot's batch renders, returns it and its sampled rays*freq/s.
+/* Autotuner: times every configuration (LDS queue or in place, refill threshold, steps per fetch,
+ * workgroup size, job order, whole-wave FP64 promotion) on every sample_stride-th wave of the
+ * batch's jobs, keeps the fastest for the snapshot's batch renders, returns it and its sampled
+ * rays*freq/s.
  * Replaces the reference's compile-time GPU_BLOCK / GPU_MINBLOCKS (kernel.cu:37-42). */
 int phx_autotune(phx_snapshot* snap, const phx_model* const* requests, int32_t n, int32_t precision,
                  int32_t sample_stride, phx_tune* best, double* rate);
@@ -267,6 +281,11 @@ int phx_server_start(phx_snapshot* const* snaps, int32_t nsnaps, const phx_serve
 /* queues a request on snapshot `snap`; res (caller-owned, img_width*img_height*nfreqs entries,
  * may be NULL) receives the result when the request completes */
 int phx_server_submit(phx_server* srv, int32_t snap, const phx_model* request, phx_freqout* res, int64_t* ticket);
+/* the same, and spec (caller-owned, nfreqs x 4 doubles, may be NULL) receives the request's flux
+ * per frequency, reduced on the device as phx_snapshot_flux does; with res NULL only the spectrum
+ * crosses to the host */
+int phx_server_submit_spec(phx_server* srv, int32_t snap, const phx_model* request, phx_freqout* res, double* spec,
+                           int64_t* ticket);
 /* waits up to timeout_s (< 0: forever) for a ticket; *done = 1 once its result is in res (the
  * ticket is then retired) */
 int phx_server_wait(phx_server* srv, int64_t ticket, double timeout_s, int32_t* done, double* latency_s);

@@ -76,7 +76,7 @@ class Config(ctypes.Structure):
 
 class Tune(ctypes.Structure):
     _fields_ = [("queue", c_int), ("blocks_per_cu", c_int), ("order", c_int), ("fp64_wave", c_int),
-                ("refill", c_int), ("escape", c_int)]
+                ("refill", c_int), ("escape", c_int), ("block", c_int), ("fetch", c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -146,6 +146,9 @@ SIGNATURES = {
     "phx_server_reset_stats": (c_int, [ctypes.c_void_p]),
     "phx_server_stop": (c_int, [ctypes.c_void_p]),
     "phx_snapshot_result": (c_int, [ctypes.c_void_p, ctypes.POINTER(FreqOut), c_int64]),
+    "phx_snapshot_flux": (c_int, [ctypes.c_void_p, P_dbl, c_int]),
+    "phx_server_submit_spec": (c_int, [ctypes.c_void_p, c_int, ctypes.POINTER(Model), ctypes.POINTER(FreqOut), P_dbl,
+                                       ctypes.POINTER(c_int64)]),
     "phx_snapshot_render_noalloc": (c_int, [ctypes.c_void_p, ctypes.POINTER(Model), c_int, ctypes.c_void_p,
                                             ctypes.POINTER(Info)]),
     "phx_snapshot_prims": (c_int, [ctypes.c_void_p, c_int, P_dbl, c_int64]),
@@ -279,7 +282,8 @@ class Snapshot:
         self.run = run
         self.h = ctypes.c_void_p()
         check(lib().phx_snapshot_open(ctypes.byref(run.model), run.config.dump_file, tables, ctypes.byref(self.h)))
-        self._tune = {"queue": -1, "blocks_per_cu": 0, "order": 0, "fp64_wave": 1, "refill": 0, "escape": 0}
+        self._tune = {"queue": -1, "blocks_per_cu": 0, "order": 0, "fp64_wave": 1, "refill": 0, "escape": 0,
+                      "block": 0, "fetch": 0}
 
     def close(self):
         if self.h:
@@ -355,10 +359,16 @@ class Snapshot:
         return t.as_dict(), rate.value
 
     def set_tune(self, queue: int = -1, blocks_per_cu: int = 0, order: int = 0, fp64_wave: int = 1, refill: int = 0,
-                 escape: int = 0):
-        t = Tune(queue, blocks_per_cu, order, fp64_wave, refill, escape)
+                 escape: int = 0, block: int = 0, fetch: int = 0):
+        t = Tune(queue, blocks_per_cu, order, fp64_wave, refill, escape, block, fetch)
         check(lib().phx_snapshot_set_tune(self.h, ctypes.byref(t)))
         self._tune = t.as_dict()
+
+    def flux(self, nfreqs: int) -> np.ndarray:
+        """phx_snapshot_flux: [nfreqs, 4] Jy of the last render, reduced on the device."""
+        spec = np.zeros((nfreqs, 4))
+        check(lib().phx_snapshot_flux(self.h, spec.ctypes.data_as(P_dbl), nfreqs))
+        return spec
 
     def render_noalloc(self, request: Model, precision: int, d_res: int) -> dict:
         """phx_snapshot_render_noalloc into the device buffer at address d_res; returns the info."""
@@ -386,25 +396,35 @@ class Server:
         check(lib().phx_server_start(arr, len(snapshots), ctypes.byref(opts), ctypes.byref(self.h)))
         self._keep = {}
 
-    def submit(self, snap: int, request: Model) -> int:
-        """Queues a request; returns its ticket.  The result array stays alive until wait()."""
+    def submit(self, snap: int, request: Model, image: bool = True, spec: bool = False) -> int:
+        """Queues a request; returns its ticket.  image: the result comes back to the host; spec: the
+        flux per frequency, reduced on the device (phx_server_submit_spec).  The arrays stay alive
+        until wait()."""
         n = request.img_width * request.img_height * request.nfreqs
-        arr, rp = _res_array(n)
+        arr, rp = _res_array(n) if image else (None, None)
+        sp = np.zeros((request.nfreqs, 4)) if spec else None
         req = Model()
         ctypes.pointer(req)[0] = request
         t = c_int64()
-        check(lib().phx_server_submit(self.h, snap, ctypes.byref(req), rp, ctypes.byref(t)))
-        self._keep[t.value] = (arr, req)
+        check(lib().phx_server_submit_spec(self.h, snap, ctypes.byref(req), rp,
+                                           sp.ctypes.data_as(P_dbl) if spec else None, ctypes.byref(t)))
+        self._keep[t.value] = (arr, req, sp)
         return t.value
 
     def wait(self, ticket: int, timeout: float = -1.0):
-        """(result [W*H, nf, 6], latency_s) once done, or (None, 0.0) on timeout."""
+        """(result [W*H, nf, 6] or None, latency_s) once done, or (None, 0.0) on timeout."""
+        res, _, lat = self.wait_spec(ticket, timeout)
+        return res, lat
+
+    def wait_spec(self, ticket: int, timeout: float = -1.0):
+        """(result or None, spectrum [nf, 4] or None, latency_s) once done, or (None, None, 0.0)."""
         done, lat = c_int(), c_double()
         check(lib().phx_server_wait(self.h, ticket, timeout, ctypes.byref(done), ctypes.byref(lat)))
         if not done.value:
-            return None, 0.0
-        arr, req = self._keep.pop(ticket)
-        return arr.reshape(req.img_width * req.img_height, req.nfreqs, 6), lat.value
+            return None, None, 0.0
+        arr, req, sp = self._keep.pop(ticket)
+        res = None if arr is None else arr.reshape(req.img_width * req.img_height, req.nfreqs, 6)
+        return res, sp, lat.value
 
     def stats(self) -> dict:
         st = ServeStats()

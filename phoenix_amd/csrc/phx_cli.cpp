@@ -9,6 +9,13 @@
 //                                  section.key=value overrides on top of the command line); all of
 //                                  them are rendered by one persistent launch on the resident snapshot
 //                                  and written to outdir/<line number>/
+//   --serve FILE                   one request per non-empty line of FILE: an arrival time in seconds
+//                                  after the start, then overrides as in --batch (a line may name another
+//                                  grmhd.dump_file / grid_file: each dump is loaded once and kept resident);
+//                                  the requests are submitted to the serving loop at their times, written
+//                                  to outdir/<line number>/ as they complete, and their latencies reported.
+//                                  PHX_SERVE_QUANTUM (jobs per launch), PHX_SERVE_ORDER (0 oldest first,
+//                                  1 least attained service) and PHX_SERVE_BUDGET (bytes) tune the loop
 #include <sys/stat.h>
 
 #include <atomic>
@@ -21,7 +28,9 @@
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
+#include <algorithm>
 
 #include "phx.h"
 
@@ -104,22 +113,150 @@ std::vector<std::vector<std::string>> read_batch(const char* path) {
   return out;
 }
 
+// The user fields of request k (config + overrides) on the grid and dump of `base`
+int request_on(const phx_model& base, const phx_model& mk, phx_model* out) {
+  phx_model r = base;
+  r.inclination = mk.inclination;
+  r.fov_x = mk.fov_x;
+  r.fov_y = mk.fov_y;
+  r.r_cam = mk.r_cam;
+  r.t_init = mk.t_init;
+  r.pa = mk.pa;
+  r.phi_cam = mk.phi_cam;
+  r.img_width = mk.img_width;
+  r.img_height = mk.img_height;
+  r.nfreqs = mk.nfreqs;
+  std::memcpy(r.frequencies, mk.frequencies, sizeof r.frequencies);
+  r.m_unit = mk.m_unit;
+  r.r_high = mk.r_high;
+  if (phx_model_derive(&r) != PHX_OK) return fail("request");
+  *out = r;
+  return EXIT_SUCCESS;
+}
+
+long env_long(const char* name, long dflt) {
+  const char* e = std::getenv(name);
+  return e ? std::atol(e) : dflt;
+}
+
+// --serve: the requests of FILE submitted to the serving loop at their arrival times
+int serve(const char* path, const std::string& cfgpath, const std::vector<const char*>& ov, const phx_config& cfg,
+          int precision, double t_start) {
+  const auto lines = read_batch(path);
+  if (lines.empty()) {
+    std::fprintf(stderr, "FAIL: empty serve file\n");
+    return EXIT_FAILURE;
+  }
+  struct Snap {
+    std::string dump;
+    phx_model model;
+    phx_snapshot* h = nullptr;
+  };
+  std::vector<Snap> snaps;
+  struct Req {
+    double at = 0.;
+    int snap = 0;
+    phx_model rq;
+    phx_config cfg;
+    std::vector<phx_freqout> res;
+    int64_t ticket = 0;
+  };
+  std::vector<Req> reqs(lines.size());
+  for (size_t k = 0; k < lines.size(); k++) {
+    char* end = nullptr;
+    reqs[k].at = std::strtod(lines[k][0].c_str(), &end);
+    if (end == lines[k][0].c_str() || *end || reqs[k].at < 0) {
+      std::fprintf(stderr, "FAIL: serve line %zu must start with an arrival time in seconds\n", k + 1);
+      return EXIT_FAILURE;
+    }
+    std::vector<const char*> ovk(ov);
+    for (size_t t = 1; t < lines[k].size(); t++) ovk.push_back(lines[k][t].c_str());
+    phx_model mk;
+    std::memset(&mk, 0, sizeof mk);
+    if (phx_load_config(cfgpath.c_str(), ovk.data(), (int32_t)ovk.size(), &reqs[k].cfg, &mk) != PHX_OK)
+      return fail("serve request");
+    size_t si = 0;
+    while (si < snaps.size() && snaps[si].dump != reqs[k].cfg.dump_file) si++;
+    if (si == snaps.size()) {  // a dump not yet resident: grid, tables, device decode
+      Snap sn;
+      sn.dump = reqs[k].cfg.dump_file;
+      sn.model = mk;
+      info("Load GRMHD data       \"%s\" (resident snapshot %zu)", sn.dump.c_str(), snaps.size());
+      if (phx_load_grid(reqs[k].cfg.grid_file, &sn.model) != PHX_OK) return fail("grid");
+      if (phx_load_dump(&sn.model, sn.dump.c_str()) != PHX_OK) return fail("dump");
+      if (phx_snapshot_open(&sn.model, sn.dump.c_str(), precision == PHX_FP64 ? PHX_TABLE_FP64 : PHX_TABLE_FP32,
+                            &sn.h) != PHX_OK)
+        return fail("snapshot");
+      snaps.push_back(sn);
+    }
+    reqs[k].snap = (int)si;
+    if (request_on(snaps[si].model, mk, &reqs[k].rq) != EXIT_SUCCESS) return EXIT_FAILURE;
+    reqs[k].res.resize((size_t)reqs[k].rq.img_width * reqs[k].rq.img_height * reqs[k].rq.nfreqs);
+    std::snprintf(reqs[k].cfg.outdir, sizeof reqs[k].cfg.outdir, "%s/%zu", cfg.outdir, k);
+  }
+  std::vector<phx_snapshot*> hs;
+  for (const Snap& sn : snaps) hs.push_back(sn.h);
+  phx_server_opts opts;
+  std::memset(&opts, 0, sizeof opts);
+  opts.precision = precision;
+  opts.quantum_jobs = (int32_t)env_long("PHX_SERVE_QUANTUM", 0);
+  opts.order = (int32_t)env_long("PHX_SERVE_ORDER", 0);
+  opts.hbm_budget = (int64_t)env_long("PHX_SERVE_BUDGET", 0);
+  phx_server* srv = nullptr;
+  if (phx_server_start(hs.data(), (int32_t)hs.size(), &opts, &srv) != PHX_OK) return fail("server");
+  std::vector<size_t> order(reqs.size());
+  for (size_t k = 0; k < order.size(); k++) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return reqs[x].at < reqs[y].at; });
+  const double t0 = now_s();
+  for (size_t k : order) {
+    const double dt = t0 + reqs[k].at - now_s();
+    if (dt > 0) std::this_thread::sleep_for(std::chrono::duration<double>(dt));
+    if (phx_server_submit(srv, reqs[k].snap, &reqs[k].rq, reqs[k].res.data(), &reqs[k].ticket) != PHX_OK)
+      return fail("submit");
+  }
+  ::mkdir(cfg.outdir, 0755);
+  int rc = EXIT_SUCCESS;
+  for (size_t k = 0; k < reqs.size(); k++) {
+    int32_t done = 0;
+    double lat = 0.;
+    if (phx_server_wait(srv, reqs[k].ticket, -1., &done, &lat) != PHX_OK) {
+      rc = fail("request");
+      continue;
+    }
+    info("Request %zu (snapshot %d, %d x %d x %d, arrived at %.3f s): latency %.3f s", k, reqs[k].snap,
+         reqs[k].rq.img_width, reqs[k].rq.img_height, reqs[k].rq.nfreqs, reqs[k].at, lat);
+    if (write_all(&reqs[k].rq, &reqs[k].cfg, reqs[k].res.data()) != EXIT_SUCCESS) rc = EXIT_FAILURE;
+  }
+  phx_serve_stats st;
+  if (phx_server_stats(srv, &st) == PHX_OK)
+    info("Served %lld requests in %.2f s: %.4g rays*freq/s, latency p50 %.3f s, p99 %.3f s, %lld launches",
+         (long long)st.completed, st.wall_s, (double)st.rays / st.wall_s, st.p50_s, st.p99_s, (long long)st.launches);
+  phx_server_stop(srv);
+  for (Snap& sn : snaps) {
+    phx_snapshot_close(sn.h);
+    phx_free_model(&sn.model);
+  }
+  info("Execution: %.2f s", now_s() - t_start);
+  return rc;
+}
+
 }  // namespace
 
 int main(int argc, char* argv[]) {
   const double t_start = now_s();
   std::string cfgpath = "input/config.toml";
   const char* batch_file = nullptr;
+  const char* serve_file = nullptr;
   std::vector<std::string> overrides;
   int a = 1;
   if (argc > 1 && !std::strchr(argv[1], '=')) {
     if (!std::strcmp(argv[1], "-h") || !std::strcmp(argv[1], "--help")) {
-      info("usage: %s [input/config.toml] [section.key=value ...] [--batch FILE]\n"
+      info("usage: %s [input/config.toml] [section.key=value ...] [--batch FILE | --serve FILE]\n"
            "       PHX_ACCURACY=FP32|FP64|MIXED selects the arithmetic (default FP32)",
            argv[0]);
       return 0;
     }
-    if (std::strcmp(argv[1], "--batch")) {
+    if (std::strcmp(argv[1], "--batch") && std::strcmp(argv[1], "--serve")) {
       cfgpath = argv[1];
       a = 2;
     }
@@ -127,6 +264,10 @@ int main(int argc, char* argv[]) {
   for (; a < argc; a++) {
     if (!std::strcmp(argv[a], "--batch") && a + 1 < argc) {
       batch_file = argv[++a];
+      continue;
+    }
+    if (!std::strcmp(argv[a], "--serve") && a + 1 < argc) {
+      serve_file = argv[++a];
       continue;
     }
     overrides.push_back(argv[a]);
@@ -141,6 +282,7 @@ int main(int argc, char* argv[]) {
   std::memset(&cfg, 0, sizeof cfg);
   std::memset(&model, 0, sizeof model);
   if (phx_load_config(cfgpath.c_str(), ov.data(), (int32_t)ov.size(), &cfg, &model) != PHX_OK) return fail("config");
+  if (serve_file) return serve(serve_file, cfgpath, ov, cfg, precision, t_start);
   info("Input Filepaths");
   info("Load config file      \"%s\"", cfgpath.c_str());
   info("Load grid information \"%s\"", cfg.grid_file);
@@ -173,21 +315,8 @@ int main(int argc, char* argv[]) {
       if (phx_load_config(cfgpath.c_str(), ovk.data(), (int32_t)ovk.size(), &cfgs[k], &mk) != PHX_OK)
         return fail("batch request");
       // the request's user fields on the loaded grid and dump
-      phx_model r = model;
-      r.inclination = mk.inclination;
-      r.fov_x = mk.fov_x;
-      r.fov_y = mk.fov_y;
-      r.r_cam = mk.r_cam;
-      r.t_init = mk.t_init;
-      r.pa = mk.pa;
-      r.phi_cam = mk.phi_cam;
-      r.img_width = mk.img_width;
-      r.img_height = mk.img_height;
-      r.nfreqs = mk.nfreqs;
-      std::memcpy(r.frequencies, mk.frequencies, sizeof r.frequencies);
-      r.m_unit = mk.m_unit;
-      r.r_high = mk.r_high;
-      if (phx_model_derive(&r) != PHX_OK) return fail("batch request");
+      phx_model r;
+      if (request_on(model, mk, &r) != EXIT_SUCCESS) return EXIT_FAILURE;
       reqs[k] = r;
       res[k].resize((size_t)r.img_width * r.img_height * r.nfreqs);
       std::snprintf(cfgs[k].outdir, sizeof cfgs[k].outdir, "%s/%zu", cfg.outdir, k);

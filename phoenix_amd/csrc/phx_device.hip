@@ -70,13 +70,15 @@ PHX_DEV void wave_sync() {
 #ifndef PHX_MIX_WAVES_PER_SIMD
 #define PHX_MIX_WAVES_PER_SIMD 2  // occupancy the mixed march is compiled for (1: 512 VGPRs, 2: 256)
 #endif
-template <int M, int NF, bool QUEUE>
-__global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_SIMD : 2) void k_march(
+// BS: the workgroup size (64 or 256 lanes in place; 256 with the queue).  The occupancy bound is in
+// waves per SIMD, whatever the workgroup size.
+template <int M, int NF, bool QUEUE, int BS>
+__global__ __launch_bounds__(BS, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_SIMD : 2) void k_march(
     const MarchArgs<M> A) {
   using T = typename ModeOf<M>::T;
   using L = typename ModeOf<M>::L;
   constexpr bool KAHAN = M == MIX;
-  constexpr int NW = QUEUE ? BLOCK / 64 : 1;  // per-wave LDS areas of the sample queue
+  constexpr int NW = QUEUE ? BS / 64 : 1;  // per-wave LDS areas of the sample queue
   __shared__ __align__(16) T qbuf[NW][QUEUE ? QCAP * QW : 1];  // queued samples, then their candidates
   __shared__ __align__(16) T lqbuf[NW][QUEUE ? LCAP * QW : 1];  // candidates left over from earlier flushes
   __shared__ int clbuf[NW][QUEUE ? QCAP : 1];                  // slots of this flush's candidates, in order
@@ -105,6 +107,7 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
   // queue bookkeeping: qcount and nleft are the same in every active lane of the wave; nmine and
   // ncm count this lane's queued samples and its candidates not yet folded in
   int qcount = 0, nleft = 0, nmine = 0, ncm = 0;
+  int since = A.fetch;  // passes since the wave last took jobs (the same in every lane of the wave)
 
   for (;;) {
     // refill: a lane whose ray ended waits until at least A.refill lanes of its wave are idle (or
@@ -116,6 +119,13 @@ __global__ __launch_bounds__(BLOCK, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_
     if (A.refill > 1) {
       const unsigned long long busy = __ballot(have);
       wait = busy != 0ull && 64 - (int)__popcll(busy) < A.refill;
+    }
+    // steps per fetch: a wave with rays in flight takes jobs for its idle lanes only every A.fetch
+    // passes (their atomic, job, task and start-record loads batched); 1 = at every pass
+    if (A.fetch > 1) {
+      if (since < A.fetch && __ballot(have) != 0ull) wait = true;
+      if (__ballot(!have && !wait) != 0ull) since = 0;
+      since++;
     }
     if (!have && !wait) {
       const SideArgs<M> S = side_of(A);
@@ -396,6 +406,31 @@ __global__ void k_store(const V v, V* dst) {
   *dst = v;
 }
 
+// The flux of each frequency of a result block (output.hpp:43-53, compute_spec): the sum over the
+// pixels whose I + Q + U + V is finite of IQUV dOm, times the Jy scale; one workgroup per frequency.
+// Each thread sums the pixels t, t + BLOCK, ... in index order, and the workgroup adds the partial
+// sums in a fixed tree: the same order at every call (not the reference's pixel-serial order, from
+// which it differs by double rounding only).
+__global__ __launch_bounds__(BLOCK) void k_flux(const phx_freqout* __restrict__ res, long npix, int nf, double dOm,
+                                                double jy, double* __restrict__ out) {
+  __shared__ double sh[4][BLOCK];
+  const int f = blockIdx.x;
+  double acc[4] = {0., 0., 0., 0.};
+  for (long p = threadIdx.x; p < npix; p += BLOCK) {
+    const phx_freqout r = res[p * nf + f];
+    if (!isfinite(r.IQUV[0] + r.IQUV[1] + r.IQUV[2] + r.IQUV[3])) continue;
+    for (int q = 0; q < 4; q++) acc[q] += r.IQUV[q] * dOm;
+  }
+  for (int q = 0; q < 4; q++) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int q = 0; q < 4; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[4 * f + threadIdx.x] = sh[threadIdx.x][0] * jy;
+}
+
 // Per leaf: the largest rho if every cell is field-free (B^i = 0), has rho >= 0 and finite
 // primitives, else -1.  One workgroup per leaf.
 template <class T>
@@ -536,6 +571,8 @@ struct phx_snapshot {
   int64_t bytes = 0;  // resident tables
   hipStream_t stream = nullptr;
   phx::DevBuf res, pix;  // per-render results and Morton pixel list (grow only)
+  phx::DevBuf spec;      // the device flux of the last render (4 doubles per frequency)
+  phx_model last_rq{};   // the last single render's request (its image, frequencies and units)
   phx::Slot slot;        // the launch buffers of the snapshot's own renders
   double decode_s = 0., upload_s = 0.;
   int64_t last_nres = 0;
@@ -604,7 +641,7 @@ static void snapshot_free(phx_snapshot* s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (void* p : s->allocs) (void)hipFree(p);
-  for (DevBuf* b : {&s->res, &s->pix})
+  for (DevBuf* b : {&s->res, &s->pix, &s->spec})
     if (b->p) (void)hipFree(b->p);
   slot_free(s->slot);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -789,6 +826,10 @@ static Tune tune_from_env(Tune t) {
   if (const char* e = std::getenv("PHX_FP64_WAVE")) t.fp64_wave = std::atoi(e) != 0;
   if (const char* e = std::getenv("PHX_ESCAPE"))
     t.escape = !std::strcmp(e, "full") ? 1 : !std::strcmp(e, "exit") ? 2 : t.escape;
+  if (const char* e = std::getenv("PHX_BLOCK"))
+    if (std::atoi(e) == 64 || std::atoi(e) == 256) t.block = std::atoi(e);
+  if (const char* e = std::getenv("PHX_FETCH"))
+    if (std::atoi(e) >= 1) t.fetch = std::atoi(e);
   if (const char* e = std::getenv("PHX_REFILL"))
     t.refill = !std::strcmp(e, "wave") ? 64 : !std::strcmp(e, "lane") ? 1 : std::atoi(e);
   return t;
@@ -801,7 +842,7 @@ struct LaunchInfo {
   bool no_alloc = false;           // the allocation-free entry point: no device allocation may happen
 };
 
-template <int M, int NF, bool Q>
+template <int M, int NF, bool Q, int BS>
 static void launch_march(phx_snapshot* s, Slot& sl, const LaunchInfo& li, int j0, int j1, const Tune& tn,
                          phx_info* info) {
   using T = typename ModeOf<M>::T;
@@ -813,15 +854,15 @@ static void launch_march(phx_snapshot* s, Slot& sl, const LaunchInfo& li, int j0
                         ensure(sl.side, sizeof side)};
   fill_args<M>(A, side, grid_of<T>(s, tables_of<T>(s)), GL, s->geo, li.uni, tn, bufs, j0, j1);
   k_store<SideArgs<M>><<<1, 1, 0, s->stream>>>(side, static_cast<SideArgs<M>*>(const_cast<void*>(bufs.side)));
-  const void* kfn = reinterpret_cast<const void*>(&k_march<M, NF, Q>);
+  const void* kfn = reinterpret_cast<const void*>(&k_march<M, NF, Q, BS>);
   int cus = 0, per_cu = 0;
   HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
-  HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, BLOCK, 0));
+  HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, BS, 0));
   per_cu = std::max(per_cu, 1);
   if (tn.blocks_per_cu > 0) per_cu = std::min(per_cu, tn.blocks_per_cu);
   int grid = per_cu * cus;
   const int njobs = j1 - j0;
-  grid = std::max(1, std::min(grid, (njobs + BLOCK - 1) / BLOCK));
+  grid = std::max(1, std::min(grid, (njobs + BS - 1) / BS));
   hipFuncAttributes fa;
   HIPCHECK(hipFuncGetAttributes(&fa, kfn));
   // a kernel whose private segment exceeds the per-lane stack limit does not run at all; raise the
@@ -836,7 +877,7 @@ static void launch_march(phx_snapshot* s, Slot& sl, const LaunchInfo& li, int j0
     HIPCHECK(hipDeviceSetLimit(hipLimitStackSize, (fa.localSizeBytes + 255) & ~size_t(255)));
   }
   k_reset<<<1, 64, 0, s->stream>>>(sl.counter, nullptr);
-  k_march<M, NF, Q><<<grid, BLOCK, 0, s->stream>>>(A);
+  k_march<M, NF, Q, BS><<<grid, BS, 0, s->stream>>>(A);
   HIPCHECK(hipGetLastError());
   sl.last_jobs = njobs;
   if (info) {
@@ -845,8 +886,8 @@ static void launch_march(phx_snapshot* s, Slot& sl, const LaunchInfo& li, int j0
     info->lds_bytes = (int32_t)fa.sharedSizeBytes;
     info->cus = cus;
     info->blocks = grid;
-    info->block = BLOCK;
-    info->waves_per_cu = per_cu * BLOCK / 64;
+    info->block = BS;
+    info->waves_per_cu = per_cu * BS / 64;
   }
 }
 
@@ -855,11 +896,12 @@ static void launch_nf(phx_snapshot* s, Slot& sl, const LaunchInfo& li, int j0, i
                       phx_info* info) {
   // default: samples in place with whole-wave refill (measured faster in every precision: rays of
   // one 8 x 8 pixel tile stay coherent; the LDS queue pays off only with per-lane refill)
-  const bool q = tn.queue == 1;
-  if (q)
-    launch_march<M, NF, true>(s, sl, li, j0, j1, tn, info);
+  if (tn.queue == 1)
+    launch_march<M, NF, true, 256>(s, sl, li, j0, j1, tn, info);
+  else if (tn.block == 64)
+    launch_march<M, NF, false, 64>(s, sl, li, j0, j1, tn, info);
   else
-    launch_march<M, NF, false>(s, sl, li, j0, j1, tn, info);
+    launch_march<M, NF, false, 256>(s, sl, li, j0, j1, tn, info);
 }
 
 // Ray start records for the jobs in sl.jobs (cameras in sl.cams), then the march over them,
@@ -1077,6 +1119,7 @@ static void render(phx_snapshot* s, const phx_model* rq, int precision, const in
   run_precision(s, precision, li, tn, info, progress, user);
   info->rays = (int64_t)njobs * rq->nfreqs;
   s->last_nres = (int64_t)nres;
+  s->last_rq = *rq;
   if (res) HIPCHECK(hipMemcpy(res, d_res, nres * sizeof(phx_freqout), hipMemcpyDeviceToHost));
 }
 
@@ -1185,24 +1228,30 @@ static void autotune(phx_snapshot* s, const phx_model* const* rqs, int n, int pr
                      phx_tune* best, double* best_rate) {
   std::lock_guard<std::mutex> lk(s->mu);
   sample_stride = std::max(1, sample_stride);
-  int max_bpc = 0;
-  {
-    int cus = 0;
-    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
-    (void)cus;
-    max_bpc = precision == PHX_FP64 ? 1 : 2;
-  }
   double rate_best = -1.;
   Tune tb;
+  // in place (workgroups of 64 or 256 lanes) or through the LDS queue (256); the whole wave or each
+  // lane refilled, each lane's refill at every pass or every 4th (steps per fetch); job order;
+  // whole-wave FP64 promotion (mixed)
+  struct Shape {
+    int queue, block, refill, fetch;
+  };
+  std::vector<Shape> shapes;
   for (int q = 0; q <= 1; q++)
-    for (int rf = 0; rf <= 1; rf++)
-    for (int bpc = 1; bpc <= max_bpc; bpc++)
-      for (int order = 0; order <= (n > 1 ? 1 : 0); order++)
-        for (int fw = 0; fw <= (precision == PHX_MIXED ? 1 : 0); fw++) {
+    for (int blk : {64, 256}) {
+      if (q == 1 && blk != 256) continue;
+      shapes.push_back({q, blk, 64, 1});
+      shapes.push_back({q, blk, 1, 1});
+      shapes.push_back({q, blk, 1, 4});
+    }
+  for (const Shape& sh : shapes)
+    for (int order = 0; order <= (n > 1 ? 1 : 0); order++)
+      for (int fw = 0; fw <= (precision == PHX_MIXED ? 1 : 0); fw++) {
         Tune t;
-        t.queue = q;
-        t.refill = rf ? 1 : 64;
-        t.blocks_per_cu = bpc;
+        t.queue = sh.queue;
+        t.block = sh.block;
+        t.refill = sh.refill;
+        t.fetch = sh.fetch;
         t.order = order;
         t.fp64_wave = fw;
         phx_info inf;
@@ -1221,8 +1270,21 @@ static void autotune(phx_snapshot* s, const phx_model* const* rqs, int n, int pr
     best->fp64_wave = tb.fp64_wave;
     best->refill = tb.refill;
     best->escape = tb.escape;
+    best->block = tb.block;
+    best->fetch = tb.fetch;
   }
   if (best_rate) *best_rate = rate_best;
+}
+
+// the flux of every frequency of the device result block of request rq into host spec[nfreqs][4]
+static void device_flux(hipStream_t stream, const phx_freqout* d_res, const phx_model* rq, double* d_spec,
+                        double* spec) {
+  double dOm = 0., jy = 0.;
+  flux_scale(rq, &dOm, &jy);
+  k_flux<<<rq->nfreqs, BLOCK, 0, stream>>>(d_res, (long)rq->img_width * rq->img_height, rq->nfreqs, dOm, jy, d_spec);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(spec, d_spec, (size_t)rq->nfreqs * 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+  HIPCHECK(hipStreamSynchronize(stream));
 }
 
 // ============================================================================ allocation-free render
@@ -1408,6 +1470,7 @@ struct Request {
   int snap = 0;
   phx_model rq{};
   phx_freqout* host_res = nullptr;
+  double* host_spec = nullptr;   // receives the flux of each frequency (reduced on the device), or null
   phx_freqout* d_res = nullptr;  // the request's block of the result pool
   size_t pool_off = 0;
   int64_t bytes = 0;
@@ -1498,6 +1561,7 @@ struct Server {
   std::vector<std::array<char, 2>> slot_busy;
   ResultPool pool;
   hipStream_t copy_stream = nullptr;
+  double* d_spec = nullptr;    // the flux of a finished request (4 doubles per frequency)
   std::deque<Flight> flights;  // queued launches, oldest first
 };
 
@@ -1572,6 +1636,7 @@ static void complete_request(Server* sv, const std::shared_ptr<Request>& r) {
   if (r->state == 1) {
     try {
       const size_t n = (size_t)r->rq.img_width * r->rq.img_height * r->rq.nfreqs;
+      if (r->host_spec) device_flux(sv->copy_stream, r->d_res, &r->rq, sv->d_spec, r->host_spec);
       if (r->host_res) {
         HIPCHECK(hipMemcpyAsync(r->host_res, r->d_res, n * sizeof(phx_freqout), hipMemcpyDeviceToHost,
                                 sv->copy_stream));
@@ -1771,6 +1836,19 @@ int phx_autotune(phx_snapshot* s, const phx_model* const* requests, int32_t n, i
   });
 }
 
+int phx_snapshot_flux(phx_snapshot* s, double* spec, int32_t nfreqs) {
+  return guarded([&] {
+    if (!s || !spec) throw Error(PHX_E_ARG, "null snapshot or spectrum");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->last_nres == 0 || s->last_rq.nfreqs != nfreqs ||
+        s->last_nres != (int64_t)s->last_rq.img_width * s->last_rq.img_height * nfreqs)
+      throw Error(PHX_E_ARG, "no single render with this many frequencies to reduce");
+    HIPCHECK(hipSetDevice(s->device));
+    double* d_spec = static_cast<double*>(ensure(s->spec, PHX_MAX_FREQS * 4 * sizeof(double)));
+    device_flux(s->stream, static_cast<const phx_freqout*>(s->res.p), &s->last_rq, d_spec, spec);
+  });
+}
+
 int phx_snapshot_render_noalloc(phx_snapshot* s, const phx_model* rq, int32_t precision, phx_freqout* d_res,
                                 phx_info* info) {
   return guarded([&] {
@@ -1783,7 +1861,8 @@ int phx_snapshot_set_tune(phx_snapshot* s, const phx_tune* t) {
   return guarded([&] {
     if (!s || !t) throw Error(PHX_E_ARG, "null snapshot or tune");
     if (t->queue < -1 || t->queue > 1 || t->blocks_per_cu < 0 || t->order < 0 || t->order > 1 || t->fp64_wave < 0 ||
-        t->fp64_wave > 1 || t->refill < 0 || t->refill > 64 || t->escape < 0 || t->escape > 2)
+        t->fp64_wave > 1 || t->refill < 0 || t->refill > 64 || t->escape < 0 || t->escape > 2 ||
+        (t->block != 0 && t->block != 64 && t->block != 256) || t->fetch < 0 || t->fetch > 1024)
       throw Error(PHX_E_ARG, "tune value out of range");
     s->tune.queue = t->queue;
     s->tune.blocks_per_cu = t->blocks_per_cu;
@@ -1791,6 +1870,8 @@ int phx_snapshot_set_tune(phx_snapshot* s, const phx_tune* t) {
     s->tune.fp64_wave = t->fp64_wave;
     s->tune.refill = t->refill > 0 ? t->refill : 64;
     s->tune.escape = t->escape;
+    s->tune.block = t->block ? t->block : 256;
+    s->tune.fetch = std::max(1, t->fetch);
   });
 }
 
@@ -1826,11 +1907,13 @@ int phx_server_start(phx_snapshot* const* snaps, int32_t nsnaps, const phx_serve
         for (Slot& sl : pair) slot_open(sl);
       sv.pool.open((size_t)sv.opts.hbm_budget);
       HIPCHECK(hipStreamCreateWithFlags(&sv.copy_stream, hipStreamNonBlocking));
+      HIPCHECK(hipMalloc(&sv.d_spec, PHX_MAX_FREQS * 4 * sizeof(double)));
     } catch (...) {
       for (auto& pair : sv.slots)
         for (Slot& sl : pair) slot_free(sl);
       sv.pool.close();
       if (sv.copy_stream) (void)hipStreamDestroy(sv.copy_stream);
+      if (sv.d_spec) (void)hipFree(sv.d_spec);
       throw;
     }
     sv.t_start = now_s();
@@ -1839,7 +1922,8 @@ int phx_server_start(phx_snapshot* const* snaps, int32_t nsnaps, const phx_serve
   });
 }
 
-int phx_server_submit(phx_server* srv, int32_t snap, const phx_model* request, phx_freqout* res, int64_t* ticket) {
+int phx_server_submit_spec(phx_server* srv, int32_t snap, const phx_model* request, phx_freqout* res, double* spec,
+                           int64_t* ticket) {
   return guarded([&] {
     if (!srv || !request) throw Error(PHX_E_ARG, "null server or request");
     Server& sv = srv->sv;
@@ -1849,6 +1933,7 @@ int phx_server_submit(phx_server* srv, int32_t snap, const phx_model* request, p
     r->snap = snap;
     r->rq = *request;
     r->host_res = res;
+    r->host_spec = spec;
     r->bytes = (int64_t)request->img_width * request->img_height * request->nfreqs * (int64_t)sizeof(phx_freqout);
     if (r->bytes > sv.opts.hbm_budget) throw Error(PHX_E_RANGE, "request larger than the server's HBM budget");
     {
@@ -1870,6 +1955,10 @@ int phx_server_submit(phx_server* srv, int32_t snap, const phx_model* request, p
     }
     sv.cv_work.notify_one();
   });
+}
+
+int phx_server_submit(phx_server* srv, int32_t snap, const phx_model* request, phx_freqout* res, int64_t* ticket) {
+  return phx_server_submit_spec(srv, snap, request, res, nullptr, ticket);
 }
 
 int phx_server_wait(phx_server* srv, int64_t ticket, double timeout_s, int32_t* done, double* latency_s) {
@@ -1958,6 +2047,7 @@ int phx_server_stop(phx_server* srv) {
       for (Slot& sl : pair) slot_free(sl);
     sv.pool.close();
     if (sv.copy_stream) (void)hipStreamDestroy(sv.copy_stream);
+    if (sv.d_spec) (void)hipFree(sv.d_spec);
     delete srv;
   });
 }

@@ -463,6 +463,14 @@ static void load_dump(phx_model* m, const std::string& path) {
 
 using namespace phx;
 
+namespace phx {
+void flux_scale(const phx_model* m, double* dOm, double* jy) {
+  const double dist = m->distance * K_KPC;
+  *dOm = (m->fov_x / m->img_width * m->L / dist) * (m->fov_y / m->img_height * m->L / dist);
+  *jy = K_JY;
+}
+}  // namespace phx
+
 extern "C" {
 
 const char* phx_last_error(void) { return g_err.c_str(); }

@@ -14,6 +14,9 @@ struct Error : std::runtime_error {
 
 void set_error(const std::string& msg);
 
+// the factors of the flux sum (output.hpp:43-53): pixel solid angle dOm and the Jy scale
+void flux_scale(const phx_model* m, double* dOm, double* jy);
+
 // Runs f, turning exceptions into an error code + thread-local message.
 template <class F>
 int guarded(F&& f) {

@@ -63,6 +63,7 @@ struct MarchArgs {
   int fp64_wave;                        // MIX: a lane's FP64 step takes its whole wave to FP64
   int refill;                           // idle lanes a wave gathers before they take new jobs (1: at once)
   int escape;                           // 1: a ray moving outward beyond rt_outer ends (it cannot emit again)
+  int fetch;                            // steps per fetch: passes between a wave's job fetches (1: every pass)
   int max_steps, max_order;
   const TaskDev<T>* tasks;
   const int2* jobs;
@@ -158,9 +159,10 @@ struct StHam {
     }
     return south;
   }
-  // precision of the coming step and its first stage; ck = the wavevector at the current point
+  // precision of the coming step and its first stage; ck = the wavevector at the current point;
+  // r32 = e^{x1} of the FP32 position, which the pass's view has computed
   template <class A>
-  PHX_DEV void begin(const A& args, bool alive, float ck[4]) {
+  PHX_DEV void begin(const A& args, bool alive, float ck[4], float r32) {
     f64 = alive && (X[0] < args.lnr_fp64 || X[1] < args.x2_pole || X[1] > PI_D - args.x2_pole);
     if (args.fp64_wave) f64 = ballot(f64) != 0ull;
     if (f64) {
@@ -172,7 +174,7 @@ struct StHam {
     } else {
       float b[5], d[5], k0;
       const bool south = base32(b);
-      tb = pt32_at(b[0], b[1], args.P.hs);
+      tb = pt32_at_r(r32, b[1], args.P.hs);  // b[0] = float(X[0]): the view's exp of it
       ham_core<float>(args.P.a, args.P.hs, float(pt), float(pp), tb.r, tb.s2x, tb.c2x, tb.s, tb.c, b[3], b[4], d, &k0);
 #pragma unroll
       for (int q = 0; q < 5; q++) d1[q] = d[q];
@@ -649,7 +651,7 @@ struct Lane {
     // the coming step's precision and first stage, and the wavevector at the current point (not
     // needed before: the pending sample carries its own); evaluated after the sample, so the stage
     // is not held in registers across it
-    if constexpr (M == MIX) st.begin(A, v.alive, v.ck);
+    if constexpr (M == MIX) st.begin(A, v.alive, v.ck, v.r);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       pk[q] = v.ck[q];

@@ -24,6 +24,8 @@ struct Tune {
   int refill = 64;        // idle lanes a wave gathers before they take new jobs (64: the whole wave)
   int escape = 0;         // 0: by precision (MIXED ends escaping rays at rt_outer, FP32 / FP64 march them
                           // to r_out as the reference does), 1: march to r_out, 2: end at rt_outer
+  int block = 256;        // workgroup size of the in-place march: 64 or 256 lanes
+  int fetch = 1;          // steps per fetch: a wave with rays in flight takes jobs every `fetch` passes
 };
 
 // block records: lower corner, upper corner (lo + nx * dxc in the table's precision, as
@@ -211,6 +213,7 @@ inline void fill_args(MarchArgs<M>& A, SideArgs<M>& side, const Grid<typename Mo
   A.fp64_wave = tn.fp64_wave;
   A.refill = std::min(64, std::max(1, tn.refill));
   A.escape = escape_allowed(tn.escape, M == MIX, m.a, rq->rt_outer);
+  A.fetch = std::max(1, tn.fetch);
   A.max_steps = rq->max_steps;
   A.max_order = rq->max_order;
   A.tasks = static_cast<const TaskDev<T>*>(b.tasks);

@@ -510,9 +510,10 @@ PHX_DEV void ham_rhs(T a, T hs, T pt, T pp, const T y[5], T d[5], T* k0) {
 struct Pt32 {
   float r, sx, cx, s2x, c2x, s, c;
 };
-PHX_DEV Pt32 pt32_at(float x1, float x2, float hs) {
+// r = e^{x1} already known (the pass's view computes it)
+PHX_DEV Pt32 pt32_at_r(float r, float x2, float hs) {
   Pt32 t;
-  t.r = exp_r(x1);
+  t.r = r;
   sincos_small(x2, &t.sx, &t.cx);
   t.s2x = 2.0f * t.sx * t.cx;
   t.c2x = (t.cx - t.sx) * (t.cx + t.sx);
@@ -530,6 +531,7 @@ PHX_DEV Pt32 pt32_at(float x1, float x2, float hs) {
   t.c = t.cx * cd - t.sx * sd;
   return t;
 }
+PHX_DEV Pt32 pt32_at(float x1, float x2, float hs) { return pt32_at_r(exp_r(x1), x2, hs); }
 // sin and cos of a small |e| <= 0.03 (Taylor to e^3 / e^4: truncation below 7e-9 of sin e and
 // 1e-12 of cos e, under FP32 rounding)
 PHX_DEV void sincos_tiny(float e, float* s, float* c) {
@@ -1008,6 +1010,13 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
     fl.ne = T(0);
     return 0;
   }
+  // The mixed-precision march stops at any sample in a field-free leaf: it can only be cut or
+  // dropped (b = 0), it changes no image, and the mixed march does not keep the reference's
+  // plasma-segment and dropped-segment counts (its samples are not counted there).
+  if (MIXED && ffleaf) {
+    fl.ne = T(0);
+    return 0;
+  }
   const int nxs[3] = {G.nx0, G.nx1, G.nx2};
   int c[3];
   T del[3];
@@ -1412,8 +1421,14 @@ PHX_DEV void segment_op(const Coef<T>& c, T h, T OA[16], T OS[4]) {
       }
     }
     const T d = A[c0][c0];
+    if constexpr (Lim<T>::fp32) {  // one reciprocal per pivot (the row's divisions share it)
+      const T id = div_rcp(T(1), d);
 #pragma unroll
-    for (int j = c0; j < 8; j++) A[c0][j] = A[c0][j] / d;
+      for (int j = c0; j < 8; j++) A[c0][j] = A[c0][j] * id;
+    } else {
+#pragma unroll
+      for (int j = c0; j < 8; j++) A[c0][j] = A[c0][j] / d;
+    }
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       if (r == c0) continue;

@@ -88,7 +88,8 @@ def test_autotune_picks_a_valid_configuration(c1):
     ref, _ = snap.render_batch(reqs, capi.FP32)
     tune, rate = snap.autotune(reqs, capi.FP32, sample_stride=2)
     print(f"\nautotune: {tune} at {rate:.3g} rays*freq/s (sampled)")
-    assert tune["queue"] in (0, 1) and tune["blocks_per_cu"] >= 1 and tune["order"] in (0, 1)
+    assert tune["queue"] in (0, 1) and tune["order"] in (0, 1)
+    assert tune["block"] in (64, 256) and tune["fetch"] in (1, 4) and tune["refill"] in (1, 64)
     assert rate > 0
     again, _ = snap.render_batch(reqs, capi.FP32)
     for rq, a, b in zip(reqs, again, ref):

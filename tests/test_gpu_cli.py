@@ -73,3 +73,36 @@ def test_cli_batch(snapdir, tmp_path):
 def test_cli_reports_errors(tmp_path):
     p = _run_cli([str(tmp_path / "missing.toml")])
     assert p.returncode != 0 and "FAIL" in p.stderr
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="phoenix driver not built")
+def test_cli_serve(snapdir, tmp_path):
+    """--serve FILE: requests with arrival times, on two resident snapshots (a line names the other
+    dump; override values are taken as written, without quotes, as the reference's config.hpp:246-262
+    takes them), through the serving loop; every request's spectrum against the same request
+    rendered alone through the C ABI."""
+    dump, grid, _ = S.ensure_snapshot("c1", snapdir)
+    d2, g2, _ = S.ensure_snapshot("c1_a05", snapdir)
+    out = tmp_path / "out"
+    cfg = configs.write_config(str(tmp_path / "c.toml"), dump, grid, output__dir=f'"{out}"', output__image='"none"',
+                               camera__img_width="16", camera__img_height="16")
+    serve = tmp_path / "serve.txt"
+    serve.write_text(f"0.0 camera.inclination=20\n"
+                     f"0.05 camera.inclination=70 camera.img_width=24 camera.frequency_list=[86e9,230e9]\n"
+                     f"0.02 grmhd.dump_file={d2} grmhd.grid_file={g2} camera.inclination=45 plasma.r_high=20\n")
+    p = _run_cli([cfg, "--serve", str(serve)], {"PHX_SERVE_QUANTUM": "100"})
+    assert p.returncode == 0
+    assert "Served 3 requests" in p.stderr
+    cases = [(cfg, dict(inclination=20.0)), (cfg, dict(inclination=70.0, img_width=24, freqs=(86e9, 230e9))),
+             (configs.write_config(str(tmp_path / "c2.toml"), d2, g2, camera__img_width="16", camera__img_height="16"),
+              dict(inclination=45.0, r_high=20.0))]
+    for k, (c, over) in enumerate(cases):
+        run = capi.Run(c)
+        snap = capi.Snapshot(run, capi.TABLE_FP32)
+        rq = run.request(**over)
+        alone, _ = snap.render(rq, capi.FP32)
+        S_ref = run.flux(alone, rq)
+        spec = np.loadtxt(out / str(k) / "spectrum.dat", ndmin=2)
+        np.testing.assert_allclose(spec[:, 1:5], S_ref, rtol=1e-6, atol=1e-12 * np.abs(S_ref[:, :1]).max())
+        snap.close()
+        run.close()

@@ -159,6 +159,22 @@ def test_wave_synchronous_refill_matches_lane_refill(c1_run, precision, refill):
         assert np.abs(queued[..., c] - wave[..., c]).max() <= 1e-5 * np.abs(wave[..., c]).max()
 
 
+@pytest.mark.parametrize("precision", [capi.FP32, capi.FP64])
+def test_workgroup_size_and_steps_per_fetch_change_nothing(c1_run, precision):
+    """Workgroups of 64 lanes instead of 256, and per-lane refill with jobs taken every 4th pass
+    (steps per fetch), only change which lane marches which ray: bit-identical images and counts."""
+    snap = capi.Snapshot(c1_run, capi.TABLE_FP32 | capi.TABLE_FP64)
+    rq = c1_run.request(img_width=40, img_height=32, inclination=50.0)
+    ref, i0 = snap.render(rq, precision)
+    for t in ({"block": 64}, {"refill": 1, "fetch": 4}, {"block": 64, "refill": 1, "fetch": 4}):
+        res, i1 = snap.render(rq, precision, tune=t)
+        print(f"\nfp{precision} {t}: kernel {i1['kernel_s'] * 1e3:.2f} ms (default {i0['kernel_s'] * 1e3:.2f} ms), "
+              f"block {i1['block']}")
+        assert np.array_equal(res, ref)
+        assert (i1["steps"], i1["nplasma"], i1["bad"]) == (i0["steps"], i0["nplasma"], i0["bad"])
+    snap.close()
+
+
 @pytest.mark.parametrize("queue", [0, 1])
 @pytest.mark.parametrize("precision", [capi.FP32, capi.FP64, capi.MIXED])
 def test_escaping_rays_end_at_rt_outer_with_the_same_results(c1_run, precision, queue):
@@ -223,7 +239,10 @@ def test_field_free_leaves_change_nothing(preset, snapdir, tmp_path, monkeypatch
             print(f"\n{preset} sigma_cut {sigma_cut} fp{precision}: kernel {ia['kernel_s'] * 1e3:.2f} vs "
                   f"{ib['kernel_s'] * 1e3:.2f} ms, segments {ia['nplasma']}, dropped {ia['bad']}")
             assert np.array_equal(a, b)
-            assert (ia["steps"], ia["nplasma"], ia["bad"]) == (ib["steps"], ib["nplasma"], ib["bad"])
+            if precision == capi.MIXED:  # the mixed march does not count samples in field-free leaves
+                assert ia["steps"] == ib["steps"] and ia["nplasma"] - ia["bad"] == ib["nplasma"] - ib["bad"]
+            else:
+                assert (ia["steps"], ia["nplasma"], ia["bad"]) == (ib["steps"], ib["nplasma"], ib["bad"])
     snap.close()
     full.close()
 

@@ -131,3 +131,23 @@ def test_server_rejects_bad_requests(env):
     with pytest.raises(capi.PhxError):
         srv.submit(3, _rq(run, 4, 4, (230e9,), 10.0))  # no such snapshot
     srv.stop()
+
+
+def test_device_flux_matches_compute_spec(env):
+    """The flux reduced on the device (phx_snapshot_flux, and a served request's spectrum with no
+    image copied back) against the host's phx_compute_spec of the same image, to 1e-12 of I."""
+    run, _, snaps = env
+    rq = _rq(run, 40, 32, (86e9, 230e9, 690e9), 60.0, 2e18, 10.0)
+    res, _ = snaps[0].render(rq, capi.FP32)
+    host = run.flux(res, rq)
+    dev = snaps[0].flux(rq.nfreqs)
+    print(f"\ndevice flux {dev[:, 0]} Jy, host {host[:, 0]} Jy")
+    assert np.all(np.abs(dev - host) <= 1e-12 * np.abs(host[:, :1]))
+    srv = capi.Server(snaps[:1], capi.FP32, quantum_jobs=500)
+    t_img = srv.submit(0, rq)
+    t_spec = srv.submit(0, rq, image=False, spec=True)
+    img, _ = srv.wait(t_img)
+    none, spec, _ = srv.wait_spec(t_spec)
+    srv.stop()
+    assert none is None and np.array_equal(img, res)
+    assert np.all(np.abs(spec - host) <= 1e-12 * np.abs(host[:, :1]))
