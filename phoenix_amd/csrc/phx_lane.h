@@ -122,6 +122,7 @@ struct StHam {
   float kap1, kap2;
   double d1[5];  // first stage of the coming step: FP64, or the (mirrored) FP32 derivative widened
   Pt32 tb;       // FP32 step: the point quantities of its first stage, the base of the others
+  Pt64 tb64;     // FP64 step: the same in FP64
   bool f64 = false;
   PHX_DEV void load(const double* st, float* bsign) {
 #pragma unroll
@@ -167,7 +168,9 @@ struct StHam {
     if (args.fp64_wave) f64 = ballot(f64) != 0ull;
     if (f64) {
       double k0;
-      ham_rhs<double, true, true>(args.a64, args.hs64, pt, pp, X, d1, &k0);
+      tb64 = pt64_at(taylor_coef(), X[0], X[1], args.hs64);
+      ham_core<double, true>(args.a64, args.hs64, pt, pp, tb64.r, tb64.s2x, tb64.c2x, tb64.s, tb64.c, X[3], X[4], d1,
+                             &k0);
       ck[0] = float(k0);
 #pragma unroll
       for (int q = 0; q < 3; q++) ck[1 + q] = float(d1[q]);
@@ -190,22 +193,29 @@ struct StHam {
   PHX_DEV bool step(const A& args, float dl) {
     if (f64) {
       const double dt = dl;
+      const TaylorSC t = taylor_coef();
       double sum[5], y[5];
 #pragma unroll
       for (int i = 0; i < 5; i++) {
         sum[i] = d1[i];
         y[i] = X[i] + 0.5 * dt * d1[i];
       }
+      // stages 2-4 at the offsets h d of stage 1's point (|h d| <= 0.02; else evaluated afresh)
+      double e1 = 0.5 * dt * d1[0], e2 = 0.5 * dt * d1[1];
 #pragma unroll 1
       for (int q = 1; q < 4; q++) {
         double d[5], k0;
-        ham_rhs<double, true, true>(args.a64, args.hs64, pt, pp, y, d, &k0);
+        const Pt64 p = (fabs(e1) <= 0.02 && fabs(e2) <= 0.02) ? pt64_offset(t, tb64, e1, e2, args.hs64)
+                                                               : pt64_at(t, y[0], y[1], args.hs64);
+        ham_core<double, true>(args.a64, args.hs64, pt, pp, p.r, p.s2x, p.c2x, p.s, p.c, y[3], y[4], d, &k0);
         const double w = q == 3 ? 1. : 2., h = q == 2 ? dt : 0.5 * dt;
 #pragma unroll
         for (int i = 0; i < 5; i++) {
           sum[i] += w * d[i];
           y[i] = X[i] + h * d[i];
         }
+        e1 = h * d[0];
+        e2 = h * d[1];
       }
       const double dt6 = dt / 6.;
 #pragma unroll

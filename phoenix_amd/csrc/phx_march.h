@@ -177,19 +177,25 @@ PHX_CONSTANT double kTaylorCos[10] = {1. / 2432902008176640000., -1. / 640237370
                                       -1. / 87178291200.,        1. / 479001600.,         -1. / 3628800.,
                                       1. / 40320.,               -1. / 720.,              1. / 24.,
                                       -0.5};
+// Taylor coefficients of (e^x - 1 - x) / x^2 in x (Horner order): 1/10! ... 1/2
+PHX_CONSTANT double kTaylorExp[9] = {1. / 3628800., 1. / 362880., 1. / 40320., 1. / 5040., 1. / 720.,
+                                    1. / 120.,     1. / 24.,     1. / 6.,     0.5};
 struct TaylorSC {
-  double s[9], c[10];
+  double s[9], c[10], e[9];
 };
 // the coefficients, loaded (scalar loads) at the call through an opaque copy of the table address,
 // so that the loads are not hoisted out of an enclosing loop
 PHX_DEV TaylorSC taylor_coef() {
   const double* ps = opaque_scalar<const double*>(kTaylorSin);
   const double* pc = opaque_scalar<const double*>(kTaylorCos);
+  const double* pe = opaque_scalar<const double*>(kTaylorExp);
   TaylorSC t;
 #pragma unroll
   for (int i = 0; i < 9; i++) t.s[i] = load_constant(ps, i);
 #pragma unroll
   for (int i = 0; i < 10; i++) t.c[i] = load_constant(pc, i);
+#pragma unroll
+  for (int i = 0; i < 9; i++) t.e[i] = load_constant(pe, i);
   return t;
 }
 // sin(r) and cos(r) by the Taylor polynomials above (|r| <= pi/4: truncation below 1e-19)
@@ -237,6 +243,57 @@ PHX_DEV void theta_trig_1(double x2, double hs, double* s2x, double* c2x, double
   taylor_sc(t, d, &sd, &cd);
   *s = sx * cd + cx * sd;
   *c = cx * cd - sx * sd;
+}
+// The point quantities of an FP64 RK4 stage: r, sin/cos x2, sin/cos 2x2, sin/cos theta
+struct Pt64 {
+  double r, sx, cx, s2x, c2x, s, c;
+};
+PHX_DEV Pt64 pt64_at(const TaylorSC& t, double x1, double x2, double hs) {
+  Pt64 p;
+  p.r = exp(x1);
+  sincos_small(t, x2, &p.sx, &p.cx);
+  p.s2x = 2. * p.sx * p.cx;
+  p.c2x = (p.cx - p.sx) * (p.cx + p.sx);
+  double sd, cd;
+  taylor_sc(t, 0.5 * hs * p.s2x, &sd, &cd);
+  p.s = p.sx * cd + p.cx * sd;
+  p.c = p.cx * cd - p.sx * sd;
+  return p;
+}
+// sin and cos of a small |e| <= 0.03 to FP64 rounding: the tails of the Taylor polynomials above
+// (sin to e^9, cos to e^10: truncation below 5e-25)
+PHX_DEV void sincos_tiny(const TaylorSC& t, double e, double* se, double* ce) {
+  const double z = e * e;
+  double ps = t.s[5];
+#pragma unroll
+  for (int i = 6; i < 9; i++) ps = fma(ps, z, t.s[i]);
+  *se = fma(ps * z, e, e);
+  double pc = t.c[5];
+#pragma unroll
+  for (int i = 6; i < 10; i++) pc = fma(pc, z, t.c[i]);
+  *ce = fma(pc, z, 1.);
+}
+// The FP64 point quantities at (x1 + e1, x2 + e2) from those at (x1, x2) for |e1|, |e2| <= 0.02 (an
+// RK4 stage of one step): e^{e1} by its Taylor polynomial to e1^10 (truncation below 1e-24), the
+// angles by addition with sincos_tiny; theta's offset is e2 + (h/2)(sin 2(x2 + e2) - sin 2x2).
+// The offsets are applied to the base point exactly, as the FP32 stages do (pt32_offset).
+PHX_DEV Pt64 pt64_offset(const TaylorSC& t, const Pt64& b, double e1, double e2, double hs) {
+  Pt64 p;
+  double pe = t.e[0];
+#pragma unroll
+  for (int i = 1; i < 9; i++) pe = fma(pe, e1, t.e[i]);
+  p.r = fma(b.r, fma(pe * e1, e1, e1), b.r);
+  double se, ce;
+  sincos_tiny(t, e2, &se, &ce);
+  p.sx = b.sx * ce + b.cx * se;
+  p.cx = b.cx * ce - b.sx * se;
+  p.s2x = 2. * p.sx * p.cx;
+  p.c2x = (p.cx - p.sx) * (p.cx + p.sx);
+  double st, ct;
+  sincos_tiny(t, fma(0.5 * hs, p.s2x - b.s2x, e2), &st, &ct);
+  p.s = b.s * ct + b.c * st;
+  p.c = b.c * ct - b.s * st;
+  return p;
 }
 template <bool ONE_SINCOS>
 PHX_DEV void theta_trig_sel(double x2, double hs, double* s2x, double* c2x, double* s, double* c) {
