@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define PHX_MAX_FREQS 32  /* frequencies per request (src/core/numerics/macros.hpp:110-112, CPU value) */
+#define PHX_FREQS_MAX 32  /* frequencies per request (the CPU value of PHX_MAX_FREQS, src/core/numerics/macros.hpp:110-112; named apart so that phx.h and the reference headers share a translation unit) */
 #define PHX_MAX_LEVELS 16 /* AMR levels (src/core/model/bhac.hpp:154-161) */
 
 /* error codes */
@@ -68,7 +68,7 @@ typedef struct phx_model {
   /* [camera] */
   double inclination, fov_x, fov_y, r_cam, t_init, pa, phi_cam;
   int32_t img_width, img_height, nfreqs, pad0;
-  double frequencies[PHX_MAX_FREQS];
+  double frequencies[PHX_FREQS_MAX];
   /* [black_hole] */
   double mass, distance;
   /* [integration] */
@@ -85,7 +85,7 @@ typedef struct phx_model {
   /* derived scalars (src/host/io/bhac_io.hpp:183-251) */
   double a, hslope, gam, Ne, Bu, rhigh, rlow, ibc, r_out, cut_in, cos_pa, sin_pa;
   double L, T_unit, rho_unit, U_unit, B_unit, Ne_unit;
-  double freq[PHX_MAX_FREQS], hconst[PHX_MAX_FREQS], jscale[PHX_MAX_FREQS];
+  double freq[PHX_FREQS_MAX], hconst[PHX_FREQS_MAX], jscale[PHX_FREQS_MAX];
   double lvl_w[PHX_MAX_LEVELS][3], x_lo[4], x_hi[4];
   int32_t level_dims[PHX_MAX_LEVELS][3];
   /* host tables, owned by phx_load_dump (freed by phx_free_model) */

@@ -60,6 +60,8 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
     if os.path.isdir("/root/reference/src"):
         subprocess.run(["make", "-s", "-C", HERE, "ref", "ref_output"], check=True)
+        if os.path.exists(os.path.join(HERE, "..", "phoenix_amd", "libphx.so")):
+            subprocess.run(["make", "-s", "-C", HERE, "ref_main_phx"], check=True)
 
 
 _lib = None

@@ -946,7 +946,7 @@ static void check_request(const phx_model* rq) {
   if (!rq) throw Error(PHX_E_ARG, "null request");
   if (rq->img_width <= 0 || rq->img_height <= 0) throw Error(PHX_E_ARG, "image size must be positive");
   if ((int64_t)rq->img_width * rq->img_height > (int64_t)1 << 30) throw Error(PHX_E_RANGE, "image too large");
-  if (rq->nfreqs < 1 || rq->nfreqs > PHX_MAX_FREQS) throw Error(PHX_E_ARG, "nfreqs must be in 1..32");
+  if (rq->nfreqs < 1 || rq->nfreqs > PHX_FREQS_MAX) throw Error(PHX_E_ARG, "nfreqs must be in 1..32");
 }
 
 // the parameters one launch shares: a batch may differ only in camera, image, frequencies,
@@ -1327,8 +1327,8 @@ static void* scratch_on(int device) {
 // jobs.hpp:18-46: pixels sorted by the key with i in the even bits and j in the odd bits)
 template <class T>
 struct NoAllocSetup {
-  TaskDev<T> tasks[PHX_MAX_FREQS / CHUNK];
-  Cam cams[PHX_MAX_FREQS / CHUNK];
+  TaskDev<T> tasks[PHX_FREQS_MAX / CHUNK];
+  Cam cams[PHX_FREQS_MAX / CHUNK];
   int ntasks, npix, W, H, levels;  // levels: log2 of the power-of-two square that holds the image
 };
 // the n-th pixel (i, j) of a W x H image in Morton order, by descending the quadtree of the
@@ -1397,8 +1397,8 @@ static void render_noalloc_t(phx_snapshot* s, const phx_model* rq, int precision
   window(sl.start, base + NOALLOC_HEAD + (size_t)max_jobs * sizeof(int2), (size_t)max_jobs * rec);
   sl.e0 = s->slot.e0;
   sl.e1 = s->slot.e1;
-  static_assert(sizeof(TaskDev<double>) * (PHX_MAX_FREQS / CHUNK) <= 16384, "tasks window");
-  static_assert(sizeof(Cam) * (PHX_MAX_FREQS / CHUNK) <= 16384, "cameras window");
+  static_assert(sizeof(TaskDev<double>) * (PHX_FREQS_MAX / CHUNK) <= 16384, "tasks window");
+  static_assert(sizeof(Cam) * (PHX_FREQS_MAX / CHUNK) <= 16384, "cameras window");
   NoAllocSetup<T> a;
   std::memset(&a, 0, sizeof a);
   std::vector<TaskDev<T>> tasks;
@@ -1844,7 +1844,7 @@ int phx_snapshot_flux(phx_snapshot* s, double* spec, int32_t nfreqs) {
         s->last_nres != (int64_t)s->last_rq.img_width * s->last_rq.img_height * nfreqs)
       throw Error(PHX_E_ARG, "no single render with this many frequencies to reduce");
     HIPCHECK(hipSetDevice(s->device));
-    double* d_spec = static_cast<double*>(ensure(s->spec, PHX_MAX_FREQS * 4 * sizeof(double)));
+    double* d_spec = static_cast<double*>(ensure(s->spec, PHX_FREQS_MAX * 4 * sizeof(double)));
     device_flux(s->stream, static_cast<const phx_freqout*>(s->res.p), &s->last_rq, d_spec, spec);
   });
 }
@@ -1907,7 +1907,7 @@ int phx_server_start(phx_snapshot* const* snaps, int32_t nsnaps, const phx_serve
         for (Slot& sl : pair) slot_open(sl);
       sv.pool.open((size_t)sv.opts.hbm_budget);
       HIPCHECK(hipStreamCreateWithFlags(&sv.copy_stream, hipStreamNonBlocking));
-      HIPCHECK(hipMalloc(&sv.d_spec, PHX_MAX_FREQS * 4 * sizeof(double)));
+      HIPCHECK(hipMalloc(&sv.d_spec, PHX_FREQS_MAX * 4 * sizeof(double)));
     } catch (...) {
       for (auto& pair : sv.slots)
         for (Slot& sl : pair) slot_free(sl);

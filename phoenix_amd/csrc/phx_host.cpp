@@ -213,8 +213,8 @@ static void load_config(const std::string& path, const std::vector<std::string>&
   dbl("camera.phi", m->phi_cam, false);
   try {
     const std::vector<double> f = read_frequencies(kv, used);
-    if (f.size() > (size_t)PHX_MAX_FREQS)
-      throw Error(PHX_E_RANGE, "at most " + std::to_string(PHX_MAX_FREQS) + " frequencies (PHX_MAX_FREQS), got " +
+    if (f.size() > (size_t)PHX_FREQS_MAX)
+      throw Error(PHX_E_RANGE, "at most " + std::to_string(PHX_FREQS_MAX) + " frequencies (PHX_MAX_FREQS), got " +
                                    std::to_string(f.size()));
     for (size_t i = 0; i < f.size(); i++) m->frequencies[i] = f[i];
     m->nfreqs = (int32_t)f.size();
@@ -286,7 +286,7 @@ static void load_grid(const std::string& path, phx_model* m) {
 // per-request scalars of finalize_model: units, plasma, camera, per-frequency
 // constants (src/host/io/bhac_io.hpp:224-250)
 static void derive_request(phx_model* m) {
-  if (m->nfreqs < 1 || m->nfreqs > PHX_MAX_FREQS) throw Error(PHX_E_ARG, "nfreqs must be in 1..32");
+  if (m->nfreqs < 1 || m->nfreqs > PHX_FREQS_MAX) throw Error(PHX_E_ARG, "nfreqs must be in 1..32");
   if (!(m->beta_c > 0.)) throw Error(PHX_E_ARG, "plasma.beta_c must be > 0");
   m->L = K_G * m->mass * K_MSUN / K_C / K_C;
   m->T_unit = m->L / K_C;
