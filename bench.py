@@ -195,6 +195,43 @@ def bench_batch(run, snap, capi, prec, sync) -> dict:
             "batched_vs_sequential_plain_fp32": wall_seq / wall}
 
 
+def queue_model(arrive, service, discipline: str) -> np.ndarray:
+    """Latencies of an ideal single server with no overhead, fed the given arrival times and
+    per-request service times: "fifo" serves one request at a time in arrival order, "ps" shares
+    the server equally among the requests present (processor sharing)."""
+    n = len(arrive)
+    done = np.zeros(n)
+    if discipline == "fifo":
+        t = 0.0
+        for i in range(n):
+            t = max(t, arrive[i]) + service[i]
+            done[i] = t
+        return done - arrive
+    rem, t, i, active = np.array(service, dtype=float), 0.0, 0, []
+    while i < n or active:
+        if not active:
+            t = max(t, arrive[i])
+            active.append(i)
+            i += 1
+            continue
+        k = len(active)
+        finish = t + min(rem[j] for j in active) * k
+        if i < n and arrive[i] < finish:
+            for j in active:
+                rem[j] -= (arrive[i] - t) / k
+            t = arrive[i]
+            active.append(i)
+            i += 1
+            continue
+        for j in active:
+            rem[j] -= (finish - t) / k
+        t = finish
+        for j in [j for j in active if rem[j] <= 1e-12]:
+            done[j] = t
+            active.remove(j)
+    return done - arrive
+
+
 def bench_serve(capi, prec, tmpdir) -> dict:
     runs, snaps = [], []
     for name in ("c5", "c5_a05"):
@@ -260,6 +297,14 @@ def bench_serve(capi, prec, tmpdir) -> dict:
                    "slowdown_p50": float(np.percentile(slow, 50)), "slowdown_p99": float(np.percentile(slow, 99)),
                    "throughput_rays_per_s": st["rays"] / max(st["wall_s"], 1e-9), "launches": st["launches"],
                    "peak_result_bytes": st["peak_bytes"]}
+            if order == 0:
+                # the same arrivals through ideal servers with the one-at-a-time latencies as service
+                # times: how much of the latency the queueing itself accounts for
+                arrive = np.cumsum(gaps)
+                for disc in ("fifo", "ps"):
+                    ideal = queue_model(arrive, single, disc)
+                    res[f"ideal_{disc}_p99_ms"] = float(np.percentile(ideal, 99)) * 1e3
+                    res[f"ideal_{disc}_p99_vs_single_p99"] = res[f"ideal_{disc}_p99_ms"] / out["single_p99_ms"]
             out[name] = res
             log(f"serving at {load:.0%} load, {name}: p50 {p50 * 1e3:.0f} ms, p99 {p99 * 1e3:.0f} ms "
                 f"(x{res['p99_vs_single_p99']:.2f} of the one-at-a-time p99 {out['single_p99_ms']:.0f} ms), "

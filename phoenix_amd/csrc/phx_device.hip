@@ -70,10 +70,13 @@ PHX_DEV void wave_sync() {
 #ifndef PHX_MIX_WAVES_PER_SIMD
 #define PHX_MIX_WAVES_PER_SIMD 2  // occupancy the mixed march is compiled for (1: 512 VGPRs, 2: 256)
 #endif
+#ifndef PHX_FP64_WAVES_PER_SIMD
+#define PHX_FP64_WAVES_PER_SIMD 1  // occupancy the FP64 march is compiled for
+#endif
 // BS: the workgroup size (64 or 256 lanes in place; 256 with the queue).  The occupancy bound is in
 // waves per SIMD, whatever the workgroup size.
 template <int M, int NF, bool QUEUE, int BS>
-__global__ __launch_bounds__(BS, M == M64 ? 1 : M == MIX ? PHX_MIX_WAVES_PER_SIMD : 2) void k_march(
+__global__ __launch_bounds__(BS, M == M64 ? PHX_FP64_WAVES_PER_SIMD : M == MIX ? PHX_MIX_WAVES_PER_SIMD : 2) void k_march(
     const MarchArgs<M> A) {
   using T = typename ModeOf<M>::T;
   using L = typename ModeOf<M>::L;

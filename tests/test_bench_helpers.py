@@ -1,0 +1,32 @@
+"""bench.py's host-side helpers: the ideal queue models its serving report compares with."""
+import numpy as np
+
+from bench import queue_model
+
+
+def test_fifo_closed_forms():
+    # simultaneous arrivals: each waits for all before it
+    np.testing.assert_allclose(queue_model(np.zeros(4), np.array([1.0, 2.0, 3.0, 4.0]), "fifo"), [1, 3, 6, 10])
+    # arrivals spaced wider than the service: no waiting
+    np.testing.assert_allclose(queue_model(np.array([0.0, 5.0, 10.0]), np.array([1.0, 2.0, 3.0]), "fifo"), [1, 2, 3])
+
+
+def test_processor_sharing_closed_forms():
+    # n equal jobs arriving together all finish at n * s
+    np.testing.assert_allclose(queue_model(np.zeros(3), np.ones(3), "ps"), [3, 3, 3])
+    # two jobs together, sizes 1 and 2: the short one ends at 2, the long one at 3
+    np.testing.assert_allclose(queue_model(np.zeros(2), np.array([1.0, 2.0]), "ps"), [2, 3])
+    # staggered arrivals: worked by hand (shares 1, then 1/2, then 1/3, then 1/2)
+    np.testing.assert_allclose(queue_model(np.array([0.0, 0.1, 0.2]), np.ones(3), "ps"), [2.75, 2.85, 2.8])
+
+
+def test_work_conservation():
+    # both disciplines finish the last request at the same time when the server never idles
+    rng = np.random.default_rng(1)
+    svc = rng.exponential(1.0, 50)
+    arr = np.sort(rng.uniform(0, 5, 50))
+    end_f = (queue_model(arr, svc, "fifo") + arr).max()
+    end_p = (queue_model(arr, svc, "ps") + arr).max()
+    assert abs(end_f - end_p) < 1e-9
+    # Little's law style check: FIFO's mean latency is at least the mean service time
+    assert queue_model(arr, svc, "fifo").mean() >= svc.mean()
