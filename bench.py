@@ -298,11 +298,13 @@ def bench_serve(capi, prec, tmpdir) -> dict:
                    "throughput_rays_per_s": st["rays"] / max(st["wall_s"], 1e-9), "launches": st["launches"],
                    "peak_result_bytes": st["peak_bytes"]}
             if order == 0:
-                # the same arrivals through ideal servers with the one-at-a-time latencies as service
-                # times: how much of the latency the queueing itself accounts for
+                # the same arrivals through ideal servers with no overhead that march rays at the
+                # measured saturation rate (a request's service time: its rays / that rate): how
+                # much of the latency the queueing itself accounts for at this load
                 arrive = np.cumsum(gaps)
+                service = np.array([r.img_width * r.img_height * r.nfreqs for _, r in arrivals]) / out["saturation_rays_per_s"]
                 for disc in ("fifo", "ps"):
-                    ideal = queue_model(arrive, single, disc)
+                    ideal = queue_model(arrive, service, disc)
                     res[f"ideal_{disc}_p99_ms"] = float(np.percentile(ideal, 99)) * 1e3
                     res[f"ideal_{disc}_p99_vs_single_p99"] = res[f"ideal_{disc}_p99_ms"] / out["single_p99_ms"]
             out[name] = res
