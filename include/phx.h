@@ -224,6 +224,25 @@ int phx_snapshot_render_batch(phx_snapshot* snap, const phx_model* const* reques
 int phx_snapshot_render_noalloc(phx_snapshot* snap, const phx_model* request, int32_t precision,
                                 phx_freqout* d_res, phx_info* info);
 
+/* Captured render (HIP graph).  No counterpart in the reference, which launches its kernel once per
+ * chunk of jobs from the host (kernel.cu:332-344).  phx_graph_create records the allocation-free
+ * render of `request` (setup, counter resets, camera rays, side arguments, march: kernels only) into
+ * a HIP graph on the snapshot's stream and instantiates it; results go to d_res, a device buffer
+ * the caller owns (img_width * img_height * nfreqs entries, as phx_snapshot_render_noalloc).
+ * phx_graph_launch stores the parameter point's tasks (one small kernel), replays the graph with
+ * one hipGraphLaunch and waits for it: for `request` NULL the parameter point it holds, else a
+ * new parameter-sweep point of the same camera, which may change
+ * m_unit, R_high and the frequency values (units re-derived with phx_model_derive) but not the
+ * camera, the image size, the number of frequencies or the launch-wide integration and cut
+ * parameters (PHX_E_ARG).  The limits of phx_snapshot_render_noalloc apply, and replays are
+ * serialized with its calls on the GPU (they share its device array).  Destroy before closing the
+ * snapshot. */
+typedef struct phx_graph phx_graph;
+int phx_graph_create(phx_snapshot* snap, const phx_model* request, int32_t precision, phx_freqout* d_res,
+                     phx_graph** out);
+int phx_graph_launch(phx_graph* graph, const phx_model* request, phx_info* info);
+int phx_graph_destroy(phx_graph* graph);
+
 /* launch configuration of batch renders */
 typedef struct phx_tune {
   int32_t queue;         /* -1 default (in place), 0 samples processed in place, 1 per-wave LDS sample queue */

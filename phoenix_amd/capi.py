@@ -152,6 +152,10 @@ SIGNATURES = {
     "phx_snapshot_render_noalloc": (c_int, [ctypes.c_void_p, ctypes.POINTER(Model), c_int, ctypes.c_void_p,
                                             ctypes.POINTER(Info)]),
     "phx_snapshot_prims": (c_int, [ctypes.c_void_p, c_int, P_dbl, c_int64]),
+    "phx_graph_create": (c_int, [ctypes.c_void_p, ctypes.POINTER(Model), c_int, ctypes.c_void_p,
+                                 ctypes.POINTER(ctypes.c_void_p)]),
+    "phx_graph_launch": (c_int, [ctypes.c_void_p, ctypes.POINTER(Model), ctypes.POINTER(Info)]),
+    "phx_graph_destroy": (c_int, [ctypes.c_void_p]),
 }
 
 
@@ -377,12 +381,45 @@ class Snapshot:
                                                 ctypes.byref(info)))
         return info.as_dict()
 
+    def graph(self, request: Model, precision: int, d_res: int) -> "Graph":
+        """phx_graph_create: the allocation-free render of request captured as a HIP graph, results
+        into the device buffer at address d_res."""
+        return Graph(self, request, precision, d_res)
+
     def prims(self, table: int) -> np.ndarray:
         m = self.run.model
         n = m.nleafs * m.cells * 8
         out = np.zeros(n)
         check(lib().phx_snapshot_prims(self.h, table, out.ctypes.data_as(P_dbl), n))
         return out.reshape(m.nleafs, m.cells, 8)
+
+
+class Graph:
+    """phx_graph_*: a captured render replayed for parameter-sweep points of one camera."""
+
+    def __init__(self, snap: Snapshot, request: Model, precision: int, d_res: int):
+        self.snap = snap
+        self.h = ctypes.c_void_p()
+        check(lib().phx_graph_create(snap.h, ctypes.byref(request), precision, ctypes.c_void_p(d_res),
+                                     ctypes.byref(self.h)))
+
+    def launch(self, request: Model | None = None) -> dict:
+        """phx_graph_launch: replays the graph (for request, a new sweep point, when given); returns the info."""
+        info = Info()
+        check(lib().phx_graph_launch(self.h, ctypes.byref(request) if request is not None else None,
+                                     ctypes.byref(info)))
+        return info.as_dict()
+
+    def close(self):
+        if self.h:
+            check(lib().phx_graph_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:

@@ -843,6 +843,7 @@ struct LaunchInfo {
   int njobs = 0, ntasks = 0, nfmax = 1;
   const phx_model* uni = nullptr;  // the request whose launch-wide parameters (stepping, cuts) apply
   bool no_alloc = false;           // the allocation-free entry point: no device allocation may happen
+  bool capture = false;            // being recorded into a HIP graph: no timing events
 };
 
 template <int M, int NF, bool Q, int BS>
@@ -1003,9 +1004,9 @@ template <int M>
 static void enqueue_march(phx_snapshot* s, Slot& sl, const LaunchInfo& li, const Tune& tn, phx_info* info,
                           phx_progress_fn progress, void* user) {
   k_reset<<<1, 64, 0, s->stream>>>(sl.counter, sl.tot);
-  HIPCHECK(hipEventRecord(sl.e0, s->stream));
+  if (!li.capture) HIPCHECK(hipEventRecord(sl.e0, s->stream));
   march_jobs<M>(s, sl, li, tn, info, progress, user);
-  HIPCHECK(hipEventRecord(sl.e1, s->stream));
+  if (!li.capture) HIPCHECK(hipEventRecord(sl.e1, s->stream));
 }
 static void enqueue_precision(phx_snapshot* s, Slot& sl, int precision, const LaunchInfo& li, const Tune& tn,
                               phx_info* info, phx_progress_fn progress, void* user) {
@@ -1329,10 +1330,15 @@ static void* scratch_on(int device) {
 // (every pixel of the image for every task, in the Morton order of the reference's GPU schedule,
 // jobs.hpp:18-46: pixels sorted by the key with i in the even bits and j in the odd bits)
 template <class T>
+struct TaskArray {
+  TaskDev<T> t[PHX_FREQS_MAX / CHUNK];
+};
+template <class T>
 struct NoAllocSetup {
-  TaskDev<T> tasks[PHX_FREQS_MAX / CHUNK];
+  TaskArray<T> tasks;
   Cam cams[PHX_FREQS_MAX / CHUNK];
   int ntasks, npix, W, H, levels;  // levels: log2 of the power-of-two square that holds the image
+  int write_tasks;                 // 0: the tasks are stored by a launch of their own (a captured render)
 };
 // the n-th pixel (i, j) of a W x H image in Morton order, by descending the quadtree of the
 // enclosing 2^levels square: at each level the four children in key order (i bit low, j bit high)
@@ -1359,7 +1365,7 @@ __global__ __launch_bounds__(BLOCK) void k_noalloc_setup(const NoAllocSetup<T> a
                                                          Cam* __restrict__ cams, int2* __restrict__ jobs) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n < a.ntasks) {
-    tasks[n] = a.tasks[n];
+    if (a.write_tasks) tasks[n] = a.tasks.t[n];
     cams[n] = a.cams[n];
   }
   const int njobs = a.npix * a.ntasks;
@@ -1370,12 +1376,10 @@ __global__ __launch_bounds__(BLOCK) void k_noalloc_setup(const NoAllocSetup<T> a
   }
 }
 
-template <class T>
-static void render_noalloc_t(phx_snapshot* s, const phx_model* rq, int precision, phx_freqout* d_res,
-                             phx_info* info) {
-  const int64_t npix = (int64_t)rq->img_width * rq->img_height;
-  const int ntasks = (rq->nfreqs + CHUNK - 1) / CHUNK;
-  const int64_t njobs = npix * ntasks;
+// The array's layout for a request: counter and statistics, side arguments, tasks, cameras; then
+// jobs and start records.  Returns the launch slot over those windows (its timing events are the
+// snapshot's own); refuses a request beyond the array's job capacity for its precision.
+static Slot noalloc_slot(phx_snapshot* s, const phx_model* rq, int precision, int64_t njobs) {
   const size_t rec = precision == PHX_FP64 ? StRef<double>::REC * sizeof(double)
                      : precision == PHX_MIXED ? StHam::REC * sizeof(double)
                                               : StRef<float>::REC * sizeof(float);
@@ -1384,7 +1388,6 @@ static void render_noalloc_t(phx_snapshot* s, const phx_model* rq, int precision
     throw Error(PHX_E_RANGE, "allocation-free render: " + std::to_string(njobs) + " jobs (pixels x frequency "
                                  "chunks of 4) above its maximum of " + std::to_string(max_jobs) + " for this precision");
   char* base = static_cast<char*>(scratch_on(s->device));
-  // the array's layout: counter and statistics, side arguments, tasks, cameras; then jobs and start records
   Slot sl;
   sl.counter = reinterpret_cast<int*>(base);
   sl.tot = reinterpret_cast<unsigned long long*>(base + 256);
@@ -1402,21 +1405,39 @@ static void render_noalloc_t(phx_snapshot* s, const phx_model* rq, int precision
   sl.e1 = s->slot.e1;
   static_assert(sizeof(TaskDev<double>) * (PHX_FREQS_MAX / CHUNK) <= 16384, "tasks window");
   static_assert(sizeof(Cam) * (PHX_FREQS_MAX / CHUNK) <= 16384, "cameras window");
+  (void)rq;
+  return sl;
+}
+
+// the setup kernel's arguments for request rq: its tasks (results into d_res) and cameras
+template <class T>
+static NoAllocSetup<T> noalloc_args(phx_snapshot* s, const phx_model* rq, phx_freqout* d_res) {
   NoAllocSetup<T> a;
   std::memset(&a, 0, sizeof a);
   std::vector<TaskDev<T>> tasks;
-  add_tasks<T>(rq, d_res, tasks);
+  const int ntasks = add_tasks<T>(rq, d_res, tasks);
   const Cam cam = cam_of(s->geo, rq);
   for (int t = 0; t < ntasks; t++) {
-    a.tasks[t] = tasks[t];
+    a.tasks.t[t] = tasks[t];
     a.cams[t] = cam;
   }
   a.ntasks = ntasks;
-  a.npix = (int)npix;
+  a.write_tasks = 1;
+  a.npix = rq->img_width * rq->img_height;
   a.W = rq->img_width;
   a.H = rq->img_height;
   a.levels = 0;
   while ((1 << a.levels) < std::max(a.W, a.H)) a.levels++;
+  return a;
+}
+
+// The allocation-free render's launch sequence on the snapshot's stream: the setup kernel (tasks,
+// cameras, Morton job list), then the counter resets, camera rays, side arguments and march.  With
+// capture set it is being recorded into a HIP graph: no timing events are recorded.
+template <class T>
+static LaunchInfo enqueue_noalloc(phx_snapshot* s, Slot& sl, const phx_model* rq, int precision,
+                                  const NoAllocSetup<T>& a, bool capture, phx_info* info) {
+  const int64_t njobs = (int64_t)a.npix * a.ntasks;
   const int grid = (int)std::min<int64_t>(4096, (njobs + BLOCK - 1) / BLOCK);
   k_noalloc_setup<T><<<std::max(grid, 1), BLOCK, 0, s->stream>>>(a, static_cast<TaskDev<T>*>(sl.tasks.p),
                                                                   static_cast<Cam*>(sl.cams.p),
@@ -1424,17 +1445,29 @@ static void render_noalloc_t(phx_snapshot* s, const phx_model* rq, int precision
   HIPCHECK(hipGetLastError());
   LaunchInfo li;
   li.njobs = (int)njobs;
-  li.ntasks = ntasks;
+  li.ntasks = a.ntasks;
   li.nfmax = std::min(CHUNK, rq->nfreqs) <= 1 ? 1 : std::min(CHUNK, rq->nfreqs) <= 2 ? 2 : 4;
   li.uni = rq;
   li.no_alloc = true;
+  li.capture = capture;
   Tune tn = s->tune;
   tn.queue = 0;  // samples in place: the LDS-queue kernels carry per-lane scratch with several frequencies
+  enqueue_precision(s, sl, precision, li, tn, info, nullptr, nullptr);
+  return li;
+}
+
+template <class T>
+static void render_noalloc_t(phx_snapshot* s, const phx_model* rq, int precision, phx_freqout* d_res,
+                             phx_info* info) {
+  const int64_t npix = (int64_t)rq->img_width * rq->img_height;
+  const int64_t njobs = npix * ((rq->nfreqs + CHUNK - 1) / CHUNK);
+  Slot sl = noalloc_slot(s, rq, precision, njobs);
+  const NoAllocSetup<T> a = noalloc_args<T>(s, rq, d_res);
   phx_info local;
   if (!info) info = &local;
   std::memset(info, 0, sizeof *info);
   fill_info_device(s, info);
-  enqueue_precision(s, sl, precision, li, tn, info, nullptr, nullptr);
+  const LaunchInfo li = enqueue_noalloc<T>(s, sl, rq, precision, a, false, info);
   finish_march(s, sl, li.njobs, info);
   info->rays = npix * rq->nfreqs;
 }
@@ -1451,6 +1484,121 @@ static void render_noalloc(phx_snapshot* s, const phx_model* rq, int precision, 
     render_noalloc_t<double>(s, rq, precision, d_res, info);
   else
     render_noalloc_t<float>(s, rq, precision, d_res, info);
+}
+
+// ============================================================================ captured renders
+// A render captured once into a HIP graph and replayed.  The graph records the allocation-free
+// render's launch sequence (kernels only: no memset or host copy nodes, so every replay starts from
+// the same state: the setup kernel rewrites cameras and jobs, and kernels clear the counters) and is
+// instantiated once.  The parameter point (plasma units and frequencies, the request's tasks) is
+// not part of the graph: each replay is one small launch that stores the point's tasks into the
+// device array, then one hipGraphLaunch, between two timing events; the graph itself never
+// changes.  Replays share the allocation-free render's device array and are serialized with its
+// calls on the device.
+struct GraphRender {
+  phx_snapshot* s = nullptr;
+  int precision = 0;
+  phx_model rq{};  // the request of the last replay (camera, image and launch-wide fields fixed)
+  phx_freqout* d_res = nullptr;
+  Slot sl;
+  LaunchInfo li;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  phx_info kinfo{};  // the march kernel's launch figures (registers, grid), noted at capture
+};
+
+// a replay's request may change only the plasma units and the frequency values
+static void check_sweep_point(const phx_model* a, const phx_model* b) {
+  check_uniform(a, b);
+  const bool same = a->img_width == b->img_width && a->img_height == b->img_height && a->nfreqs == b->nfreqs &&
+                    a->inclination == b->inclination && a->fov_x == b->fov_x && a->fov_y == b->fov_y &&
+                    a->r_cam == b->r_cam && a->t_init == b->t_init && a->phi_cam == b->phi_cam &&
+                    a->cos_pa == b->cos_pa && a->sin_pa == b->sin_pa;
+  if (!same)
+    throw Error(PHX_E_ARG, "a replay of a captured render may change only m_unit, R_high and the frequency values "
+                           "(not the camera, the image size or the number of frequencies)");
+}
+
+template <class T>
+static void graph_capture(GraphRender* g) {
+  phx_snapshot* s = g->s;
+  const int64_t njobs = (int64_t)g->rq.img_width * g->rq.img_height * ((g->rq.nfreqs + CHUNK - 1) / CHUNK);
+  g->sl = noalloc_slot(s, &g->rq, g->precision, njobs);
+  NoAllocSetup<T> a = noalloc_args<T>(s, &g->rq, g->d_res);
+  a.write_tasks = 0;
+  HIPCHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
+  try {
+    g->li = enqueue_noalloc<T>(s, g->sl, &g->rq, g->precision, a, true, &g->kinfo);
+  } catch (...) {
+    hipGraph_t dropped = nullptr;
+    (void)hipStreamEndCapture(s->stream, &dropped);
+    if (dropped) (void)hipGraphDestroy(dropped);
+    throw;
+  }
+  g->li.uni = nullptr;  // pointed at the caller's request
+  HIPCHECK(hipStreamEndCapture(s->stream, &g->graph));
+  HIPCHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+}
+
+// stores request rq's tasks (its plasma units and frequencies; results into d_res) where the
+// graph's march reads them
+template <class T>
+static void graph_store_point(GraphRender* g, const phx_model* rq) {
+  const NoAllocSetup<T> a = noalloc_args<T>(g->s, rq, g->d_res);
+  k_store<TaskArray<T>><<<1, 1, 0, g->s->stream>>>(a.tasks, static_cast<TaskArray<T>*>(g->sl.tasks.p));
+  HIPCHECK(hipGetLastError());
+}
+
+static void graph_free(GraphRender* g) {
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  g->exec = nullptr;
+  g->graph = nullptr;
+}
+
+static void graph_create(GraphRender* g, phx_snapshot* s, const phx_model* rq, int precision, phx_freqout* d_res) {
+  check_precision(s, precision);
+  check_request(rq);
+  if (!d_res) throw Error(PHX_E_ARG, "d_res must be a device buffer of img_width * img_height * nfreqs results");
+  if (s->device < 0 || s->device >= 64) throw Error(PHX_E_RANGE, "device index above 63");
+  g->s = s;
+  g->precision = precision;
+  g->rq = *rq;
+  g->d_res = d_res;
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::lock_guard<std::mutex> dk(g_noalloc[s->device].mu);
+  HIPCHECK(hipSetDevice(s->device));
+  try {
+    if (precision == PHX_FP64)
+      graph_capture<double>(g);
+    else
+      graph_capture<float>(g);
+  } catch (...) {
+    graph_free(g);
+    throw;
+  }
+}
+
+static void graph_launch(GraphRender* g, const phx_model* rq, phx_info* info) {
+  phx_snapshot* s = g->s;
+  if (rq) check_sweep_point(&g->rq, rq);
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::lock_guard<std::mutex> dk(g_noalloc[s->device].mu);
+  HIPCHECK(hipSetDevice(s->device));
+  if (rq) g->rq = *rq;
+  phx_info local;
+  if (!info) info = &local;
+  *info = g->kinfo;
+  fill_info_device(s, info);
+  HIPCHECK(hipEventRecord(g->sl.e0, s->stream));
+  if (g->precision == PHX_FP64)
+    graph_store_point<double>(g, &g->rq);
+  else
+    graph_store_point<float>(g, &g->rq);
+  HIPCHECK(hipGraphLaunch(g->exec, s->stream));
+  HIPCHECK(hipEventRecord(g->sl.e1, s->stream));
+  finish_march(s, g->sl, g->li.njobs, info);
+  info->rays = (int64_t)g->rq.img_width * g->rq.img_height * g->rq.nfreqs;
 }
 
 // ============================================================================ serving loop
@@ -1857,6 +2005,43 @@ int phx_snapshot_render_noalloc(phx_snapshot* s, const phx_model* rq, int32_t pr
   return guarded([&] {
     if (!s) throw Error(PHX_E_ARG, "null snapshot");
     render_noalloc(s, rq, precision, d_res, info);
+  });
+}
+
+}  // extern "C"
+
+struct phx_graph {
+  phx::GraphRender g;
+};
+
+extern "C" {
+
+int phx_graph_create(phx_snapshot* s, const phx_model* rq, int32_t precision, phx_freqout* d_res, phx_graph** out) {
+  return guarded([&] {
+    if (!s || !out) throw Error(PHX_E_ARG, "null snapshot or output handle");
+    *out = nullptr;
+    std::unique_ptr<phx_graph> h(new phx_graph());
+    graph_create(&h->g, s, rq, precision, d_res);
+    *out = h.release();
+  });
+}
+
+int phx_graph_launch(phx_graph* h, const phx_model* rq, phx_info* info) {
+  return guarded([&] {
+    if (!h) throw Error(PHX_E_ARG, "null graph");
+    graph_launch(&h->g, rq, info);
+  });
+}
+
+int phx_graph_destroy(phx_graph* h) {
+  return guarded([&] {
+    if (!h) return;
+    {
+      std::lock_guard<std::mutex> lk(h->g.s->mu);
+      HIPCHECK(hipSetDevice(h->g.s->device));
+      graph_free(&h->g);
+    }
+    delete h;
   });
 }
 

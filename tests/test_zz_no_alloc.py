@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from phoenix_amd import capi, configs, snapshot as S
+from tests.devbuf import DeviceBuffer
 
 pytestmark = pytest.mark.gpu
 
@@ -19,34 +20,6 @@ def c1(snapdir, tmp_path_factory):
     yield run, snap
     snap.close()
     run.close()
-
-
-class DeviceBuffer:
-    """n FreqOut entries of device memory from the HIP runtime libphx.so itself uses (torch ships
-    a runtime of its own, and two runtimes in one process do not share devices)."""
-
-    def __init__(self, n):
-        import ctypes
-
-        capi.lib()  # libphx.so first: its libamdhip64.so.7 is the one loaded below
-        self.hip = ctypes.CDLL("libamdhip64.so.7")
-        self.n = n
-        self.ptr = ctypes.c_void_p()
-        assert self.hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(n * 48)) == 0
-        fill = np.full((n, 6), np.nan)
-        assert self.hip.hipMemcpy(self.ptr, fill.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(n * 48), 1) == 0
-
-    def get(self):
-        import ctypes
-
-        out = np.empty((self.n, 6))
-        assert self.hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), self.ptr, ctypes.c_size_t(self.n * 48), 2) == 0
-        return out
-
-    def free(self):
-        if self.ptr:
-            self.hip.hipFree(self.ptr)
-            self.ptr = None
 
 
 @pytest.mark.parametrize("precision,freqs", [(capi.MIXED, (230e9,)), (capi.FP32, (230e9, 345e9)),
