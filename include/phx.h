@@ -274,8 +274,9 @@ int phx_snapshot_set_tune(phx_snapshot* snap, const phx_tune* tune);
  * serves render requests on resident snapshots with continuous batching: requests are admitted
  * in arrival order while their result buffers fit hbm_budget; every launch marches the next
  * quantum_jobs jobs (pixels x frequency chunks) across all admitted requests of a snapshot, so
- * requests arriving while others are on the GPU join the next launch; snapshots with work run
- * on their own streams, each with two launches queued (one runs while the next is built).
+ * requests arriving while others are on the GPU join the next launch; each snapshot keeps two
+ * launches queued, on two streams, so one runs while the next is built and the next fills the
+ * compute units the running one's last rays leave idle.
  * Result buffers come from one pool of hbm_budget bytes allocated at start (no allocation per
  * request).  Latency = submit -> result copied to the caller's buffer.
  * The requests of one snapshot must share the launch-wide parameters of a batch render. */

@@ -560,6 +560,7 @@ struct Slot {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int last_jobs = 0;
   void* ctl = nullptr;  // the allocation holding counter and tot (null when they live elsewhere)
+  hipStream_t stream = nullptr;  // the stream its launches run on (null: the snapshot's)
 };
 
 }  // namespace phx
@@ -587,6 +588,9 @@ struct phx_snapshot {
 };
 
 namespace phx {
+
+// the stream a slot's launches run on
+static hipStream_t stream_of(const phx_snapshot* s, const Slot& sl) { return sl.stream ? sl.stream : s->stream; }
 
 static int g_device = 0;
 static std::thread* g_warm = nullptr;
@@ -637,6 +641,7 @@ static void slot_free(Slot& sl) {
   if (sl.ctl) (void)hipFree(sl.ctl);
   if (sl.e0) (void)hipEventDestroy(sl.e0);
   if (sl.e1) (void)hipEventDestroy(sl.e1);
+  if (sl.stream) (void)hipStreamDestroy(sl.stream);
   sl = Slot();
 }
 
@@ -857,7 +862,7 @@ static void launch_march(phx_snapshot* s, Slot& sl, const LaunchInfo& li, int j0
   const LaunchBufs bufs{sl.tasks.p, (const int2*)sl.jobs.p, sl.start.p, sl.counter, sl.tot,
                         ensure(sl.side, sizeof side)};
   fill_args<M>(A, side, grid_of<T>(s, tables_of<T>(s)), GL, s->geo, li.uni, tn, bufs, j0, j1);
-  k_store<SideArgs<M>><<<1, 1, 0, s->stream>>>(side, static_cast<SideArgs<M>*>(const_cast<void*>(bufs.side)));
+  k_store<SideArgs<M>><<<1, 1, 0, stream_of(s, sl)>>>(side, static_cast<SideArgs<M>*>(const_cast<void*>(bufs.side)));
   const void* kfn = reinterpret_cast<const void*>(&k_march<M, NF, Q, BS>);
   int cus = 0, per_cu = 0;
   HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
@@ -880,8 +885,8 @@ static void launch_march(phx_snapshot* s, Slot& sl, const LaunchInfo& li, int j0
   if (fa.localSizeBytes > limit) {
     HIPCHECK(hipDeviceSetLimit(hipLimitStackSize, (fa.localSizeBytes + 255) & ~size_t(255)));
   }
-  k_reset<<<1, 64, 0, s->stream>>>(sl.counter, nullptr);
-  k_march<M, NF, Q, BS><<<grid, BS, 0, s->stream>>>(A);
+  k_reset<<<1, 64, 0, stream_of(s, sl)>>>(sl.counter, nullptr);
+  k_march<M, NF, Q, BS><<<grid, BS, 0, stream_of(s, sl)>>>(A);
   HIPCHECK(hipGetLastError());
   sl.last_jobs = njobs;
   if (info) {
@@ -917,7 +922,7 @@ static void march_jobs(phx_snapshot* s, Slot& sl, const LaunchInfo& li, const Tu
   const int rec = M == MIX ? StHam::REC : StRef<typename ModeOf<M>::T>::REC;
   ensure(sl.start, (size_t)std::max(li.njobs, 1) * rec * sizeof(R));
   if (li.njobs > 0)
-    k_rays<M><<<(li.njobs + BLOCK - 1) / BLOCK, BLOCK, 0, s->stream>>>((const Cam*)sl.cams.p,
+    k_rays<M><<<(li.njobs + BLOCK - 1) / BLOCK, BLOCK, 0, stream_of(s, sl)>>>((const Cam*)sl.cams.p,
                                                                       (const int2*)sl.jobs.p, li.njobs,
                                                                       sl.start.p);
   HIPCHECK(hipGetLastError());
@@ -933,7 +938,7 @@ static void march_jobs(phx_snapshot* s, Slot& sl, const LaunchInfo& li, const Tu
     else
       launch_nf<M, 4>(s, sl, li, j0, j1, tn, info);
     if (progress) {
-      HIPCHECK(hipStreamSynchronize(s->stream));
+      HIPCHECK(hipStreamSynchronize(stream_of(s, sl)));
       progress(j1 - j0, user);
     }
   }
@@ -1003,10 +1008,10 @@ static void fill_info_device(const phx_snapshot* s, phx_info* info) {
 template <int M>
 static void enqueue_march(phx_snapshot* s, Slot& sl, const LaunchInfo& li, const Tune& tn, phx_info* info,
                           phx_progress_fn progress, void* user) {
-  k_reset<<<1, 64, 0, s->stream>>>(sl.counter, sl.tot);
-  if (!li.capture) HIPCHECK(hipEventRecord(sl.e0, s->stream));
+  k_reset<<<1, 64, 0, stream_of(s, sl)>>>(sl.counter, sl.tot);
+  if (!li.capture) HIPCHECK(hipEventRecord(sl.e0, stream_of(s, sl)));
   march_jobs<M>(s, sl, li, tn, info, progress, user);
-  if (!li.capture) HIPCHECK(hipEventRecord(sl.e1, s->stream));
+  if (!li.capture) HIPCHECK(hipEventRecord(sl.e1, stream_of(s, sl)));
 }
 static void enqueue_precision(phx_snapshot* s, Slot& sl, int precision, const LaunchInfo& li, const Tune& tn,
                               phx_info* info, phx_progress_fn progress, void* user) {
@@ -1054,8 +1059,8 @@ static void upload_tasks(phx_snapshot* s, Slot& sl, const std::vector<TaskDev<T>
   ensure(sl.tasks, tasks.size() * sizeof(TaskDev<T>));
   ensure(sl.cams, cams.size() * sizeof(Cam));
   HIPCHECK(hipMemcpyAsync(sl.tasks.p, tasks.data(), tasks.size() * sizeof(TaskDev<T>), hipMemcpyHostToDevice,
-                          s->stream));
-  HIPCHECK(hipMemcpyAsync(sl.cams.p, cams.data(), cams.size() * sizeof(Cam), hipMemcpyHostToDevice, s->stream));
+                          stream_of(s, sl)));
+  HIPCHECK(hipMemcpyAsync(sl.cams.p, cams.data(), cams.size() * sizeof(Cam), hipMemcpyHostToDevice, stream_of(s, sl)));
 }
 
 // One request: its frequency chunks become tasks over the same pixel list (the full image in
@@ -1607,9 +1612,10 @@ static void graph_launch(GraphRender* g, const phx_model* rq, phx_info* info) {
 // server's result pool, allocated once); its jobs (pixels x frequency chunks) join the active
 // work of its snapshot, and every launch takes the next `quantum` jobs across the active requests
 // of a snapshot, so requests that arrive while others are on the GPU share the following launches
-// with them.  Each snapshot has two launch slots: while one launch runs, the next is built and
-// queued behind it on the snapshot's stream, so the GPU does not wait for the host between
-// launches; snapshots run on their own streams and overlap on the device.  A request is done when
+// with them.  Each snapshot has two launch slots, each on a stream of its own: while one launch
+// runs, the next is built and queued, so the GPU does not wait for the host between launches, and
+// the next launch's waves take the compute units that the running launch's last rays leave idle
+// (PHX_SERVE_STREAMS=0 queues both on the snapshot's stream instead).  A request is done when
 // its last job has been marched; its result is copied to the caller's buffer on a copy stream of
 // its own (concurrently with the running launch) and its latency (submit -> copied) recorded.
 // Jobs are taken oldest request first (order 0), or least attained service first (order 1: the
@@ -1716,6 +1722,15 @@ struct Server {
   std::deque<Flight> flights;  // queued launches, oldest first
 };
 
+// waits for everything the server queued: every snapshot's stream and every launch slot's
+static void sync_launches(Server* sv) {
+  for (int k = 0; k < (int)sv->snaps.size(); k++) {
+    (void)hipStreamSynchronize(sv->snaps[k]->stream);
+    for (Slot& sl : sv->slots[k])
+      if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+  }
+}
+
 static void fail_request(Server* sv, Request* r, const std::string& why) {  // sv->mu held
   r->state = 3;
   r->error = why;
@@ -1762,7 +1777,7 @@ static bool launch_tick(Server* sv, int k, const std::vector<std::shared_ptr<Req
   HIPCHECK(hipSetDevice(s->device));
   upload_tasks(s, sl, tasks, cams);
   ensure(sl.jobs, jobs.size() * sizeof(int2));
-  HIPCHECK(hipMemcpyAsync(sl.jobs.p, jobs.data(), jobs.size() * sizeof(int2), hipMemcpyHostToDevice, s->stream));
+  HIPCHECK(hipMemcpyAsync(sl.jobs.p, jobs.data(), jobs.size() * sizeof(int2), hipMemcpyHostToDevice, stream_of(s, sl)));
   LaunchInfo li;
   li.njobs = fl.njobs;
   li.ntasks = (int)tasks.size();
@@ -1888,7 +1903,7 @@ static void serve_loop(Server* sv) {
           complete_request(sv, x.r);
     } catch (const std::exception& e) {
       // a device error: drain what is queued, fail every admitted request, refuse new ones
-      for (int k = 0; k < (int)sv->snaps.size(); k++) (void)hipStreamSynchronize(sv->snaps[k]->stream);
+      sync_launches(sv);
       sv->flights.clear();
       for (auto& b : sv->slot_busy) b[0] = b[1] = 0;
       std::vector<std::shared_ptr<Request>> failed;
@@ -1905,7 +1920,7 @@ static void serve_loop(Server* sv) {
     sv->cv_done.notify_all();
   }
   // stopping: the queued launches finish before their buffers go away
-  for (int k = 0; k < (int)sv->snaps.size(); k++) (void)hipStreamSynchronize(sv->snaps[k]->stream);
+  sync_launches(sv);
   sv->flights.clear();
 }
 
@@ -2091,8 +2106,15 @@ int phx_server_start(phx_snapshot* const* snaps, int32_t nsnaps, const phx_serve
     sv.slots.resize(nsnaps);
     sv.slot_busy.assign(nsnaps, std::array<char, 2>{0, 0});
     try {
+      const char* ps = std::getenv("PHX_SERVE_STREAMS");
+      const bool own_streams = !ps || std::atoi(ps) != 0;
       for (auto& pair : sv.slots)
-        for (Slot& sl : pair) slot_open(sl);
+        for (Slot& sl : pair) {
+          slot_open(sl);
+          // each launch slot on a stream of its own: the next launch's waves fill the compute units
+          // that the running launch's last rays leave idle
+          if (own_streams) HIPCHECK(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+        }
       sv.pool.open((size_t)sv.opts.hbm_budget);
       HIPCHECK(hipStreamCreateWithFlags(&sv.copy_stream, hipStreamNonBlocking));
       HIPCHECK(hipMalloc(&sv.d_spec, PHX_FREQS_MAX * 4 * sizeof(double)));
