@@ -399,7 +399,25 @@ __global__ __launch_bounds__(BLOCK) void k_rays(const Cam* __restrict__ cams, co
 // memset, so that a launch sequence stays valid inside a captured HIP graph
 __global__ void k_reset(int* counter, unsigned long long* tot) {
   if (threadIdx.x == 0) counter[0] = 0;
+  if (tot && threadIdx.x == 1) counter[16] = 0;  // the record blocks taken (two-phase renders)
   if (tot && threadIdx.x < 4) tot[threadIdx.x] = 0ull;
+}
+
+// Phase 2 of a two-phase render: one lane per (job, frequency), lane n taking job n / F and
+// frequency n % F (F: the largest frequency count of the launch's requests), running
+// transfer_one; the dropped operators are counted per wave.
+template <int M>
+__global__ __launch_bounds__(256) void k_transfer(const TaskDev<typename ModeOf<M>::T>* __restrict__ tasks,
+                                                  const int2* __restrict__ jobs, int njobs, int F,
+                                                  const typename ModeOf<M>::T* __restrict__ rec,
+                                                  const int* __restrict__ rec_next, const int2* __restrict__ rray,
+                                                  unsigned long long* tot) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int job = (int)(n / F), f = (int)(n - (long long)job * F);
+  unsigned long long w = job < njobs ? (unsigned long long)transfer_one<M>(tasks, jobs, job, f, rec, rec_next, rray) : 0ull;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off);
+  if ((threadIdx.x & 63) == 0 && w) atomicAdd(&tot[2], w);
 }
 
 // one value to device memory (a launch's side arguments): a kernel rather than a copy from host
@@ -555,6 +573,8 @@ struct DevBuf {
 // while the other runs.
 struct Slot {
   DevBuf jobs, start, tasks, cams, side;
+  DevBuf fq, rec, rnext, rray;  // two-phase renders: frequency tables, record pool, block links, per-job records
+  int rec_cap = 0;              // record blocks the pool holds
   int* counter = nullptr;             // [0]: jobs taken by the launch in flight
   unsigned long long* tot = nullptr;  // steps, plasma segments, dropped segments, FP64 steps
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -636,7 +656,7 @@ static void slot_open(Slot& sl) {
   sl.tot = reinterpret_cast<unsigned long long*>(static_cast<char*>(sl.ctl) + 256);
 }
 static void slot_free(Slot& sl) {
-  for (DevBuf* b : {&sl.jobs, &sl.start, &sl.tasks, &sl.cams, &sl.side})
+  for (DevBuf* b : {&sl.jobs, &sl.start, &sl.tasks, &sl.cams, &sl.side, &sl.fq, &sl.rec, &sl.rnext, &sl.rray})
     if (b->p && !b->fixed) (void)hipFree(b->p);
   if (sl.ctl) (void)hipFree(sl.ctl);
   if (sl.e0) (void)hipEventDestroy(sl.e0);
@@ -849,6 +869,8 @@ struct LaunchInfo {
   const phx_model* uni = nullptr;  // the request whose launch-wide parameters (stepping, cuts) apply
   bool no_alloc = false;           // the allocation-free entry point: no device allocation may happen
   bool capture = false;            // being recorded into a HIP graph: no timing events
+  bool two_phase = false;          // march once recording samples, then one transfer lane per frequency
+  int64_t rec_blocks = 0;          // two-phase: record blocks to provide (0: the default per job)
 };
 
 template <int M, int NF, bool Q, int BS>
@@ -859,8 +881,14 @@ static void launch_march(phx_snapshot* s, Slot& sl, const LaunchInfo& li, int j0
   SideArgs<M> side;
   const Grid<double> g64 = grid_of<double>(s, tables_of<double>(s));
   const Loc<double> GL{g64.blk, g64.lo1, g64.lo2, g64.lo3, g64.hi1, g64.hi2, g64.hi3};
-  const LaunchBufs bufs{sl.tasks.p, (const int2*)sl.jobs.p, sl.start.p, sl.counter, sl.tot,
-                        ensure(sl.side, sizeof side)};
+  LaunchBufs bufs{sl.tasks.p, (const int2*)sl.jobs.p, sl.start.p, sl.counter, sl.tot, ensure(sl.side, sizeof side)};
+  if (li.two_phase) {
+    bufs.rec = sl.rec.p;
+    bufs.rec_next = static_cast<int*>(sl.rnext.p);
+    bufs.rec_used = sl.counter + 16;
+    bufs.rray = sl.rray.p;
+    bufs.rec_cap = sl.rec_cap;
+  }
   fill_args<M>(A, side, grid_of<T>(s, tables_of<T>(s)), GL, s->geo, li.uni, tn, bufs, j0, j1);
   k_store<SideArgs<M>><<<1, 1, 0, stream_of(s, sl)>>>(side, static_cast<SideArgs<M>*>(const_cast<void*>(bufs.side)));
   const void* kfn = reinterpret_cast<const void*>(&k_march<M, NF, Q, BS>);
@@ -905,12 +933,17 @@ static void launch_nf(phx_snapshot* s, Slot& sl, const LaunchInfo& li, int j0, i
                       phx_info* info) {
   // default: samples in place with whole-wave refill (measured faster in every precision: rays of
   // one 8 x 8 pixel tile stay coherent; the LDS queue pays off only with per-lane refill)
-  if (tn.queue == 1)
-    launch_march<M, NF, true, 256>(s, sl, li, j0, j1, tn, info);
-  else if (tn.block == 64)
-    launch_march<M, NF, false, 64>(s, sl, li, j0, j1, tn, info);
-  else
-    launch_march<M, NF, false, 256>(s, sl, li, j0, j1, tn, info);
+  // the recording march of a two-phase render (NF = 0) runs in place with 256-lane workgroups
+  if constexpr (NF == 0) {
+    launch_march<M, 0, false, 256>(s, sl, li, j0, j1, tn, info);
+  } else {
+    if (tn.queue == 1)
+      launch_march<M, NF, true, 256>(s, sl, li, j0, j1, tn, info);
+    else if (tn.block == 64)
+      launch_march<M, NF, false, 64>(s, sl, li, j0, j1, tn, info);
+    else
+      launch_march<M, NF, false, 256>(s, sl, li, j0, j1, tn, info);
+  }
 }
 
 // Ray start records for the jobs in sl.jobs (cameras in sl.cams), then the march over them,
@@ -926,6 +959,32 @@ static void march_jobs(phx_snapshot* s, Slot& sl, const LaunchInfo& li, const Tu
                                                                       (const int2*)sl.jobs.p, li.njobs,
                                                                       sl.start.p);
   HIPCHECK(hipGetLastError());
+  if (li.two_phase) {
+    // the record pool (grow-only): RB-record blocks, by default 6 per job (192 records a ray on
+    // average; a launch whose rays need more is run again with a larger pool, run_precision)
+    using T = typename ModeOf<M>::T;
+    const int64_t want = std::max<int64_t>(li.rec_blocks, (int64_t)li.njobs * 6 + 64);
+    if (want > INT32_MAX) throw Error(PHX_E_RANGE, "two-phase render: record pool above 2^31 blocks");
+    ensure(sl.rec, (size_t)want * RB * RECW * sizeof(T));
+    ensure(sl.rnext, (size_t)want * sizeof(int));
+    ensure(sl.rray, (size_t)std::max(li.njobs, 1) * sizeof(int2));
+    sl.rec_cap = (int)want;
+    if (li.njobs > 0) {
+      launch_nf<M, 0>(s, sl, li, 0, li.njobs, tn, info);
+      const int F = std::max(1, li.nfmax);
+      const int64_t lanes = (int64_t)li.njobs * F;
+      k_transfer<M><<<(unsigned)((lanes + 255) / 256), 256, 0, stream_of(s, sl)>>>(
+          static_cast<const TaskDev<T>*>(sl.tasks.p), (const int2*)sl.jobs.p, li.njobs, F,
+          static_cast<const T*>(sl.rec.p), static_cast<const int*>(sl.rnext.p), static_cast<const int2*>(sl.rray.p),
+          sl.tot);
+      HIPCHECK(hipGetLastError());
+    }
+    if (progress) {
+      HIPCHECK(hipStreamSynchronize(stream_of(s, sl)));
+      progress(li.njobs, user);
+    }
+    return;
+  }
   int chunk = std::max(li.njobs, 1);
   if (const char* e = std::getenv("PHX_CHUNK"))
     if (std::atoi(e) > 0) chunk = std::atoi(e);
@@ -1047,10 +1106,24 @@ static double finish_march(phx_snapshot* s, Slot& sl, int njobs, phx_info* info)
   }
   return ms * 1e-3;
 }
-static void run_precision(phx_snapshot* s, int precision, const LaunchInfo& li, const Tune& tn, phx_info* info,
+// The record blocks a two-phase launch took, when they were more than its pool held (0 otherwise):
+// its rays that found no block left recorded nothing, and it must run again with a larger pool.
+static int64_t record_overflow(Slot& sl, const LaunchInfo& li) {
+  if (!li.two_phase) return 0;
+  int used = 0;
+  HIPCHECK(hipMemcpy(&used, sl.counter + 16, sizeof used, hipMemcpyDeviceToHost));
+  return used > sl.rec_cap ? used : 0;
+}
+static void run_precision(phx_snapshot* s, int precision, const LaunchInfo& li0, const Tune& tn, phx_info* info,
                           phx_progress_fn progress, void* user) {
-  enqueue_precision(s, s->slot, precision, li, tn, info, progress, user);
-  finish_march(s, s->slot, li.njobs, info);
+  LaunchInfo li = li0;
+  for (;;) {
+    enqueue_precision(s, s->slot, precision, li, tn, info, progress, user);
+    finish_march(s, s->slot, li.njobs, info);
+    const int64_t used = record_overflow(s->slot, li);
+    if (!used) break;
+    li.rec_blocks = used + used / 4 + 64;
+  }
 }
 
 template <class T>
@@ -1063,8 +1136,49 @@ static void upload_tasks(phx_snapshot* s, Slot& sl, const std::vector<TaskDev<T>
   HIPCHECK(hipMemcpyAsync(sl.cams.p, cams.data(), cams.size() * sizeof(Cam), hipMemcpyHostToDevice, stream_of(s, sl)));
 }
 
+// Whether a request renders in two phases: each ray marched once, its emitting samples recorded,
+// then every frequency folded over the records by the transfer kernel, with the same results bit
+// for bit.  A march with 4 frequencies per lane holds 104 accumulator registers per lane and
+// spills, and above 4 every chunk of 4 marches the ray again: at 1024^2 on the c2 snapshot, mixed
+// precision, the two phases take 75.7 / 78.7 / 81.0 / 96.7 ms for 2 / 3 / 4 / 8 frequencies against
+// 76.9 / 121.0 / 128.4 / 229.1 ms in place.  PHX_TWO_PHASE_MIN_NF sets the threshold (0: never).
+static int two_phase_min_nf() {
+  const char* e = std::getenv("PHX_TWO_PHASE_MIN_NF");
+  return e ? std::atoi(e) : 2;
+}
+static bool two_phase(const phx_model* rq) { return two_phase_min_nf() > 0 && rq->nfreqs >= two_phase_min_nf(); }
+
+// the tasks of the requests of one launch and, for two-phase requests, their frequency tables
+// (uploaded to sl.fq; the tasks point into it)
+template <class T>
+static void build_tasks(phx_snapshot* s, Slot& sl, const phx_model* const* rqs, const phx_freqout* const* outs, int n,
+                        bool two, std::vector<TaskDev<T>>& tasks, std::vector<int>& first) {
+  first.assign(n, 0);
+  if (!two) {
+    for (int r = 0; r < n; r++) {
+      first[r] = (int)tasks.size();
+      add_tasks<T>(rqs[r], const_cast<phx_freqout*>(outs[r]), tasks);
+    }
+    return;
+  }
+  std::vector<T> fq;
+  std::vector<size_t> at(n);
+  for (int r = 0; r < n; r++) {
+    at[r] = fq.size();
+    freq_table<T>(rqs[r], fq);
+  }
+  const T* d_fq = static_cast<const T*>(ensure(sl.fq, fq.size() * sizeof(T)));
+  // a pageable source: the copy is staged before the call returns, so fq may go
+  HIPCHECK(hipMemcpyAsync(sl.fq.p, fq.data(), fq.size() * sizeof(T), hipMemcpyHostToDevice, stream_of(s, sl)));
+  for (int r = 0; r < n; r++) {
+    first[r] = (int)tasks.size();
+    add_task_all<T>(rqs[r], const_cast<phx_freqout*>(outs[r]), d_fq + at[r], tasks);
+  }
+}
+
 // One request: its frequency chunks become tasks over the same pixel list (the full image in
-// Morton order, cached on the device per image size, or the caller's jobs).
+// Morton order, cached on the device per image size, or the caller's jobs); a two-phase request
+// is one task over the pixel list.
 static void render(phx_snapshot* s, const phx_model* rq, int precision, const int32_t* jobs, int32_t njobs,
                    phx_freqout* res, phx_info* info, phx_progress_fn progress, void* user, const Tune& tn0) {
   check_precision(s, precision);
@@ -1098,14 +1212,19 @@ static void render(phx_snapshot* s, const phx_model* rq, int precision, const in
   }
   // tasks, cameras, jobs
   const Cam cam = cam_of(s->geo, rq);
+  const bool two = two_phase(rq);
   int ntasks = 0;
+  std::vector<int> first;
+  const phx_freqout* outs[1] = {d_res};
   if (precision == PHX_FP64) {
     std::vector<TaskDev<double>> tk;
-    ntasks = add_tasks<double>(rq, d_res, tk);
+    build_tasks<double>(s, s->slot, &rq, outs, 1, two, tk, first);
+    ntasks = (int)tk.size();
     upload_tasks(s, s->slot, tk, std::vector<Cam>(ntasks, cam));
   } else {
     std::vector<TaskDev<float>> tk;
-    ntasks = add_tasks<float>(rq, d_res, tk);
+    build_tasks<float>(s, s->slot, &rq, outs, 1, two, tk, first);
+    ntasks = (int)tk.size();
     upload_tasks(s, s->slot, tk, std::vector<Cam>(ntasks, cam));
   }
   const int64_t ntot = (int64_t)njobs * ntasks;
@@ -1123,8 +1242,9 @@ static void render(phx_snapshot* s, const phx_model* rq, int precision, const in
   LaunchInfo li;
   li.njobs = (int)ntot;
   li.ntasks = ntasks;
-  li.nfmax = std::min(CHUNK, rq->nfreqs) <= 1 ? 1 : std::min(CHUNK, rq->nfreqs) <= 2 ? 2 : 4;
+  li.nfmax = two ? rq->nfreqs : std::min(CHUNK, rq->nfreqs) <= 1 ? 1 : std::min(CHUNK, rq->nfreqs) <= 2 ? 2 : 4;
   li.uni = rq;
+  li.two_phase = two;
   run_precision(s, precision, li, tn, info, progress, user);
   info->rays = (int64_t)njobs * rq->nfreqs;
   s->last_nres = (int64_t)nres;
@@ -1135,64 +1255,68 @@ static void render(phx_snapshot* s, const phx_model* rq, int precision, const in
 
 // Many requests in one launch: every request's frequency chunks become tasks, and the job list
 // holds all their pixels (each request's in Morton order, requests one after another, or
-// interleaved 64 jobs at a time).  Results go to one device block per request.
+// interleaved 64 jobs at a time).  Results go to one device block per request.  Requests that
+// render in two phases (two_phase) form a launch of their own, after the others.
 struct BatchPlan {
   std::vector<int64_t> off;  // result offset of each request in s->res (entries)
   int64_t nres = 0, rays = 0;
   int njobs = 0, ntasks = 0, nfmax = 1;
 };
+// the launch of the requests idx (results at offsets off of d_res): tasks, cameras, jobs
 template <class T>
-static BatchPlan plan_batch(phx_snapshot* s, const phx_model* const* rqs, int n, int order, int sample_stride) {
+static BatchPlan plan_batch(phx_snapshot* s, const phx_model* const* rqs, const std::vector<int>& idx,
+                            const std::vector<int64_t>& off, phx_freqout* d_res, bool two, int order,
+                            int sample_stride) {
   BatchPlan bp;
-  bp.off.resize(n);
-  for (int r = 0; r < n; r++) {
-    check_request(rqs[r]);
-    check_uniform(rqs[0], rqs[r]);
-    bp.off[r] = bp.nres;
-    bp.nres += (int64_t)rqs[r]->img_width * rqs[r]->img_height * rqs[r]->nfreqs;
+  const int n = (int)idx.size();
+  std::vector<const phx_model*> sub(n);
+  std::vector<const phx_freqout*> outs(n);
+  for (int q = 0; q < n; q++) {
+    sub[q] = rqs[idx[q]];
+    outs[q] = d_res + off[idx[q]];
   }
-  phx_freqout* d_res = (phx_freqout*)ensure(s->res, (size_t)bp.nres * sizeof(phx_freqout));
   std::vector<TaskDev<T>> tasks;
+  std::vector<int> first;
+  build_tasks<T>(s, s->slot, sub.data(), outs.data(), n, two, tasks, first);
   std::vector<Cam> cams;
   std::vector<std::vector<int2>> per(n);
   std::vector<int32_t> pix;
-  for (int r = 0; r < n; r++) {
-    const int t0 = (int)tasks.size();
-    const int nt = add_tasks<T>(rqs[r], d_res + bp.off[r], tasks);
-    const Cam c = cam_of(s->geo, rqs[r]);
+  for (int q = 0; q < n; q++) {
+    const int t0 = first[q];
+    const int nt = (q + 1 < n ? first[q + 1] : (int)tasks.size()) - t0;
+    const Cam c = cam_of(s->geo, sub[q]);
     for (int t = 0; t < nt; t++) cams.push_back(c);
-    bp.nfmax = std::max(bp.nfmax, std::min(CHUNK, rqs[r]->nfreqs));
-    morton_pixels(rqs[r], pix);
-    std::vector<int2>& jl = per[r];
+    bp.nfmax = std::max(bp.nfmax, two ? sub[q]->nfreqs : std::min(CHUNK, sub[q]->nfreqs));
+    morton_pixels(sub[q], pix);
+    std::vector<int2>& jl = per[q];
     for (int t = 0; t < nt; t++)
       for (size_t i = 0; i < pix.size(); i += 64 * (size_t)sample_stride)
         for (size_t k = i; k < std::min(pix.size(), i + 64); k++) jl.push_back(make_int2(t0 + t, pix[k]));
-    bp.rays += (int64_t)(jl.size() / nt) * rqs[r]->nfreqs;
+    bp.rays += (int64_t)(jl.size() / nt) * sub[q]->nfreqs;
   }
   std::vector<int2> jobs;
   if (order == 1) {  // round-robin over requests, 64 jobs (one wave) at a time
     std::vector<size_t> pos(n, 0);
     for (bool any = true; any;) {
       any = false;
-      for (int r = 0; r < n; r++)
-        if (pos[r] < per[r].size()) {
-          const size_t e = std::min(per[r].size(), pos[r] + 64);
-          jobs.insert(jobs.end(), per[r].begin() + pos[r], per[r].begin() + e);
-          pos[r] = e;
+      for (int q = 0; q < n; q++)
+        if (pos[q] < per[q].size()) {
+          const size_t e = std::min(per[q].size(), pos[q] + 64);
+          jobs.insert(jobs.end(), per[q].begin() + pos[q], per[q].begin() + e);
+          pos[q] = e;
           any = true;
         }
     }
   } else {
-    for (int r = 0; r < n; r++) jobs.insert(jobs.end(), per[r].begin(), per[r].end());
+    for (int q = 0; q < n; q++) jobs.insert(jobs.end(), per[q].begin(), per[q].end());
   }
   if (jobs.size() > (size_t)INT32_MAX) throw Error(PHX_E_RANGE, "too many jobs in one batch");
   bp.njobs = (int)jobs.size();
   bp.ntasks = (int)tasks.size();
-  bp.nfmax = bp.nfmax <= 1 ? 1 : bp.nfmax <= 2 ? 2 : 4;
+  if (!two) bp.nfmax = bp.nfmax <= 1 ? 1 : bp.nfmax <= 2 ? 2 : 4;
   upload_tasks(s, s->slot, tasks, cams);
   ensure(s->slot.jobs, jobs.size() * sizeof(int2));
   HIPCHECK(hipMemcpyAsync(s->slot.jobs.p, jobs.data(), jobs.size() * sizeof(int2), hipMemcpyHostToDevice, s->stream));
-  if (sample_stride > 1) HIPCHECK(hipMemsetAsync(d_res, 0, (size_t)bp.nres * sizeof(phx_freqout), s->stream));
   HIPCHECK(hipStreamSynchronize(s->stream));
   return bp;
 }
@@ -1202,18 +1326,60 @@ static BatchPlan run_batch(phx_snapshot* s, const phx_model* const* rqs, int n, 
   check_precision(s, precision);
   if (n < 1) throw Error(PHX_E_ARG, "empty batch");
   HIPCHECK(hipSetDevice(s->device));
-  const BatchPlan bp = precision == PHX_FP64 ? plan_batch<double>(s, rqs, n, tn.order, sample_stride)
-                                             : plan_batch<float>(s, rqs, n, tn.order, sample_stride);
+  BatchPlan all;
+  all.off.resize(n);
+  std::vector<int> groups[2];  // chunked, two-phase
+  for (int r = 0; r < n; r++) {
+    check_request(rqs[r]);
+    check_uniform(rqs[0], rqs[r]);
+    all.off[r] = all.nres;
+    all.nres += (int64_t)rqs[r]->img_width * rqs[r]->img_height * rqs[r]->nfreqs;
+    groups[two_phase(rqs[r]) ? 1 : 0].push_back(r);
+  }
+  phx_freqout* d_res = (phx_freqout*)ensure(s->res, (size_t)all.nres * sizeof(phx_freqout));
+  if (sample_stride > 1) HIPCHECK(hipMemsetAsync(d_res, 0, (size_t)all.nres * sizeof(phx_freqout), s->stream));
   std::memset(info, 0, sizeof *info);
   fill_info_device(s, info);
-  LaunchInfo li;
-  li.njobs = bp.njobs;
-  li.ntasks = bp.ntasks;
-  li.nfmax = bp.nfmax;
-  li.uni = rqs[0];
-  run_precision(s, precision, li, tn, info, nullptr, nullptr);
-  info->rays = bp.rays;
-  return bp;
+  for (int g = 0; g < 2; g++) {
+    if (groups[g].empty()) continue;
+    const bool two = g == 1;
+    const BatchPlan bp = precision == PHX_FP64
+                             ? plan_batch<double>(s, rqs, groups[g], all.off, d_res, two, tn.order, sample_stride)
+                             : plan_batch<float>(s, rqs, groups[g], all.off, d_res, two, tn.order, sample_stride);
+    phx_info gi;
+    std::memset(&gi, 0, sizeof gi);
+    LaunchInfo li;
+    li.njobs = bp.njobs;
+    li.ntasks = bp.ntasks;
+    li.nfmax = bp.nfmax;
+    li.uni = rqs[0];
+    li.two_phase = two;
+    run_precision(s, precision, li, tn, &gi, nullptr, nullptr);
+    // the launch figures of the first launch; times and counts summed
+    if (all.njobs == 0) {
+      info->cus = gi.cus;
+      info->blocks = gi.blocks;
+      info->block = gi.block;
+      info->vgprs = gi.vgprs;
+      info->sgprs = gi.sgprs;
+      info->scratch_bytes = gi.scratch_bytes;
+      info->lds_bytes = gi.lds_bytes;
+      info->waves_per_cu = gi.waves_per_cu;
+      info->upload_s = gi.upload_s;
+      info->decode_s = gi.decode_s;
+    }
+    info->kernel_s += gi.kernel_s;
+    info->steps += gi.steps;
+    info->nplasma += gi.nplasma;
+    info->bad += gi.bad;
+    info->fp64_steps += gi.fp64_steps;
+    all.njobs += bp.njobs;
+    all.ntasks += bp.ntasks;
+    all.nfmax = std::max(all.nfmax, bp.nfmax);
+    all.rays += bp.rays;
+  }
+  info->rays = all.rays;
+  return all;
 }
 
 static void render_batch(phx_snapshot* s, const phx_model* const* rqs, int n, int precision, phx_freqout* const* res,
@@ -1631,7 +1797,8 @@ struct Request {
   phx_freqout* d_res = nullptr;  // the request's block of the result pool
   size_t pool_off = 0;
   int64_t bytes = 0;
-  std::vector<int2> jobs;  // (chunk, pixel), chunk-major
+  std::vector<int2> jobs;  // (chunk, pixel), chunk-major; two-phase: (0, pixel)
+  bool two = false;        // renders in two phases (two_phase)
   size_t next = 0, done = 0;
   double t_submit = 0., t_done = 0.;
   int state = 0;  // 0 waiting, 1 active, 2 done, 3 failed
@@ -1692,6 +1859,8 @@ struct Slice {
 };
 struct Flight {
   int snap = 0, slot = 0, njobs = 0;
+  bool two = false;  // a two-phase launch
+  LaunchInfo li;     // what it was launched with (li.uni: the first slice's request)
   std::vector<Slice> slices;
 };
 
@@ -1749,47 +1918,63 @@ static bool launch_tick(Server* sv, int k, const std::vector<std::shared_ptr<Req
   if (sv->opts.order == 1)  // least attained service first (ties: arrival order)
     std::stable_sort(order.begin(), order.end(),
                      [](const std::shared_ptr<Request>& x, const std::shared_ptr<Request>& y) { return x->next < y->next; });
-  const int M = sv->opts.precision == PHX_FP32 ? M32 : sv->opts.precision == PHX_FP64 ? M64 : MIX;
+  // one launch marches requests of one kind: those of the first request in the order (chunked
+  // or two-phase), the others wait for a launch of their kind
+  const bool two = order.front()->two;
+  order.erase(std::remove_if(order.begin(), order.end(),
+                             [&](const std::shared_ptr<Request>& r) { return r->two != two; }),
+              order.end());
   const size_t quantum = (size_t)std::max(1, sv->opts.quantum_jobs);
   phx_snapshot* s = sv->snaps[k];
   Slot& sl = sv->slots[k][si];
   Flight fl;
   fl.snap = k;
   fl.slot = si;
+  std::vector<const phx_model*> rqs;
+  std::vector<const phx_freqout*> outs;
+  std::vector<size_t> takes;
+  size_t njob = 0;
+  for (auto& r : order) {
+    if (njob >= quantum) break;
+    const size_t take = std::min(r->jobs.size() - r->next, quantum - njob);
+    rqs.push_back(&r->rq);
+    outs.push_back(r->d_res);
+    takes.push_back(take);
+    njob += take;
+  }
+  std::unique_lock<std::mutex> snap_lock(s->mu);  // the snapshot's own renders wait while a launch is queued
+  HIPCHECK(hipSetDevice(s->device));
   std::vector<TaskDev<T>> tasks;
+  std::vector<int> first;
+  build_tasks<T>(s, sl, rqs.data(), outs.data(), (int)rqs.size(), two, tasks, first);
   std::vector<Cam> cams;
   std::vector<int2> jobs;
   int nfmax = 1;
-  for (auto& r : order) {
-    if (jobs.size() >= quantum) break;
-    const size_t take = std::min(r->jobs.size() - r->next, quantum - jobs.size());
-    const int t0 = (int)tasks.size();
-    const int nt = add_tasks<T>(&r->rq, r->d_res, tasks);
+  for (size_t q = 0; q < rqs.size(); q++) {
+    const std::shared_ptr<Request>& r = order[q];
+    const int t0 = first[q];
+    const int nt = (q + 1 < rqs.size() ? first[q + 1] : (int)tasks.size()) - t0;
     const Cam c = cam_of(s->geo, &r->rq);
     for (int t = 0; t < nt; t++) cams.push_back(c);
-    nfmax = std::max(nfmax, std::min(CHUNK, r->rq.nfreqs));
-    for (size_t j = r->next; j < r->next + take; j++) jobs.push_back(make_int2(t0 + r->jobs[j].x, r->jobs[j].y));
-    fl.slices.push_back({r, r->next, r->next + take});
-    r->next += take;
+    nfmax = std::max(nfmax, two ? r->rq.nfreqs : std::min(CHUNK, r->rq.nfreqs));
+    for (size_t j = r->next; j < r->next + takes[q]; j++) jobs.push_back(make_int2(t0 + r->jobs[j].x, r->jobs[j].y));
+    fl.slices.push_back({r, r->next, r->next + takes[q]});
+    r->next += takes[q];
   }
   fl.njobs = (int)jobs.size();
-  std::unique_lock<std::mutex> snap_lock(s->mu);  // the snapshot's own renders wait while a launch is queued
-  HIPCHECK(hipSetDevice(s->device));
+  fl.two = two;
   upload_tasks(s, sl, tasks, cams);
   ensure(sl.jobs, jobs.size() * sizeof(int2));
   HIPCHECK(hipMemcpyAsync(sl.jobs.p, jobs.data(), jobs.size() * sizeof(int2), hipMemcpyHostToDevice, stream_of(s, sl)));
   LaunchInfo li;
   li.njobs = fl.njobs;
   li.ntasks = (int)tasks.size();
-  li.nfmax = nfmax <= 1 ? 1 : nfmax <= 2 ? 2 : 4;
+  li.nfmax = two ? nfmax : nfmax <= 1 ? 1 : nfmax <= 2 ? 2 : 4;
   li.uni = &fl.slices.front().r->rq;
-  const Tune tn = tune_from_env(s->tune);
-  if (M == M32)
-    enqueue_march<M32>(s, sl, li, tn, nullptr, nullptr, nullptr);
-  else if (M == M64)
-    enqueue_march<M64>(s, sl, li, tn, nullptr, nullptr, nullptr);
-  else
-    enqueue_march<MIX>(s, sl, li, tn, nullptr, nullptr, nullptr);
+  li.two_phase = two;
+  li.rec_blocks = sl.rec_cap;  // a pool grown by an earlier launch of the slot keeps its size
+  fl.li = li;
+  enqueue_precision(s, sl, sv->opts.precision, li, tune_from_env(s->tune), nullptr, nullptr, nullptr);
   // the host job list and tasks were staged by the copies above (pageable sources): they may go
   sv->slot_busy[k][si] = 1;
   sv->flights.push_back(std::move(fl));
@@ -1871,7 +2056,8 @@ static void serve_loop(Server* sv) {
     for (auto& r : fresh) {
       std::vector<int32_t> pix;
       morton_pixels(&r->rq, pix);
-      const int nch = (r->rq.nfreqs + CHUNK - 1) / CHUNK;
+      r->two = two_phase(&r->rq);
+      const int nch = r->two ? 1 : (r->rq.nfreqs + CHUNK - 1) / CHUNK;
       r->jobs.reserve(pix.size() * nch);
       for (int c = 0; c < nch; c++)
         for (int32_t p : pix) r->jobs.push_back(make_int2(c, p));
@@ -1889,7 +2075,17 @@ static void serve_loop(Server* sv) {
       Flight fl = std::move(sv->flights.front());
       sv->flights.pop_front();
       Slot& sl = sv->slots[fl.snap][fl.slot];
-      const double t = finish_march(sv->snaps[fl.snap], sl, fl.njobs, nullptr);
+      phx_snapshot* fs = sv->snaps[fl.snap];
+      double t = finish_march(fs, sl, fl.njobs, nullptr);
+      // a two-phase launch whose record pool ran out runs again, at once, with a larger pool
+      for (int64_t used; (used = record_overflow(sl, fl.li)) != 0;) {
+        fl.li.rec_blocks = used + used / 4 + 64;
+        std::unique_lock<std::mutex> snap_lock(fs->mu);
+        HIPCHECK(hipSetDevice(fs->device));
+        enqueue_precision(fs, sl, sv->opts.precision, fl.li, tune_from_env(fs->tune), nullptr, nullptr, nullptr);
+        snap_lock.unlock();
+        t += finish_march(fs, sl, fl.njobs, nullptr);
+      }
       sv->slot_busy[fl.snap][fl.slot] = 0;
       {
         std::lock_guard<std::mutex> lk(sv->mu);

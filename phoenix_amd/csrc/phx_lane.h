@@ -31,7 +31,17 @@ struct TaskDev {
   T freq[CHUNK], hconst[CHUNK], jscale[CHUNK];   // per frequency (bhac_io.hpp:244-249)
   phx_freqout* out;                              // result of (pixel 0, first frequency of the chunk)
   int nf, nf_total, stats, pad1;                 // frequencies in the chunk / in the request; count steps
+  const T* fq;  // two-phase renders: the request's freq[nf], hconst[nf], jscale[nf] (nf = nf_total)
 };
+
+// Two-phase renders (requests with many frequencies): the march (phase 1) marches each ray once
+// and appends the frequency-independent part of each emitting sample (a Prep record) to the ray's
+// chain of record blocks; the transfer kernel (phase 2) then folds every frequency's segment
+// operators over the ray's records, in march order, one lane per (ray, frequency).  Blocks of
+// RB records are taken from a per-launch pool with one atomic per wave; a ray whose block the pool
+// cannot hold is marked, and the launch is run again with a larger pool.
+constexpr int RB = 32;    // records per block
+constexpr int RECW = 12;  // scalars per record (a Prep record, padded)
 
 // Launch parameters only the sample work needs.  They live in device memory and are loaded where a
 // sample is processed (side_of), so that they do not hold scalar registers across the whole march:
@@ -48,6 +58,13 @@ struct SideArgs {
   int* counter;
   unsigned long long* tot;
   int j0, j1;
+  // two-phase renders: the record pool (blocks of RB records), its block links, the count of blocks
+  // taken, and per job the ray's first block and record count (count -1: the pool ran out)
+  typename ModeOf<M>::T* rec;
+  int* rec_next;
+  int* rec_used;
+  int2* rray;
+  int rec_cap, pad;
 };
 
 template <int M>
@@ -271,6 +288,16 @@ using RecOf = typename std::conditional<M == M32, float, double>::type;  // star
 // the loads are not hoisted out of the march loop.  The FP32 march reads the launch arguments,
 // which measured faster there (162 vs 172 ms; FP64 349 -> 346 ms, mixed about -7%).
 template <int M>
+PHX_DEV SideArgs<M> side_mem(const MarchArgs<M>& A) {
+  const unsigned* w = opaque_scalar(reinterpret_cast<const unsigned*>(A.side));
+  static_assert(sizeof(SideArgs<M>) % 4 == 0, "whole dwords");
+  SideArgs<M> out;
+  unsigned* d = reinterpret_cast<unsigned*>(&out);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(SideArgs<M>) / 4); i++) d[i] = load_constant(w, i);
+  return out;
+}
+template <int M>
 PHX_DEV SideArgs<M> side_of(const MarchArgs<M>& A) {
   if constexpr (M == M32) {
     SideArgs<M> out;
@@ -285,13 +312,7 @@ PHX_DEV SideArgs<M> side_of(const MarchArgs<M>& A) {
     out.j1 = A.j1;
     return out;
   }
-  const unsigned* w = opaque_scalar(reinterpret_cast<const unsigned*>(A.side));
-  static_assert(sizeof(SideArgs<M>) % 4 == 0, "whole dwords");
-  SideArgs<M> out;
-  unsigned* d = reinterpret_cast<unsigned*>(&out);
-#pragma unroll
-  for (int i = 0; i < (int)(sizeof(SideArgs<M>) / 4); i++) d[i] = load_constant(w, i);
-  return out;
+  return side_mem(A);
 }
 // the table a sample's cell is located in
 template <int M>
@@ -442,13 +463,14 @@ PHX_DEV int sample_prepare(const MarchArgs<M>& A, const TaskDev<typename ModeOf<
   kratios(fl.te, &sp.kr1, &sp.kr0);
   return 1;
 }
-// segment operator of frequency f; false when it is not finite (the reference drops it)
+// segment operator at frequency freq (jscale = freq^3, hconst = L / freq); false when it is not
+// finite (the reference drops it)
 template <class T>
-PHX_DEV bool sample_op(const TaskDev<T>& tk, const Prep<T>& sp, int f, T OA[16], T OS[4], T* dtau, T* dtauF) {
-  const T nup = tk.freq[f] * sp.ratio;
+PHX_DEV bool sample_op_at(T freq, T jscale, T hconst, const Prep<T>& sp, T OA[16], T OS[4], T* dtau, T* dtauF) {
+  const T nup = freq * sp.ratio;
   const Coef<T> seg = invariant_rotated(thermal(sp.te, sp.ne, nup, sp.B, sp.sth, sp.cth, sp.kr1, sp.kr0), nup,
-                                        tk.jscale[f], sp.c2, sp.s2);
-  const T h = sp.dl * tk.hconst[f];
+                                        jscale, sp.c2, sp.s2);
+  const T h = sp.dl * hconst;
   segment_op(seg, h, OA, OS);
   bool fin = true;
 #pragma unroll
@@ -458,6 +480,11 @@ PHX_DEV bool sample_op(const TaskDev<T>& tk, const Prep<T>& sp, int f, T OA[16],
   *dtau = seg.aI * h;
   *dtauF = fabs(seg.rV * h);
   return fin;
+}
+// segment operator of frequency f of a task's chunk
+template <class T>
+PHX_DEV bool sample_op(const TaskDev<T>& tk, const Prep<T>& sp, int f, T OA[16], T OS[4], T* dtau, T* dtauF) {
+  return sample_op_at(tk.freq[f], tk.jscale[f], tk.hconst[f], sp, OA, OS, dtau, dtauF);
 }
 // S += A O_S; A <- A O_A (operators.hpp:74-86), one row at a time in place.  KAHAN: the
 // Stokes sum carries a compensation term per component (the mixed-precision march).
@@ -525,12 +552,17 @@ struct Lane {
   using St = StateOf<M>;
   using R = RecOf<M>;
   static constexpr bool KAHAN = M == MIX;
+  static constexpr bool REC = NF == 0;      // phase 1 of a two-phase render: records, no accumulators
+  static constexpr int NA = REC ? 1 : NF;  // accumulator sets held
   St st;
   T px[4] = {T(0), T(0), T(0), T(0)}, pk[4] = {T(0), T(0), T(0), T(0)}, pf[4] = {T(0), T(0), T(0), T(0)};
   L pl[4] = {L(0), L(0), L(0), L(0)};
   T pdl = T(0);
-  T AA[NF][16], SS[NF][4], SC[NF][4], tau[NF], tauF[NF];
+  T AA[NA][16], SS[NA][4], SC[NA][4], tau[NA], tauF[NA];
   int steps = 0, turns = 0, stop = 0, np = 0, bad = 0, pix = 0, task = 0, nf = 0;
+  int jobn = 0;                           // the ray's job (REC: its entry of the per-job record table)
+  int rhead = -1, rcur = -1, rn = 0;      // REC: first and current record block, records appended
+  bool rovf = false;                      // REC: the pool had no block left for this ray
   T thdot = T(0), dlp = T(0);
   T r_prev = T(0);  // r of the previous pass's position, the pending sample's (read from the third pass on)
   bool bneg = false, bpos = false, stats = false;
@@ -557,6 +589,10 @@ struct Lane {
       tau[f] = tauF[f] = T(0);
     }
     steps = turns = stop = np = bad = 0;
+    jobn = n;
+    rhead = rcur = -1;
+    rn = 0;
+    rovf = false;
     thdot = dlp = T(0);
     bneg = bsign < T(0);
     bpos = bsign > T(0);
@@ -628,12 +664,40 @@ struct Lane {
     return v;
   }
 
+  // REC: appends the frequency-independent record of an emitting sample to the ray's block chain
+  // (a new block from the pool every RB records; none left: the ray is marked and records nothing)
+  PHX_DEV void record(const MarchArgs<M>& A, const Prep<T>& sp) {
+    const SideArgs<M> S = side_mem(A);
+    const bool need = !rovf && rn % RB == 0;
+    const int b = wave_take(need, S.rec_used);
+    if (need) {
+      if (b >= S.rec_cap) {
+        rovf = true;
+      } else {
+        if (rcur >= 0) glb_mut(S.rec_next)[rcur] = b;
+        else rhead = b;
+        rcur = b;
+      }
+    }
+    if (rovf) return;
+    const T r[RECW] = {sp.te, sp.ne, sp.B, sp.sth, sp.cth, sp.kr1, sp.kr0, sp.c2, sp.s2, sp.ratio, sp.dl, T(0)};
+    const gmut<T> o = glb_mut(S.rec + ((size_t)rcur * RB + rn % RB) * RECW);
+#pragma unroll
+    for (int q = 0; q < RECW; q++) o[q] = r[q];
+    rn++;
+  }
+
   // the pending sample, in place by its own lane (pixel.hpp:52-123, 186-188)
   PHX_DEV void sample_here(const MarchArgs<M>& A, const PassView<T>& v) {
     const TaskDev<T> tk = load_global(side_of(A).tasks + task);
     Prep<T> sp;
     const int status = sample_prepare<M>(A, tk, px, pl, pk, pf, pdl, v.leaf_next, sp);
     if (status) np++;
+    if constexpr (REC) {
+      if (status == 2) bad += nf;  // the task holds every frequency of the request
+      if (status == 1) record(A, sp);
+      return;
+    }
     if (status == 2) {
       // a NaN rotation or frequency ratio makes every frequency's operator non-finite: the
       // reference drops each of them (pixel.hpp:107-116); count them without computing them
@@ -694,6 +758,12 @@ struct Lane {
   // zeros when no plasma segment was processed.  np_counted: the sums were built by this lane
   // (otherwise a ray without segments has folded nothing in and its sums are the zeros anyway).
   PHX_DEV void write_out(const MarchArgs<M>& A, bool np_counted) const {
+    if constexpr (REC) {  // the ray's records for the transfer kernel
+      const gmut<int> o = glb_mut(reinterpret_cast<int*>(side_mem(A).rray) + 2 * (size_t)jobn);
+      o[0] = rhead;
+      o[1] = rovf ? -1 : rn;
+      return;
+    }
     const TaskDev<T> tkw = load_global(side_of(A).tasks + task);
     phx_freqout* o0 = tkw.out + (size_t)pix * tkw.nf_total;
 #pragma unroll
@@ -703,6 +773,55 @@ struct Lane {
     }
   }
 };
+
+// Phase 2 of a two-phase render for job `job` and frequency f of its task (nothing when f is not
+// one of the task's frequencies): the frequency's segment operators folded over the ray's records
+// in march order, exactly as the march folds them in place (sample_op, accumulate), and the ray's
+// FreqOut stored; an operator that is not finite is dropped (pixel.hpp:107-116).  Returns the
+// operators dropped.
+template <int M>
+PHX_DEV int transfer_one(const TaskDev<typename ModeOf<M>::T>* tasks, const int2* jobs, int job, int f,
+                         const typename ModeOf<M>::T* rec, const int* rec_next, const int2* rray) {
+  using T = typename ModeOf<M>::T;
+  constexpr bool KAHAN = M == MIX;
+  const int2 jb = jobs[job];
+  const TaskDev<T> tk = load_global(tasks + jb.x);
+  if (f >= tk.nf) return 0;
+  const int2 rr = rray[job];
+  const T freq = tk.fq[f], hconst = tk.fq[tk.nf + f], jscale = tk.fq[2 * tk.nf + f];
+  T AA[16], SS[4], SC[4], tau = T(0), tauF = T(0);
+#pragma unroll
+  for (int q = 0; q < 16; q++) AA[q] = (q % 5 == 0) ? T(1) : T(0);
+#pragma unroll
+  for (int q = 0; q < 4; q++) SS[q] = SC[q] = T(0);
+  int bad = 0, blk = rr.x;
+  for (int i = 0; i < rr.y; i++) {
+    if (i > 0 && i % RB == 0) blk = rec_next[blk];
+    const T* r = rec + ((size_t)blk * RB + i % RB) * RECW;
+    Prep<T> sp;
+    sp.te = r[0];
+    sp.ne = r[1];
+    sp.B = r[2];
+    sp.sth = r[3];
+    sp.cth = r[4];
+    sp.kr1 = r[5];
+    sp.kr0 = r[6];
+    sp.c2 = r[7];
+    sp.s2 = r[8];
+    sp.ratio = r[9];
+    sp.dl = r[10];
+    T OA[16], OS[4], dtau, dtauF;
+    if (!sample_op_at(freq, jscale, hconst, sp, OA, OS, &dtau, &dtauF)) {
+      bad++;
+      continue;
+    }
+    accumulate<T, KAHAN>(AA, SS, SC, OA, OS);
+    tau += dtau;
+    tauF += dtauF;
+  }
+  store_freqout(tk.out + (size_t)jb.y * tk.nf_total + f, true, SS, tau, tauF);
+  return bad;
+}
 
 // Starting state of job n's ray: pixel -> impact parameters with the position-angle rotation
 // (camera.hpp:11-23, jobs.hpp:64-76), the RAPTOR camera photon in FP64 (camera.hpp:71-135), the

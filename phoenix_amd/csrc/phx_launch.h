@@ -170,6 +170,36 @@ inline int add_tasks(const phx_model* rq, phx_freqout* out, std::vector<TaskDev<
   return (int)tasks.size() - first;
 }
 
+// The task of a request in a two-phase render: every frequency in one task (fq: the device copy of
+// its frequency table, filled from freq_table)
+template <class T>
+inline int add_task_all(const phx_model* rq, phx_freqout* out, const T* fq, std::vector<TaskDev<T>>& tasks) {
+  TaskDev<T> t;
+  std::memset(&t, 0, sizeof t);
+  t.Ne = T(rq->Ne);
+  t.Bu = T(rq->Bu);
+  t.rhigh = T(rq->rhigh);
+  t.nf = rq->nfreqs;
+  t.nf_total = rq->nfreqs;
+  t.out = out;
+  t.stats = 1;
+  t.fq = fq;
+  tasks.push_back(t);
+  return 1;
+}
+// freq[nf], hconst[nf], jscale[nf] of a request, rounded to T as the chunked tasks round them
+template <class T>
+inline void freq_table(const phx_model* rq, std::vector<T>& out) {
+  const int nf = rq->nfreqs;
+  const size_t o = out.size();
+  out.resize(o + 3 * (size_t)nf);
+  for (int f = 0; f < nf; f++) {
+    out[o + f] = T(rq->freq[f]);
+    out[o + nf + f] = T(rq->hconst[f]);
+    out[o + 2 * nf + f] = T(rq->jscale[f]);
+  }
+}
+
 // The device buffers one launch reads besides the snapshot's tables
 struct LaunchBufs {
   const void* tasks;
@@ -178,6 +208,13 @@ struct LaunchBufs {
   int* counter;
   unsigned long long* tot;
   const void* side;  // device copy of the SideArgs (MIX, FP64)
+  // two-phase renders: the record pool, its block links and count of blocks taken, the per-job
+  // record table, and the pool's capacity in blocks (0: not a two-phase launch)
+  void* rec = nullptr;
+  int* rec_next = nullptr;
+  int* rec_used = nullptr;
+  void* rray = nullptr;
+  int rec_cap = 0;
 };
 
 // The launch arguments of precision M: the grid G (and the FP64 locating table GL of the mixed
@@ -229,6 +266,11 @@ inline void fill_args(MarchArgs<M>& A, SideArgs<M>& side, const Grid<typename Mo
   side.tot = A.tot;
   side.j0 = A.j0;
   side.j1 = A.j1;
+  side.rec = static_cast<T*>(b.rec);
+  side.rec_next = b.rec_next;
+  side.rec_used = b.rec_used;
+  side.rray = static_cast<int2*>(b.rray);
+  side.rec_cap = b.rec_cap;
   A.side = static_cast<const SideArgs<M>*>(b.side);
 }
 

@@ -69,6 +69,8 @@ PHX_DEV gmut<T> glb_mut(T* p) {
 }
 // one lane: the ballot of a predicate is the lane's own bit
 PHX_DEV unsigned long long ballot(bool p) { return p ? 1ull : 0ull; }
+// the next index of a counter for a lane that needs one, else -1
+PHX_DEV int wave_take(bool need, int* counter) { return need ? (*counter)++ : -1; }
 // a pointer whose loads the device build keeps from hoisting (an opaque copy); the host has no such concern
 template <class P>
 PHX_DEV P opaque_scalar(P p) {
@@ -115,6 +117,19 @@ PHX_DEV gmut<T> glb_mut(T* p) {
 }
 PHX_DEV float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 PHX_DEV unsigned long long ballot(bool p) { return __ballot(p); }
+// The next index of a device counter for every active lane that needs one (consecutive indices in
+// lane order), with one atomic per wave; -1 for the others.  May be called in divergent code: the
+// ballot and the shuffle involve only the active lanes.
+PHX_DEV int wave_take(bool need, int* counter) {
+  const unsigned long long m = __ballot(need);
+  if (m == 0ull) return -1;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, (int)__popcll(m));
+  base = __shfl(base, leader);
+  return need ? base + (int)__popcll(m & ((1ull << lane) - 1ull)) : -1;
+}
 // A copy of a uniform pointer that the compiler treats as a new value (an empty asm on a scalar
 // register): loads through it stay where they are written instead of being hoisted out of the
 // march loop, which would hold their results in scalar registers across the whole loop.

@@ -45,16 +45,17 @@ def write_input(path: str, model, prim: np.ndarray, precision: int, pix: np.ndar
 
 
 def run(model, prim: np.ndarray, precision: int, pix, batched: int = 1, sanitized: bool = False, escape: int = 1,
-        field_free: int = 1, timeout: float = 900.0):
-    """Returns (res [npix, nf, 6], stats [steps, nplasma, bad, fp64_steps])."""
+        field_free: int = 1, timeout: float = 900.0, two_phase: bool = False, env: dict | None = None):
+    """Returns (res [npix, nf, 6], stats [steps, nplasma, bad, fp64_steps]).  two_phase: the recording
+    march and the transfer of every frequency over the records instead of the in-place march."""
     pix = np.ascontiguousarray(pix, dtype=np.int32)
     with tempfile.TemporaryDirectory() as td:
         inp, out = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         write_input(inp, model, prim, precision, pix, escape, field_free)
         env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0",
-                   UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
-        p = subprocess.run([binary(batched, sanitized), inp, out], capture_output=True, text=True, env=env,
-                           timeout=timeout)
+                   UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1", **(env or {}))
+        p = subprocess.run([binary(batched, sanitized), inp, out] + (["two"] if two_phase else []), capture_output=True,
+                           text=True, env=env, timeout=timeout)
         if p.returncode:
             raise RuntimeError(f"lane_host failed ({p.returncode}):\n{p.stderr[-4000:]}")
         with open(out, "rb") as f:

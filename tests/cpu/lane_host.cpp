@@ -7,7 +7,7 @@
 // kernel lane carries it.  Built by tests/cpu/Makefile, plain and under
 // -fsanitize=address,undefined, once per memory form of the FP64 march (PHX_FP64_BATCHED).
 //
-// Usage: lane_host in.bin out.bin
+// Usage: lane_host in.bin out.bin [two]   (two: the two-phase render, recording march + transfer)
 //   in.bin: int32 {magic 0x50485831, sizeof(phx_model), precision, escape, npix, field_free}
 //           phx_model bytes (pointer fields ignored)
 //           f64 lb[nleafs*3], f64 dxc[nleafs*3], int32 level_map[L][level_dims[L] product] per level,
@@ -125,8 +125,43 @@ void march(const MarchArgs<M>& A, int njobs, unsigned long long tot[4]) {
   }
 }
 
+// A two-phase render: the recording march of every ray (Lane<M, 0>), then the transfer of every
+// (ray, frequency) over the records (transfer_one), as the device's two launches run them; a pool
+// too small for the rays' records is grown and the render run again, as the device runtime does.
 template <int M>
-void render(const Input& in, std::vector<phx_freqout>& res, unsigned long long tot[4]) {
+void render_two_phase(const MarchArgs<M>& A0, SideArgs<M>& side, const Input& in, std::vector<phx_freqout>& res,
+                      std::vector<TaskDev<typename ModeOf<M>::T>>& tasks, const std::vector<int2>& jobs,
+                      unsigned long long tot[4]) {
+  using T = typename ModeOf<M>::T;
+  const int njobs = (int)jobs.size();
+  int cap = njobs * 6 + 64;
+  if (const char* e = std::getenv("LANE_REC_BLOCKS")) cap = std::atoi(e);  // a small pool: the overflow path
+  for (;;) {
+    std::vector<T> rec((size_t)cap * RB * RECW);
+    std::vector<int> next(cap, -1);
+    std::vector<int2> rray(njobs);
+    int used = 0;
+    side.rec = rec.data();
+    side.rec_next = next.data();
+    side.rec_used = &used;
+    side.rray = rray.data();
+    side.rec_cap = cap;
+    *side.counter = 0;
+    for (int q = 0; q < 4; q++) tot[q] = 0;
+    march<M, 0>(A0, njobs, tot);
+    if (used > cap) {
+      cap = used + used / 4 + 64;
+      continue;
+    }
+    for (int n = 0; n < njobs; n++)
+      for (int f = 0; f < in.m.nfreqs; f++)
+        tot[2] += transfer_one<M>(tasks.data(), jobs.data(), n, f, rec.data(), next.data(), rray.data());
+    return;
+  }
+}
+
+template <int M>
+void render(const Input& in, std::vector<phx_freqout>& res, unsigned long long tot[4], bool two) {
   using T = typename ModeOf<M>::T;
   using R = RecOf<M>;
   const phx_model& m = in.m;
@@ -147,7 +182,9 @@ void render(const Input& in, std::vector<phx_freqout>& res, unsigned long long t
   const Loc<double> GL{d_rec64, m.x_lo[1], m.x_lo[2], m.x_lo[3], m.x_hi[1], m.x_hi[2], m.x_hi[3]};
   // tasks over the pixel list, job = (task, pixel index in the output), the output indexed by pixel
   std::vector<TaskDev<T>> tasks;
-  const int ntasks = add_tasks<T>(&m, res.data(), tasks);
+  std::vector<T> fq;
+  freq_table<T>(&m, fq);
+  const int ntasks = two ? add_task_all<T>(&m, res.data(), fq.data(), tasks) : add_tasks<T>(&m, res.data(), tasks);
   std::vector<int2> jobs;
   for (int t = 0; t < ntasks; t++)
     for (int p : in.pix) jobs.push_back(make_int2(t, p));
@@ -163,7 +200,9 @@ void render(const Input& in, std::vector<phx_freqout>& res, unsigned long long t
   const LaunchBufs bufs{tasks.data(), jobs.data(), start.data(), &counter, tot, &side};
   fill_args<M>(A, side, G, GL, m, &m, tn, bufs, 0, (int)jobs.size());
   const int nfmax = std::min(CHUNK, m.nfreqs);
-  if (nfmax <= 1)
+  if (two)
+    render_two_phase<M>(A, side, in, res, tasks, jobs, tot);
+  else if (nfmax <= 1)
     march<M, 1>(A, (int)jobs.size(), tot);
   else if (nfmax <= 2)
     march<M, 2>(A, (int)jobs.size(), tot);
@@ -179,10 +218,11 @@ void render(const Input& in, std::vector<phx_freqout>& res, unsigned long long t
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc != 3) {
-    std::fprintf(stderr, "usage: %s in.bin out.bin\n", argv[0]);
+  if (argc != 3 && !(argc == 4 && std::strcmp(argv[3], "two") == 0)) {
+    std::fprintf(stderr, "usage: %s in.bin out.bin [two]\n", argv[0]);
     return 2;
   }
+  const bool two = argc == 4;
   try {
     const Input in = read_input(argv[1]);
     const phx_model& m = in.m;
@@ -190,11 +230,11 @@ int main(int argc, char** argv) {
     std::memset(res.data(), 0, res.size() * sizeof(phx_freqout));
     unsigned long long tot[4] = {0, 0, 0, 0};
     if (in.precision == PHX_FP32)
-      render<M32>(in, res, tot);
+      render<M32>(in, res, tot, two);
     else if (in.precision == PHX_FP64)
-      render<M64>(in, res, tot);
+      render<M64>(in, res, tot, two);
     else
-      render<MIX>(in, res, tot);
+      render<MIX>(in, res, tot, two);
     FILE* f = std::fopen(argv[2], "wb");
     if (!f) throw std::runtime_error("cannot write output");
     const long long st[4] = {(long long)tot[0], (long long)tot[1], (long long)tot[2], (long long)tot[3]};

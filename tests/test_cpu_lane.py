@@ -95,3 +95,19 @@ def test_escape_changes_nothing_and_is_off_inside_the_photon_orbits(precision, s
     full3, s3 = lane.run(rq, prim, precision, pix, escape=1)
     cut3, c3 = lane.run(rq, prim, precision, pix, escape=2)
     assert np.array_equal(full3, cut3) and np.array_equal(s3, c3)
+
+
+@pytest.mark.parametrize("precision", [capi.MIXED, capi.FP32, capi.FP64])
+def test_two_phase_equals_in_place_march(precision, snapdir, tmp_path, oracle_lib):
+    """The two-phase render (the recording march, then every frequency folded over the records) gives
+    the in-place march's results and counts bit for bit on the CPU, also when the record pool is too
+    small for the rays (8 blocks: the render runs again with a larger pool), under the sanitizers."""
+    case = "c2_i163_8f_40"
+    run, _, prim = _setup(case, snapdir, tmp_path, oracle_lib)
+    pix = _pixels(run, 13)
+    r0, s0 = lane.run(run.model, prim, precision, pix, sanitized=True)
+    r1, s1 = lane.run(run.model, prim, precision, pix, sanitized=True, two_phase=True)
+    r2, s2 = lane.run(run.model, prim, precision, pix, sanitized=True, two_phase=True, env={"LANE_REC_BLOCKS": "8"})
+    print(f"\n{case} fp{precision} nf={run.model.nfreqs}: counts {s0.tolist()}")
+    assert np.array_equal(r0, r1) and np.array_equal(s0, s1)
+    assert np.array_equal(r1, r2) and np.array_equal(s1, s2)
