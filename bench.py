@@ -11,7 +11,8 @@ and the poles, FP64 cell location, Kahan Stokes sums).  The same image by the pl
 (seeded) on first use, decoded on the device once and kept resident; the timed region renders
 into device memory (no host copies).
 
-Also measured (skip with --quick): the 64-image batch of uneven size in one persistent launch
+Also measured (skip with --quick): the same image at 8 frequencies (86-690 GHz), two-phase
+against in place; the 64-image batch of uneven size in one persistent launch
 (autotuned) against rendering the same images one after another with the plain FP32 path, and
 the serving loop on two resident 1.9M-cell snapshots under Poisson arrivals at 80% of its
 measured saturation (p50 / p99 latency against the same kind of requests served one at a time).
@@ -158,6 +159,49 @@ def c5_request(run, rng):
     return run.request(img_width=n, img_height=n, inclination=float(rng.uniform(5, 175)), freqs=freqs,
                        m_unit=float(10 ** rng.uniform(np.log10(5e17), np.log10(5e18))),
                        r_high=float(rng.uniform(1, 40)))
+
+
+def bench_multifreq(run, snap, capi, prec, sync, steps: int = 3) -> dict:
+    """The same 1024^2 image at the 8 frequencies log-spaced from 86 to 690 GHz (the precision-form
+    frequency set): two-phase renders (each ray marched once, then every frequency folded over its
+    recorded samples; the default from 2 frequencies) against the in-place march with 4 frequencies
+    per lane and one march per chunk of 4 (PHX_TWO_PHASE_MIN_NF=0), and against the plain FP32 march
+    in place (the reference's method: samples in place, whole-wave refill, every ray to r_out)."""
+    freqs = tuple(float(f) for f in np.geomspace(86e9, 690e9, 8))
+    rq = run.request(freqs=freqs)
+    rays = W * H * len(freqs)
+    plain = {"queue": 0, "refill": 64, "escape": 1}
+    cases = [("two_phase", prec, None, None), ("in_place", prec, None, "0"),
+             ("plain_fp32_two_phase", capi.FP32, plain, None), ("plain_fp32_in_place", capi.FP32, plain, "0")]
+    out = {"frequencies_hz": list(freqs)}
+    old = os.environ.get("PHX_TWO_PHASE_MIN_NF")
+    try:
+        for name, p, tune, env in cases:
+            if env is None:
+                os.environ.pop("PHX_TWO_PHASE_MIN_NF", None)
+                if old is not None:
+                    os.environ["PHX_TWO_PHASE_MIN_NF"] = old
+            else:
+                os.environ["PHX_TWO_PHASE_MIN_NF"] = env
+            snap.render(rq, p, to_host=False, tune=tune)
+            sync()
+            t0 = time.perf_counter()
+            ks = []
+            for _ in range(steps):
+                ks.append(snap.render(rq, p, to_host=False, tune=tune)[1]["kernel_s"])
+            sync()
+            out[f"{name}_rays_freq_per_s"] = steps * rays / (time.perf_counter() - t0)
+            out[f"{name}_kernel_ms"] = float(np.mean(ks)) * 1e3
+    finally:
+        os.environ.pop("PHX_TWO_PHASE_MIN_NF", None)
+        if old is not None:
+            os.environ["PHX_TWO_PHASE_MIN_NF"] = old
+    out["two_phase_vs_in_place"] = out["two_phase_rays_freq_per_s"] / out["in_place_rays_freq_per_s"]
+    out["two_phase_vs_plain_fp32_in_place"] = out["two_phase_rays_freq_per_s"] / out["plain_fp32_in_place_rays_freq_per_s"]
+    out["two_phase_vs_plain_fp32_two_phase"] = out["two_phase_rays_freq_per_s"] / out["plain_fp32_two_phase_rays_freq_per_s"]
+    log(f"8 frequencies: two-phase {out['two_phase_kernel_ms']:.1f} ms, in place {out['in_place_kernel_ms']:.1f} ms, "
+        f"plain FP32 in place {out['plain_fp32_in_place_kernel_ms']:.1f} ms")
+    return out
 
 
 def bench_batch(run, snap, capi, prec, sync) -> dict:
@@ -419,6 +463,10 @@ def main() -> int:
         if "fp32_escape_rays_freq_per_s" in extra:
             extra["value_vs_fp32_escape"] = value / extra["fp32_escape_rays_freq_per_s"]
         if not args.quick:
+            try:
+                extra["multifreq_8"] = bench_multifreq(run, snap, capi, prec, sync)
+            except Exception as e:  # reported, not fatal: the headline is already measured
+                extra["multifreq_8"] = {"error": str(e)}
             try:
                 extra["batch_c4"] = bench_batch(run, snap, capi, prec, sync)
             except Exception as e:  # reported, not fatal: the headline is already measured

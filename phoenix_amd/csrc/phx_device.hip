@@ -36,6 +36,12 @@ namespace phx {
 
 constexpr int BLOCK = 256;  // 4 wave64 per workgroup
 
+// an integer knob from the environment, or its default
+static int env_int(const char* name, int dflt) {
+  const char* e = std::getenv(name);
+  return e && *e ? std::atoi(e) : dflt;
+}
+
 // ============================================================================ kernels
 // ---------------------------------------------------------------------------- the march
 // Persistent lane-refill march: each lane owns one ray at a time; a lane whose ray ends
@@ -399,7 +405,8 @@ __global__ __launch_bounds__(BLOCK) void k_rays(const Cam* __restrict__ cams, co
 // memset, so that a launch sequence stays valid inside a captured HIP graph
 __global__ void k_reset(int* counter, unsigned long long* tot) {
   if (threadIdx.x == 0) counter[0] = 0;
-  if (tot && threadIdx.x == 1) counter[16] = 0;  // the record blocks taken (two-phase renders)
+  // two-phase renders: the record blocks taken and the rays the pool could not hold
+  if (tot && (threadIdx.x == 16 || threadIdx.x == 17)) counter[threadIdx.x] = 0;
   if (tot && threadIdx.x < 4) tot[threadIdx.x] = 0ull;
 }
 
@@ -408,13 +415,14 @@ __global__ void k_reset(int* counter, unsigned long long* tot) {
 // transfer_one; the dropped operators are counted per wave.
 template <int M>
 __global__ __launch_bounds__(256) void k_transfer(const TaskDev<typename ModeOf<M>::T>* __restrict__ tasks,
-                                                  const int2* __restrict__ jobs, int njobs, int F,
+                                                  const int2* __restrict__ jobs, int j0, int j1, int F,
                                                   const typename ModeOf<M>::T* __restrict__ rec,
                                                   const int* __restrict__ rec_next, const int2* __restrict__ rray,
                                                   unsigned long long* tot) {
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int job = (int)(n / F), f = (int)(n - (long long)job * F);
-  unsigned long long w = job < njobs ? (unsigned long long)transfer_one<M>(tasks, jobs, job, f, rec, rec_next, rray) : 0ull;
+  const long long q = n / F;
+  const int job = j0 + (int)q, f = (int)(n - q * F);
+  unsigned long long w = job < j1 ? (unsigned long long)transfer_one<M>(tasks, jobs, job, f, rec, rec_next, rray) : 0ull;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off);
   if ((threadIdx.x & 63) == 0 && w) atomicAdd(&tot[2], w);
@@ -575,6 +583,7 @@ struct Slot {
   DevBuf jobs, start, tasks, cams, side;
   DevBuf fq, rec, rnext, rray;  // two-phase renders: frequency tables, record pool, block links, per-job records
   int rec_cap = 0;              // record blocks the pool holds
+  int rec_chunk = 0;            // jobs per recording chunk of the last two-phase launch
   int* counter = nullptr;             // [0]: jobs taken by the launch in flight
   unsigned long long* tot = nullptr;  // steps, plasma segments, dropped segments, FP64 steps
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -870,7 +879,7 @@ struct LaunchInfo {
   bool no_alloc = false;           // the allocation-free entry point: no device allocation may happen
   bool capture = false;            // being recorded into a HIP graph: no timing events
   bool two_phase = false;          // march once recording samples, then one transfer lane per frequency
-  int64_t rec_blocks = 0;          // two-phase: record blocks to provide (0: the default per job)
+  int rec_chunk = 0;               // two-phase: jobs per recording chunk (0: as many as the pool holds)
 };
 
 template <int M, int NF, bool Q, int BS>
@@ -886,6 +895,7 @@ static void launch_march(phx_snapshot* s, Slot& sl, const LaunchInfo& li, int j0
     bufs.rec = sl.rec.p;
     bufs.rec_next = static_cast<int*>(sl.rnext.p);
     bufs.rec_used = sl.counter + 16;
+    bufs.rec_ovf = sl.counter + 17;
     bufs.rray = sl.rray.p;
     bufs.rec_cap = sl.rec_cap;
   }
@@ -960,28 +970,37 @@ static void march_jobs(phx_snapshot* s, Slot& sl, const LaunchInfo& li, const Tu
                                                                       sl.start.p);
   HIPCHECK(hipGetLastError());
   if (li.two_phase) {
-    // the record pool (grow-only): RB-record blocks, by default 6 per job (192 records a ray on
-    // average; a launch whose rays need more is run again with a larger pool, run_precision)
+    // The record pool (grow-only, at most PHX_REC_POOL_MB MiB, 16 GiB by default): RB-record blocks,
+    // PHX_REC_BLOCKS_PER_JOB (8: 256 records) a job on average.  The jobs are marched and
+    // transferred in chunks of as many jobs as it then holds; a launch in which a ray found no block
+    // left is run again with smaller chunks (run_precision).
     using T = typename ModeOf<M>::T;
-    const int64_t want = std::max<int64_t>(li.rec_blocks, (int64_t)li.njobs * 6 + 64);
-    if (want > INT32_MAX) throw Error(PHX_E_RANGE, "two-phase render: record pool above 2^31 blocks");
-    ensure(sl.rec, (size_t)want * RB * RECW * sizeof(T));
-    ensure(sl.rnext, (size_t)want * sizeof(int));
+    const size_t blk = (size_t)RB * RECW * sizeof(T);
+    const int64_t per_job = std::max(1, env_int("PHX_REC_BLOCKS_PER_JOB", 8));
+    const int64_t limit = std::max<int64_t>(1, env_int("PHX_REC_POOL_MB", 16384)) << 20;
+    const int64_t want = std::max<int64_t>(64, std::min<int64_t>((int64_t)li.njobs * per_job + 64, limit / (int64_t)blk));
+    ensure(sl.rec, (size_t)want * blk);
+    const int64_t cap = std::min<int64_t>((int64_t)(sl.rec.cap / blk), INT32_MAX);
+    ensure(sl.rnext, (size_t)cap * sizeof(int));
     ensure(sl.rray, (size_t)std::max(li.njobs, 1) * sizeof(int2));
-    sl.rec_cap = (int)want;
-    if (li.njobs > 0) {
-      launch_nf<M, 0>(s, sl, li, 0, li.njobs, tn, info);
-      const int F = std::max(1, li.nfmax);
-      const int64_t lanes = (int64_t)li.njobs * F;
+    sl.rec_cap = (int)cap;
+    const int chunk = li.rec_chunk > 0 ? li.rec_chunk : (int)std::max<int64_t>(1, (cap - 64) / per_job);
+    sl.rec_chunk = chunk;
+    const int F = std::max(1, li.nfmax);
+    for (int j0 = 0; j0 < li.njobs; j0 += chunk) {
+      const int j1 = std::min(li.njobs, j0 + chunk);
+      if (j0 > 0) k_reset<<<1, 1, 0, stream_of(s, sl)>>>(sl.counter + 16, nullptr);  // the pool, for this chunk
+      launch_nf<M, 0>(s, sl, li, j0, j1, tn, info);
+      const int64_t lanes = (int64_t)(j1 - j0) * F;
       k_transfer<M><<<(unsigned)((lanes + 255) / 256), 256, 0, stream_of(s, sl)>>>(
-          static_cast<const TaskDev<T>*>(sl.tasks.p), (const int2*)sl.jobs.p, li.njobs, F,
+          static_cast<const TaskDev<T>*>(sl.tasks.p), (const int2*)sl.jobs.p, j0, j1, F,
           static_cast<const T*>(sl.rec.p), static_cast<const int*>(sl.rnext.p), static_cast<const int2*>(sl.rray.p),
           sl.tot);
       HIPCHECK(hipGetLastError());
-    }
-    if (progress) {
-      HIPCHECK(hipStreamSynchronize(stream_of(s, sl)));
-      progress(li.njobs, user);
+      if (progress) {
+        HIPCHECK(hipStreamSynchronize(stream_of(s, sl)));
+        progress(j1 - j0, user);
+      }
     }
     return;
   }
@@ -1106,13 +1125,16 @@ static double finish_march(phx_snapshot* s, Slot& sl, int njobs, phx_info* info)
   }
   return ms * 1e-3;
 }
-// The record blocks a two-phase launch took, when they were more than its pool held (0 otherwise):
-// its rays that found no block left recorded nothing, and it must run again with a larger pool.
-static int64_t record_overflow(Slot& sl, const LaunchInfo& li) {
+// After a two-phase launch: 0 when every ray's records fitted the pool, else the smaller chunk of
+// jobs (half the one used) with which the launch must run again (its rays that found no block
+// left recorded nothing).
+static int record_overflow(Slot& sl, const LaunchInfo& li) {
   if (!li.two_phase) return 0;
-  int used = 0;
-  HIPCHECK(hipMemcpy(&used, sl.counter + 16, sizeof used, hipMemcpyDeviceToHost));
-  return used > sl.rec_cap ? used : 0;
+  int ovf = 0;
+  HIPCHECK(hipMemcpy(&ovf, sl.counter + 17, sizeof ovf, hipMemcpyDeviceToHost));
+  if (ovf == 0) return 0;
+  if (sl.rec_chunk <= 1) throw Error(PHX_E_RANGE, "two-phase render: one ray's records exceed the record pool");
+  return std::max(1, sl.rec_chunk / 2);
 }
 static void run_precision(phx_snapshot* s, int precision, const LaunchInfo& li0, const Tune& tn, phx_info* info,
                           phx_progress_fn progress, void* user) {
@@ -1120,9 +1142,9 @@ static void run_precision(phx_snapshot* s, int precision, const LaunchInfo& li0,
   for (;;) {
     enqueue_precision(s, s->slot, precision, li, tn, info, progress, user);
     finish_march(s, s->slot, li.njobs, info);
-    const int64_t used = record_overflow(s->slot, li);
-    if (!used) break;
-    li.rec_blocks = used + used / 4 + 64;
+    const int chunk = record_overflow(s->slot, li);
+    if (!chunk) break;
+    li.rec_chunk = chunk;
   }
 }
 
@@ -1972,7 +1994,6 @@ static bool launch_tick(Server* sv, int k, const std::vector<std::shared_ptr<Req
   li.nfmax = two ? nfmax : nfmax <= 1 ? 1 : nfmax <= 2 ? 2 : 4;
   li.uni = &fl.slices.front().r->rq;
   li.two_phase = two;
-  li.rec_blocks = sl.rec_cap;  // a pool grown by an earlier launch of the slot keeps its size
   fl.li = li;
   enqueue_precision(s, sl, sv->opts.precision, li, tune_from_env(s->tune), nullptr, nullptr, nullptr);
   // the host job list and tasks were staged by the copies above (pageable sources): they may go
@@ -2077,9 +2098,9 @@ static void serve_loop(Server* sv) {
       Slot& sl = sv->slots[fl.snap][fl.slot];
       phx_snapshot* fs = sv->snaps[fl.snap];
       double t = finish_march(fs, sl, fl.njobs, nullptr);
-      // a two-phase launch whose record pool ran out runs again, at once, with a larger pool
-      for (int64_t used; (used = record_overflow(sl, fl.li)) != 0;) {
-        fl.li.rec_blocks = used + used / 4 + 64;
+      // a two-phase launch whose record pool ran out runs again, at once, in smaller chunks
+      for (int chunk; (chunk = record_overflow(sl, fl.li)) != 0;) {
+        fl.li.rec_chunk = chunk;
         std::unique_lock<std::mutex> snap_lock(fs->mu);
         HIPCHECK(hipSetDevice(fs->device));
         enqueue_precision(fs, sl, sv->opts.precision, fl.li, tune_from_env(fs->tune), nullptr, nullptr, nullptr);

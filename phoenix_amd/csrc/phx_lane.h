@@ -38,8 +38,9 @@ struct TaskDev {
 // and appends the frequency-independent part of each emitting sample (a Prep record) to the ray's
 // chain of record blocks; the transfer kernel (phase 2) then folds every frequency's segment
 // operators over the ray's records, in march order, one lane per (ray, frequency).  Blocks of
-// RB records are taken from a per-launch pool with one atomic per wave; a ray whose block the pool
-// cannot hold is marked, and the launch is run again with a larger pool.
+// RB records are taken from a per-launch pool with one atomic per wave; the jobs are marched in
+// chunks whose records the pool holds, and a ray whose block the pool cannot hold is counted, and
+// the launch is run again with smaller chunks.
 constexpr int RB = 32;    // records per block
 constexpr int RECW = 12;  // scalars per record (a Prep record, padded)
 
@@ -59,10 +60,12 @@ struct SideArgs {
   unsigned long long* tot;
   int j0, j1;
   // two-phase renders: the record pool (blocks of RB records), its block links, the count of blocks
-  // taken, and per job the ray's first block and record count (count -1: the pool ran out)
+  // taken, the count of rays the pool could not hold, and per job the ray's first block and record
+  // count (count -1: the pool ran out)
   typename ModeOf<M>::T* rec;
   int* rec_next;
   int* rec_used;
+  int* rec_ovf;
   int2* rray;
   int rec_cap, pad;
 };
@@ -673,6 +676,7 @@ struct Lane {
     if (need) {
       if (b >= S.rec_cap) {
         rovf = true;
+        count_one(S.rec_ovf);
       } else {
         if (rcur >= 0) glb_mut(S.rec_next)[rcur] = b;
         else rhead = b;

@@ -213,6 +213,7 @@ struct LaunchBufs {
   void* rec = nullptr;
   int* rec_next = nullptr;
   int* rec_used = nullptr;
+  int* rec_ovf = nullptr;
   void* rray = nullptr;
   int rec_cap = 0;
 };
@@ -269,6 +270,7 @@ inline void fill_args(MarchArgs<M>& A, SideArgs<M>& side, const Grid<typename Mo
   side.rec = static_cast<T*>(b.rec);
   side.rec_next = b.rec_next;
   side.rec_used = b.rec_used;
+  side.rec_ovf = b.rec_ovf;
   side.rray = static_cast<int2*>(b.rray);
   side.rec_cap = b.rec_cap;
   A.side = static_cast<const SideArgs<M>*>(b.side);

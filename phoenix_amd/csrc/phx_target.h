@@ -71,6 +71,7 @@ PHX_DEV gmut<T> glb_mut(T* p) {
 PHX_DEV unsigned long long ballot(bool p) { return p ? 1ull : 0ull; }
 // the next index of a counter for a lane that needs one, else -1
 PHX_DEV int wave_take(bool need, int* counter) { return need ? (*counter)++ : -1; }
+PHX_DEV void count_one(int* counter) { (*counter)++; }
 // a pointer whose loads the device build keeps from hoisting (an opaque copy); the host has no such concern
 template <class P>
 PHX_DEV P opaque_scalar(P p) {
@@ -130,6 +131,7 @@ PHX_DEV int wave_take(bool need, int* counter) {
   base = __shfl(base, leader);
   return need ? base + (int)__popcll(m & ((1ull << lane) - 1ull)) : -1;
 }
+PHX_DEV void count_one(int* counter) { atomicAdd(counter, 1); }
 // A copy of a uniform pointer that the compiler treats as a new value (an empty asm on a scalar
 // register): loads through it stay where they are written instead of being hoisted out of the
 // march loop, which would hold their results in scalar registers across the whole loop.

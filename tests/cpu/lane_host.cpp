@@ -140,16 +140,17 @@ void render_two_phase(const MarchArgs<M>& A0, SideArgs<M>& side, const Input& in
     std::vector<T> rec((size_t)cap * RB * RECW);
     std::vector<int> next(cap, -1);
     std::vector<int2> rray(njobs);
-    int used = 0;
+    int used = 0, ovf = 0;
     side.rec = rec.data();
     side.rec_next = next.data();
     side.rec_used = &used;
+    side.rec_ovf = &ovf;
     side.rray = rray.data();
     side.rec_cap = cap;
     *side.counter = 0;
     for (int q = 0; q < 4; q++) tot[q] = 0;
     march<M, 0>(A0, njobs, tot);
-    if (used > cap) {
+    if (ovf > 0) {
       cap = used + used / 4 + 64;
       continue;
     }

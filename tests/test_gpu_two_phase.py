@@ -1,0 +1,70 @@
+"""Two-phase renders (requests with 2 or more frequencies): each ray marched once with its emitting
+samples recorded, then every frequency folded over the records by the transfer kernel.  They must
+give the in-place march's results and counts bit for bit (the same operations in the same order per
+ray and frequency), also when the record pool is small enough that the jobs run in several chunks
+and a chunk runs out of blocks (the launch then runs again in smaller chunks)."""
+import os
+
+import numpy as np
+import pytest
+
+from phoenix_amd import capi, configs, snapshot as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1(snapdir, tmp_path_factory):
+    dump, grid, _ = S.ensure_snapshot("c1", snapdir)
+    run = capi.Run(configs.write_config(str(tmp_path_factory.mktemp("tp") / "c1.toml"), dump, grid))
+    snap = capi.Snapshot(run, capi.TABLE_FP32 | capi.TABLE_FP64)
+    yield run, snap
+    snap.close()
+    run.close()
+
+
+def _render(snap, rq, precision, **env):
+    keys = ("PHX_TWO_PHASE_MIN_NF", "PHX_REC_BLOCKS_PER_JOB", "PHX_REC_POOL_MB")
+    old = {k: os.environ.get(k) for k in keys}
+    try:
+        for k in keys:
+            os.environ.pop(k, None)
+        for k, v in env.items():
+            os.environ[k] = str(v)
+        return snap.render(rq, precision)
+    finally:
+        for k, v in old.items():
+            os.environ.pop(k, None)
+            if v is not None:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("precision", [capi.MIXED, capi.FP32, capi.FP64])
+def test_two_phase_equals_in_place(c1, precision):
+    run, snap = c1
+    rq = run.request(img_width=48, img_height=40, inclination=60.0, freqs=tuple(np.geomspace(86e9, 690e9, 5)))
+    ref, iref = _render(snap, rq, precision, PHX_TWO_PHASE_MIN_NF=0)  # in place, frequency chunks of 4
+    counts = (iref["steps"], iref["nplasma"], iref["bad"])
+    variants = [dict(),  # the default: two phases from 2 frequencies
+                dict(PHX_REC_BLOCKS_PER_JOB=1, PHX_REC_POOL_MB=1)]  # chunks of a few hundred jobs, reruns
+    for env in variants:
+        res, info = _render(snap, rq, precision, **env)
+        print(f"\nfp{precision} {env or 'default'}: kernel {info['kernel_s'] * 1e3:.2f} ms "
+              f"(in place {iref['kernel_s'] * 1e3:.2f} ms); counts {info['steps']}, {info['nplasma']}, {info['bad']}")
+        assert np.array_equal(res, ref), f"two-phase render differs from the in-place march ({env})"
+        assert (info["steps"], info["nplasma"], info["bad"]) == counts
+
+
+def test_two_phase_batch_of_mixed_frequency_counts(c1):
+    """A batch whose requests render in both ways (1 frequency in place, 3 and 8 in two phases, as two
+    launches) equals each request rendered alone."""
+    run, snap = c1
+    reqs = [run.request(img_width=32, img_height=24, inclination=30.0, freqs=(230e9,)),
+            run.request(img_width=24, img_height=24, inclination=150.0, freqs=(86e9, 230e9, 345e9), m_unit=3e18),
+            run.request(img_width=40, img_height=16, inclination=80.0, freqs=tuple(np.geomspace(86e9, 690e9, 8)),
+                        r_high=20.0)]
+    got, info = snap.render_batch(reqs, capi.MIXED)
+    assert info["rays"] == sum(r.img_width * r.img_height * r.nfreqs for r in reqs)
+    for rq, rb in zip(reqs, got):
+        alone, _ = snap.render(rq, capi.MIXED)
+        assert np.array_equal(rb, alone)
