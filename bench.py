@@ -242,7 +242,8 @@ def bench_batch(run, snap, capi, prec, sync) -> dict:
 def queue_model(arrive, service, discipline: str) -> np.ndarray:
     """Latencies of an ideal single server with no overhead, fed the given arrival times and
     per-request service times: "fifo" serves one request at a time in arrival order, "ps" shares
-    the server equally among the requests present (processor sharing)."""
+    the server equally among the requests present (processor sharing), "srpt" serves the request
+    with the least remaining service first, preempting at arrivals."""
     n = len(arrive)
     done = np.zeros(n)
     if discipline == "fifo":
@@ -250,6 +251,27 @@ def queue_model(arrive, service, discipline: str) -> np.ndarray:
         for i in range(n):
             t = max(t, arrive[i]) + service[i]
             done[i] = t
+        return done - arrive
+    if discipline == "srpt":
+        rem, t, i, active = np.array(service, dtype=float), 0.0, 0, []
+        while i < n or active:
+            if not active:
+                t = max(t, arrive[i])
+                active.append(i)
+                i += 1
+                continue
+            j = min(active, key=lambda q: (rem[q], q))
+            finish = t + rem[j]
+            if i < n and arrive[i] < finish:
+                rem[j] -= arrive[i] - t
+                t = arrive[i]
+                active.append(i)
+                i += 1
+                continue
+            t = finish
+            rem[j] = 0.0
+            done[j] = t
+            active.remove(j)
         return done - arrive
     rem, t, i, active = np.array(service, dtype=float), 0.0, 0, []
     while i < n or active:
@@ -347,7 +369,7 @@ def bench_serve(capi, prec, tmpdir) -> dict:
                 # much of the latency the queueing itself accounts for at this load
                 arrive = np.cumsum(gaps)
                 service = np.array([r.img_width * r.img_height * r.nfreqs for _, r in arrivals]) / out["saturation_rays_per_s"]
-                for disc in ("fifo", "ps"):
+                for disc in ("fifo", "ps", "srpt"):
                     ideal = queue_model(arrive, service, disc)
                     res[f"ideal_{disc}_p99_ms"] = float(np.percentile(ideal, 99)) * 1e3
                     res[f"ideal_{disc}_p99_vs_single_p99"] = res[f"ideal_{disc}_p99_ms"] / out["single_p99_ms"]

@@ -20,13 +20,24 @@ def test_processor_sharing_closed_forms():
     np.testing.assert_allclose(queue_model(np.array([0.0, 0.1, 0.2]), np.ones(3), "ps"), [2.75, 2.85, 2.8])
 
 
+def test_srpt_closed_forms():
+    # a long job preempted by two short ones arriving during it: they run at once, it ends last
+    np.testing.assert_allclose(queue_model(np.array([0.0, 0.1, 0.2]), np.array([1.0, 0.1, 0.1]), "srpt"),
+                               [1.2, 0.1, 0.1])
+    # simultaneous arrivals: shortest first, so the sum of latencies is the least possible
+    np.testing.assert_allclose(queue_model(np.zeros(3), np.array([3.0, 1.0, 2.0]), "srpt"), [6, 1, 3])
+
+
 def test_work_conservation():
-    # both disciplines finish the last request at the same time when the server never idles
+    # every discipline finishes the last request at the same time when the server never idles
     rng = np.random.default_rng(1)
     svc = rng.exponential(1.0, 50)
     arr = np.sort(rng.uniform(0, 5, 50))
     end_f = (queue_model(arr, svc, "fifo") + arr).max()
     end_p = (queue_model(arr, svc, "ps") + arr).max()
-    assert abs(end_f - end_p) < 1e-9
+    end_s = (queue_model(arr, svc, "srpt") + arr).max()
+    assert abs(end_f - end_p) < 1e-9 and abs(end_f - end_s) < 1e-9
+    # SRPT minimises the mean latency
+    assert queue_model(arr, svc, "srpt").mean() <= min(queue_model(arr, svc, d).mean() for d in ("fifo", "ps")) + 1e-12
     # Little's law style check: FIFO's mean latency is at least the mean service time
     assert queue_model(arr, svc, "fifo").mean() >= svc.mean()
