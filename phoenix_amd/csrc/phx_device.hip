@@ -428,6 +428,34 @@ __global__ __launch_bounds__(256) void k_transfer(const TaskDev<typename ModeOf<
   if ((threadIdx.x & 63) == 0 && w) atomicAdd(&tot[2], w);
 }
 
+// Phase 2 of a parameter sweep: lane n takes frequency n % F, sweep point (n / F) % R and job
+// j0 + n / (F R) (the points of one ray adjacent), running transfer_sweep; the emitting segments and
+// the dropped operators are counted per wave.
+template <int M>
+__global__ __launch_bounds__(256) void k_transfer_sweep(const TaskDev<typename ModeOf<M>::T>* __restrict__ tasks, int R,
+                                                        const int2* __restrict__ jobs, int j0, int j1, int F,
+                                                        const Plasma<typename ModeOf<M>::T> P,
+                                                        const typename ModeOf<M>::T* __restrict__ rec,
+                                                        const int* __restrict__ rec_next,
+                                                        const int2* __restrict__ rray, unsigned long long* tot) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long q = n / F, jq = q / R;
+  const int job = j0 + (int)jq, f = (int)(n - q * F), r = (int)(q - jq * R);
+  int np = 0;
+  unsigned long long w = job < j1 ? (unsigned long long)transfer_sweep<M>(tasks, r, jobs, job, f, P, rec, rec_next, rray, &np)
+                                  : 0ull;
+  unsigned long long e = (unsigned long long)np;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    w += __shfl_xor(w, off);
+    e += __shfl_xor(e, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (w) atomicAdd(&tot[2], w);
+    if (e) atomicAdd(&tot[1], e);
+  }
+}
+
 // one value to device memory (a launch's side arguments): a kernel rather than a copy from host
 // memory, so that no host staging is involved and a launch sequence stays capturable
 template <class V>
@@ -584,6 +612,8 @@ struct Slot {
   DevBuf fq, rec, rnext, rray;  // two-phase renders: frequency tables, record pool, block links, per-job records
   int rec_cap = 0;              // record blocks the pool holds
   int rec_chunk = 0;            // jobs per recording chunk of the last two-phase launch
+  int rec_per_job = 0;          // record blocks provided per job (0: PHX_REC_BLOCKS_PER_JOB, 8); doubled
+                                // after a launch whose rays needed more, and kept for the next ones
   int* counter = nullptr;             // [0]: jobs taken by the launch in flight
   unsigned long long* tot = nullptr;  // steps, plasma segments, dropped segments, FP64 steps
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -879,7 +909,7 @@ struct LaunchInfo {
   bool no_alloc = false;           // the allocation-free entry point: no device allocation may happen
   bool capture = false;            // being recorded into a HIP graph: no timing events
   bool two_phase = false;          // march once recording samples, then one transfer lane per frequency
-  int rec_chunk = 0;               // two-phase: jobs per recording chunk (0: as many as the pool holds)
+  int sweep = 0;                   // two-phase: the number of parameter points sharing the rays (0: none)
 };
 
 template <int M, int NF, bool Q, int BS>
@@ -943,9 +973,10 @@ static void launch_nf(phx_snapshot* s, Slot& sl, const LaunchInfo& li, int j0, i
                       phx_info* info) {
   // default: samples in place with whole-wave refill (measured faster in every precision: rays of
   // one 8 x 8 pixel tile stay coherent; the LDS queue pays off only with per-lane refill)
-  // the recording march of a two-phase render (NF = 0) runs in place with 256-lane workgroups
-  if constexpr (NF == 0) {
-    launch_march<M, 0, false, 256>(s, sl, li, j0, j1, tn, info);
+  // the recording march of a two-phase render (NF = 0; a sweep's, NF = -1) runs in place with
+  // 256-lane workgroups
+  if constexpr (NF <= 0) {
+    launch_march<M, NF, false, 256>(s, sl, li, j0, j1, tn, info);
   } else {
     if (tn.queue == 1)
       launch_march<M, NF, true, 256>(s, sl, li, j0, j1, tn, info);
@@ -971,12 +1002,13 @@ static void march_jobs(phx_snapshot* s, Slot& sl, const LaunchInfo& li, const Tu
   HIPCHECK(hipGetLastError());
   if (li.two_phase) {
     // The record pool (grow-only, at most PHX_REC_POOL_MB MiB, 16 GiB by default): RB-record blocks,
-    // PHX_REC_BLOCKS_PER_JOB (8: 256 records) a job on average.  The jobs are marched and
+    // sl.rec_per_job a job on average (8 at first: 256 records).  The jobs are marched and
     // transferred in chunks of as many jobs as it then holds; a launch in which a ray found no block
-    // left is run again with smaller chunks (run_precision).
+    // left is run again with twice the blocks per job (record_overflow).
     using T = typename ModeOf<M>::T;
     const size_t blk = (size_t)RB * RECW * sizeof(T);
-    const int64_t per_job = std::max(1, env_int("PHX_REC_BLOCKS_PER_JOB", 8));
+    if (sl.rec_per_job <= 0) sl.rec_per_job = std::max(1, env_int("PHX_REC_BLOCKS_PER_JOB", 8));
+    const int64_t per_job = sl.rec_per_job;
     const int64_t limit = std::max<int64_t>(1, env_int("PHX_REC_POOL_MB", 16384)) << 20;
     const int64_t want = std::max<int64_t>(64, std::min<int64_t>((int64_t)li.njobs * per_job + 64, limit / (int64_t)blk));
     ensure(sl.rec, (size_t)want * blk);
@@ -984,18 +1016,27 @@ static void march_jobs(phx_snapshot* s, Slot& sl, const LaunchInfo& li, const Tu
     ensure(sl.rnext, (size_t)cap * sizeof(int));
     ensure(sl.rray, (size_t)std::max(li.njobs, 1) * sizeof(int2));
     sl.rec_cap = (int)cap;
-    const int chunk = li.rec_chunk > 0 ? li.rec_chunk : (int)std::max<int64_t>(1, (cap - 64) / per_job);
+    const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(li.njobs, (cap - 64) / per_job));
     sl.rec_chunk = chunk;
     const int F = std::max(1, li.nfmax);
     for (int j0 = 0; j0 < li.njobs; j0 += chunk) {
       const int j1 = std::min(li.njobs, j0 + chunk);
       if (j0 > 0) k_reset<<<1, 1, 0, stream_of(s, sl)>>>(sl.counter + 16, nullptr);  // the pool, for this chunk
-      launch_nf<M, 0>(s, sl, li, j0, j1, tn, info);
-      const int64_t lanes = (int64_t)(j1 - j0) * F;
-      k_transfer<M><<<(unsigned)((lanes + 255) / 256), 256, 0, stream_of(s, sl)>>>(
-          static_cast<const TaskDev<T>*>(sl.tasks.p), (const int2*)sl.jobs.p, j0, j1, F,
-          static_cast<const T*>(sl.rec.p), static_cast<const int*>(sl.rnext.p), static_cast<const int2*>(sl.rray.p),
-          sl.tot);
+      if (li.sweep > 0) {
+        launch_nf<M, -1>(s, sl, li, j0, j1, tn, info);
+        const int64_t lanes = (int64_t)(j1 - j0) * li.sweep * F;
+        k_transfer_sweep<M><<<(unsigned)((lanes + 255) / 256), 256, 0, stream_of(s, sl)>>>(
+            static_cast<const TaskDev<T>*>(sl.tasks.p), li.sweep, (const int2*)sl.jobs.p, j0, j1, F,
+            plasma_of<T>(s->geo, li.uni), static_cast<const T*>(sl.rec.p), static_cast<const int*>(sl.rnext.p),
+            static_cast<const int2*>(sl.rray.p), sl.tot);
+      } else {
+        launch_nf<M, 0>(s, sl, li, j0, j1, tn, info);
+        const int64_t lanes = (int64_t)(j1 - j0) * F;
+        k_transfer<M><<<(unsigned)((lanes + 255) / 256), 256, 0, stream_of(s, sl)>>>(
+            static_cast<const TaskDev<T>*>(sl.tasks.p), (const int2*)sl.jobs.p, j0, j1, F,
+            static_cast<const T*>(sl.rec.p), static_cast<const int*>(sl.rnext.p), static_cast<const int2*>(sl.rray.p),
+            sl.tot);
+      }
       HIPCHECK(hipGetLastError());
       if (progress) {
         HIPCHECK(hipStreamSynchronize(stream_of(s, sl)));
@@ -1125,27 +1166,24 @@ static double finish_march(phx_snapshot* s, Slot& sl, int njobs, phx_info* info)
   }
   return ms * 1e-3;
 }
-// After a two-phase launch: 0 when every ray's records fitted the pool, else the smaller chunk of
-// jobs (half the one used) with which the launch must run again (its rays that found no block
-// left recorded nothing).
-static int record_overflow(Slot& sl, const LaunchInfo& li) {
-  if (!li.two_phase) return 0;
+// After a two-phase launch: whether some ray found no record block left (it recorded nothing, and
+// the launch must run again).  The slot then provides twice the blocks per job: a larger pool, up
+// to its limit, and smaller chunks beyond it; the next launches of the slot keep the new figure.
+static bool record_overflow(Slot& sl, const LaunchInfo& li) {
+  if (!li.two_phase) return false;
   int ovf = 0;
   HIPCHECK(hipMemcpy(&ovf, sl.counter + 17, sizeof ovf, hipMemcpyDeviceToHost));
-  if (ovf == 0) return 0;
+  if (ovf == 0) return false;
   if (sl.rec_chunk <= 1) throw Error(PHX_E_RANGE, "two-phase render: one ray's records exceed the record pool");
-  return std::max(1, sl.rec_chunk / 2);
+  sl.rec_per_job = std::min(sl.rec_per_job * 2, 1 << 16);
+  return true;
 }
-static void run_precision(phx_snapshot* s, int precision, const LaunchInfo& li0, const Tune& tn, phx_info* info,
+static void run_precision(phx_snapshot* s, int precision, const LaunchInfo& li, const Tune& tn, phx_info* info,
                           phx_progress_fn progress, void* user) {
-  LaunchInfo li = li0;
-  for (;;) {
+  do {
     enqueue_precision(s, s->slot, precision, li, tn, info, progress, user);
     finish_march(s, s->slot, li.njobs, info);
-    const int chunk = record_overflow(s->slot, li);
-    if (!chunk) break;
-    li.rec_chunk = chunk;
-  }
+  } while (record_overflow(s->slot, li));
 }
 
 template <class T>
@@ -1343,6 +1381,53 @@ static BatchPlan plan_batch(phx_snapshot* s, const phx_model* const* rqs, const 
   return bp;
 }
 
+// whether two requests look through the same camera at the same image (their rays are the same)
+static bool same_camera(const phx_model* a, const phx_model* b) {
+  return a->img_width == b->img_width && a->img_height == b->img_height && a->inclination == b->inclination &&
+         a->fov_x == b->fov_x && a->fov_y == b->fov_y && a->r_cam == b->r_cam && a->t_init == b->t_init &&
+         a->phi_cam == b->phi_cam && a->cos_pa == b->cos_pa && a->sin_pa == b->sin_pa;
+}
+// A parameter sweep: two or more requests through one camera (differing only in m_unit, R_high and
+// frequencies).  Their rays are marched once, each sample recorded without plasma units, and every
+// point's images are folded from the records (transfer_sweep).  PHX_SWEEP=0 renders them as a batch
+// of separate images instead.
+static bool is_sweep(const phx_model* const* rqs, int n) {
+  if (n < 2 || env_int("PHX_SWEEP", 1) == 0) return false;
+  for (int r = 1; r < n; r++)
+    if (!same_camera(rqs[0], rqs[r])) return false;
+  return true;
+}
+// the sweep's launch: one task per point (every frequency), one job per pixel of the common image
+template <class T>
+static BatchPlan plan_sweep(phx_snapshot* s, const phx_model* const* rqs, int n, const std::vector<int64_t>& off,
+                            phx_freqout* d_res, int sample_stride) {
+  BatchPlan bp;
+  std::vector<const phx_freqout*> outs(n);
+  for (int r = 0; r < n; r++) outs[r] = d_res + off[r];
+  std::vector<TaskDev<T>> tasks;
+  std::vector<int> first;
+  build_tasks<T>(s, s->slot, rqs, outs.data(), n, true, tasks, first);
+  const std::vector<Cam> cams(tasks.size(), cam_of(s->geo, rqs[0]));
+  std::vector<int32_t> pix;
+  morton_pixels(rqs[0], pix);
+  std::vector<int2> jobs;
+  for (size_t i = 0; i < pix.size(); i += 64 * (size_t)sample_stride)
+    for (size_t k = i; k < std::min(pix.size(), i + 64); k++) jobs.push_back(make_int2(0, pix[k]));
+  int nfsum = 0;
+  for (int r = 0; r < n; r++) {
+    bp.nfmax = std::max(bp.nfmax, rqs[r]->nfreqs);
+    nfsum += rqs[r]->nfreqs;
+  }
+  bp.njobs = (int)jobs.size();
+  bp.ntasks = (int)tasks.size();
+  bp.rays = (int64_t)jobs.size() * nfsum;
+  upload_tasks(s, s->slot, tasks, cams);
+  ensure(s->slot.jobs, jobs.size() * sizeof(int2));
+  HIPCHECK(hipMemcpyAsync(s->slot.jobs.p, jobs.data(), jobs.size() * sizeof(int2), hipMemcpyHostToDevice, s->stream));
+  HIPCHECK(hipStreamSynchronize(s->stream));
+  return bp;
+}
+
 static BatchPlan run_batch(phx_snapshot* s, const phx_model* const* rqs, int n, int precision, const Tune& tn,
                            int sample_stride, phx_info* info) {
   check_precision(s, precision);
@@ -1362,6 +1447,24 @@ static BatchPlan run_batch(phx_snapshot* s, const phx_model* const* rqs, int n, 
   if (sample_stride > 1) HIPCHECK(hipMemsetAsync(d_res, 0, (size_t)all.nres * sizeof(phx_freqout), s->stream));
   std::memset(info, 0, sizeof *info);
   fill_info_device(s, info);
+  if (is_sweep(rqs, n)) {
+    const BatchPlan bp = precision == PHX_FP64 ? plan_sweep<double>(s, rqs, n, all.off, d_res, sample_stride)
+                                               : plan_sweep<float>(s, rqs, n, all.off, d_res, sample_stride);
+    LaunchInfo li;
+    li.njobs = bp.njobs;
+    li.ntasks = bp.ntasks;
+    li.nfmax = bp.nfmax;
+    li.uni = rqs[0];
+    li.two_phase = true;
+    li.sweep = n;
+    run_precision(s, precision, li, tn, info, nullptr, nullptr);
+    all.njobs = bp.njobs;
+    all.ntasks = bp.ntasks;
+    all.nfmax = bp.nfmax;
+    all.rays = bp.rays;
+    info->rays = all.rays;
+    return all;
+  }
   for (int g = 0; g < 2; g++) {
     if (groups[g].empty()) continue;
     const bool two = g == 1;
@@ -2098,9 +2201,8 @@ static void serve_loop(Server* sv) {
       Slot& sl = sv->slots[fl.snap][fl.slot];
       phx_snapshot* fs = sv->snaps[fl.snap];
       double t = finish_march(fs, sl, fl.njobs, nullptr);
-      // a two-phase launch whose record pool ran out runs again, at once, in smaller chunks
-      for (int chunk; (chunk = record_overflow(sl, fl.li)) != 0;) {
-        fl.li.rec_chunk = chunk;
+      // a two-phase launch whose record pool ran out runs again, at once, with more blocks per job
+      while (record_overflow(sl, fl.li)) {
         std::unique_lock<std::mutex> snap_lock(fs->mu);
         HIPCHECK(hipSetDevice(fs->device));
         enqueue_precision(fs, sl, sv->opts.precision, fl.li, tune_from_env(fs->tune), nullptr, nullptr, nullptr);

@@ -341,6 +341,7 @@ PHX_DEV void null_resolve(const Met<T>& g, T k[4], T r) {
 template <class T>
 struct Prep {
   T te, ne, B, sth, cth, kr1, kr0, c2, s2, ratio, dl;
+  T rho, uu, Bsq;  // the unit-free inputs of te, ne and B (a sample prepared without plasma units)
 };
 // What the second half of a sample needs from the first, a candidate record of 23 scalars (one
 // queue slot): [0..1] the position the sampler used (x1, x2), [2..5] the (re-solved) wavevector,
@@ -427,8 +428,10 @@ PHX_DEV int sample_frame_stage(const MarchArgs<M>& A, const typename ModeOf<M>::
   return 1;
 }
 // Both halves at once, for the sample processed in place by its own lane (the reference
-// kernel's way): 0 no emission, 1 emits, 2 every frequency's operator is dropped.
-template <int M>
+// kernel's way): 0 no emission, 1 emits, 2 every frequency's operator is dropped.  UNITS false: the
+// sample is prepared for any plasma units (a sweep over m_unit and R_high): the Theta_e cut, te, ne,
+// B and the Bessel ratios are left out, and sp keeps rho, uu and Bsq instead.
+template <int M, bool UNITS = true>
 PHX_DEV int sample_prepare(const MarchArgs<M>& A, const TaskDev<typename ModeOf<M>::T>& tk,
                            const typename ModeOf<M>::T px[4], const typename ModeOf<M>::L pl[4],
                            const typename ModeOf<M>::T pk[4], const typename ModeOf<M>::T pf[4],
@@ -438,7 +441,8 @@ PHX_DEV int sample_prepare(const MarchArgs<M>& A, const TaskDev<typename ModeOf<
   const SideArgs<M> S = side_of(A);
   if constexpr (M != MIX) null_resolve(metric(S.P.a, S.P.hs, px[1], px[2]), k, exp(px[1]));
   Fluid<T> fl;
-  if (!sample_fluid(S.G, loc_for(A, S), S.P, tk.Ne, tk.Bu, tk.rhigh, pl, leaf, fl)) return 0;
+  if (!sample_fluid<T, typename ModeOf<M>::L, UNITS>(S.G, loc_for(A, S), S.P, tk.Ne, tk.Bu, tk.rhigh, pl, leaf, fl))
+    return 0;
   if (fl.b[0] == T(0) && fl.b[1] == T(0) && fl.b[2] == T(0) && fl.b[3] == T(0)) return 2;
   // the mixed march's wavevector k^mu = g^{mu nu} p_nu is null by construction up to FP32 rounding;
   // it is checked (and re-solved) with the metric at the sampler's position, the one every later
@@ -458,12 +462,18 @@ PHX_DEV int sample_prepare(const MarchArgs<M>& A, const TaskDev<typename ModeOf<
   }
   sp.ratio = -dot(fl.g, k, fl.U);
   if (__builtin_isnan(sp.c2) || __builtin_isnan(sp.s2) || __builtin_isnan(sp.ratio)) return 2;
-  sp.te = fl.te;
-  sp.ne = fl.ne;
-  sp.B = fl.B;
   sp.dl = dl;
   scs(pa, &sp.sth, &sp.cth);
-  kratios(fl.te, &sp.kr1, &sp.kr0);
+  if constexpr (UNITS) {
+    sp.te = fl.te;
+    sp.ne = fl.ne;
+    sp.B = fl.B;
+    kratios(fl.te, &sp.kr1, &sp.kr0);
+  } else {
+    sp.rho = fl.rho;
+    sp.uu = fl.uu;
+    sp.Bsq = fl.Bsq;
+  }
   return 1;
 }
 // segment operator at frequency freq (jscale = freq^3, hconst = L / freq); false when it is not
@@ -555,7 +565,8 @@ struct Lane {
   using St = StateOf<M>;
   using R = RecOf<M>;
   static constexpr bool KAHAN = M == MIX;
-  static constexpr bool REC = NF == 0;      // phase 1 of a two-phase render: records, no accumulators
+  static constexpr bool REC = NF <= 0;     // phase 1 of a two-phase render: records, no accumulators
+  static constexpr bool SWEEP = NF < 0;    // records for any plasma units (a sweep over m_unit, R_high)
   static constexpr int NA = REC ? 1 : NF;  // accumulator sets held
   St st;
   T px[4] = {T(0), T(0), T(0), T(0)}, pk[4] = {T(0), T(0), T(0), T(0)}, pf[4] = {T(0), T(0), T(0), T(0)};
@@ -684,7 +695,16 @@ struct Lane {
       }
     }
     if (rovf) return;
-    const T r[RECW] = {sp.te, sp.ne, sp.B, sp.sth, sp.cth, sp.kr1, sp.kr0, sp.c2, sp.s2, sp.ratio, sp.dl, T(0)};
+    T r[RECW];
+    if constexpr (SWEEP) {
+      const T q[RECW] = {sp.rho, sp.uu, sp.Bsq, sp.sth, sp.cth, sp.c2, sp.s2, sp.ratio, sp.dl, T(0), T(0), T(0)};
+#pragma unroll
+      for (int i = 0; i < RECW; i++) r[i] = q[i];
+    } else {
+      const T q[RECW] = {sp.te, sp.ne, sp.B, sp.sth, sp.cth, sp.kr1, sp.kr0, sp.c2, sp.s2, sp.ratio, sp.dl, T(0)};
+#pragma unroll
+      for (int i = 0; i < RECW; i++) r[i] = q[i];
+    }
     const gmut<T> o = glb_mut(S.rec + ((size_t)rcur * RB + rn % RB) * RECW);
 #pragma unroll
     for (int q = 0; q < RECW; q++) o[q] = r[q];
@@ -695,13 +715,15 @@ struct Lane {
   PHX_DEV void sample_here(const MarchArgs<M>& A, const PassView<T>& v) {
     const TaskDev<T> tk = load_global(side_of(A).tasks + task);
     Prep<T> sp;
-    const int status = sample_prepare<M>(A, tk, px, pl, pk, pf, pdl, v.leaf_next, sp);
-    if (status) np++;
+    const int status = sample_prepare<M, !SWEEP>(A, tk, px, pl, pk, pf, pdl, v.leaf_next, sp);
     if constexpr (REC) {
-      if (status == 2) bad += nf;  // the task holds every frequency of the request
+      // a sweep's segments and drops are counted per parameter point by the transfer
+      if (status && !SWEEP) np++;
+      if (status == 2 && !SWEEP) bad += nf;  // the task holds every frequency of the request
       if (status == 1) record(A, sp);
       return;
     }
+    if (status) np++;
     if (status == 2) {
       // a NaN rotation or frequency ratio makes every frequency's operator non-finite: the
       // reference drops each of them (pixel.hpp:107-116); count them without computing them
@@ -824,6 +846,59 @@ PHX_DEV int transfer_one(const TaskDev<typename ModeOf<M>::T>* tasks, const int2
     tauF += dtauF;
   }
   store_freqout(tk.out + (size_t)jb.y * tk.nf_total + f, true, SS, tau, tauF);
+  return bad;
+}
+
+// Phase 2 of a parameter sweep (requests that share the camera and differ in m_unit, R_high and
+// frequencies; their rays are marched once, recorded without plasma units): for sweep point r (task
+// r), job `job` and frequency f, the point's n_e, |B| and Theta_e of each record (plasma_units, as
+// derive_fluid forms them), its Theta_e cut, the Bessel ratios, then the segment operators folded
+// in as transfer_one does.  Returns the operators dropped; *np (frequency 0 only) the emitting
+// segments.  The images equal the point's own render bit for bit; the counts leave out the samples
+// with b = 0 or a non-finite rotation, which the march drops for every point without recording them.
+template <int M>
+PHX_DEV int transfer_sweep(const TaskDev<typename ModeOf<M>::T>* tasks, int r, const int2* jobs, int job, int f,
+                           const Plasma<typename ModeOf<M>::T>& P, const typename ModeOf<M>::T* rec,
+                           const int* rec_next, const int2* rray, int* np) {
+  using T = typename ModeOf<M>::T;
+  constexpr bool KAHAN = M == MIX;
+  *np = 0;
+  const int2 jb = jobs[job];
+  const TaskDev<T> tk = load_global(tasks + r);
+  if (f >= tk.nf) return 0;
+  const int2 rr = rray[job];
+  const T freq = tk.fq[f], hconst = tk.fq[tk.nf + f], jscale = tk.fq[2 * tk.nf + f];
+  T AA[16], SS[4], SC[4], tau = T(0), tauF = T(0);
+#pragma unroll
+  for (int q = 0; q < 16; q++) AA[q] = (q % 5 == 0) ? T(1) : T(0);
+#pragma unroll
+  for (int q = 0; q < 4; q++) SS[q] = SC[q] = T(0);
+  int bad = 0, emit = 0, blk = rr.x;
+  for (int i = 0; i < rr.y; i++) {
+    if (i > 0 && i % RB == 0) blk = rec_next[blk];
+    const T* q = rec + ((size_t)blk * RB + i % RB) * RECW;
+    Prep<T> sp;
+    plasma_units(P, tk.Ne, tk.Bu, tk.rhigh, q[0], q[1], q[2], &sp.ne, &sp.B, &sp.te);
+    if (sp.te > P.te_max || sp.te < P.te_min) continue;  // the Theta_e cut (sampler.hpp:358)
+    emit++;
+    sp.sth = q[3];
+    sp.cth = q[4];
+    sp.c2 = q[5];
+    sp.s2 = q[6];
+    sp.ratio = q[7];
+    sp.dl = q[8];
+    kratios(sp.te, &sp.kr1, &sp.kr0);
+    T OA[16], OS[4], dtau, dtauF;
+    if (!sample_op_at(freq, jscale, hconst, sp, OA, OS, &dtau, &dtauF)) {
+      bad++;
+      continue;
+    }
+    accumulate<T, KAHAN>(AA, SS, SC, OA, OS);
+    tau += dtau;
+    tauF += dtauF;
+  }
+  store_freqout(tk.out + (size_t)jb.y * tk.nf_total + f, true, SS, tau, tauF);
+  if (f == 0) *np = emit;
   return bad;
 }
 

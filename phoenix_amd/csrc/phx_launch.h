@@ -218,6 +218,23 @@ struct LaunchBufs {
   int rec_cap = 0;
 };
 
+// the plasma parameters a launch shares, in precision T: snapshot m's spin, stretching and
+// adiabatic index, request rq's cuts and R_low
+template <class T>
+inline Plasma<T> plasma_of(const phx_model& m, const phx_model* rq) {
+  Plasma<T> P;
+  P.a = T(m.a);
+  P.hs = T(m.hslope);
+  P.gam = T(m.gam);
+  P.rlow = T(rq->rlow);
+  P.ibc = T(rq->ibc);
+  P.sigma_cut = T(rq->sigma_cut);
+  P.te_min = T(rq->thetae_min);
+  P.te_max = T(rq->thetae_max);
+  P.rt_outer = T(rq->rt_outer);
+  return P;
+}
+
 // The launch arguments of precision M: the grid G (and the FP64 locating table GL of the mixed
 // march), the launch-wide parameters of request rq on snapshot geo, the job range and the knobs.
 template <int M>
@@ -230,15 +247,7 @@ inline void fill_args(MarchArgs<M>& A, SideArgs<M>& side, const Grid<typename Mo
   A.G = G;
   side.G = G;
   side.GL = GL;
-  A.P.a = T(m.a);
-  A.P.hs = T(m.hslope);
-  A.P.gam = T(m.gam);
-  A.P.rlow = T(rq->rlow);
-  A.P.ibc = T(rq->ibc);
-  A.P.sigma_cut = T(rq->sigma_cut);
-  A.P.te_min = T(rq->thetae_min);
-  A.P.te_max = T(rq->thetae_max);
-  A.P.rt_outer = T(rq->rt_outer);
+  A.P = plasma_of<T>(m, rq);
   side.P = A.P;
   A.start = b.start;
   A.stepsize = T(rq->stepsize);

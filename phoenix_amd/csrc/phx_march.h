@@ -969,7 +969,8 @@ PHX_DEV int find_leaf_fill(const Grid<T>& G, const T x[4], LeafCache<T>& c, bool
 template <class T>
 struct Fluid {
   T U[4], b[4], ne, B, te;
-  T x[4];  // the (wrapped) position the sampler used
+  T rho, uu, Bsq;  // the interpolated rho and u, and b.b + 1e-6: the unit-free inputs of ne, B and Theta_e
+  T x[4];          // the (wrapped) position the sampler used
   Met<T> g;
 };
 template <class T>
@@ -999,13 +1000,30 @@ PHX_DEV void load8(gptr<double> p, double v[8]) {
   v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
   v[4] = c.x; v[5] = c.y; v[6] = d.x; v[7] = d.y;
 }
+// n_e, |B| and Theta_e of a sample (sampler.hpp:314-358) from rho, u and b.b + 1e-6 and the
+// request's plasma units: the part of derive_fluid that depends on m_unit and R_high
+template <class T>
+PHX_DEV void plasma_units(const Plasma<T>& P, T Ne, T Bu, T rhigh, T rho, T uu, T Bsq, T* ne, T* B, T* te) {
+  const T tiny = T(1.e-6);
+  *ne = rho * Ne + tiny;
+  *B = sqrt(Bsq) * Bu;
+  const T beta = div_rcp(uu * (P.gam - T(1)), T(0.5) * (Bsq + tiny));
+  const T bb = beta * P.ibc;
+  const T b2 = bb * bb;
+  const T trat = div_rcp(rhigh * b2, T(1) + b2) + div_rcp(P.rlow, T(1) + b2);
+  const T te_unit = div_rcp(T(1. / 3. * (MP / ME)), trat + T(1));
+  *te = div_rcp(uu, rho) * te_unit;
+}
+
 // get_fluid_params + derive_fluid (src/core/model/sampler.hpp:305-427): wrap, cached-leaf test,
 // re-wrap and lookup on a miss, cell by truncation, weights from the cell's lower corner,
 // block-local trilinear gather of the 8 primitives, then U^mu, b^mu, |B|, Theta_e and the cuts.
 // The cell is located in precision L (the block table GL) and the gather and the fluid
 // derivation run in T (the cell table G); L == T except in the mixed-precision march, which
-// locates the cell in FP64.  Ne, Bu, rhigh are the request's (per image) plasma units.
-template <class T, class L>
+// locates the cell in FP64.  Ne, Bu, rhigh are the request's (per image) plasma units.  UNITS
+// false: the Theta_e cut, the only cut that depends on them, is left to the caller (rho, uu and Bsq
+// are kept in fl for it).
+template <class T, class L, bool UNITS = true>
 PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P, T Ne, T Bu, T rhigh, const L X[4],
                          int cache, Fluid<T>& fl) {
   constexpr bool F32 = Lim<L>::fp32;
@@ -1188,17 +1206,13 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
     const T rho = qv[0], uu = qv[1];
 #pragma unroll
     for (int i = 0; i < 4; i++) fl.b[i] = T(0);
-    fl.ne = rho * Ne + tiny;
     const T Bsq = tiny;
-    fl.B = sqrt(Bsq) * Bu;
-    const T beta = div_rcp(uu * (P.gam - T(1)), T(0.5) * (Bsq + tiny));
-    const T bb = beta * P.ibc;
-    const T b2 = bb * bb;
-    const T trat = div_rcp(rhigh * b2, T(1) + b2) + div_rcp(P.rlow, T(1) + b2);
-    const T te_unit = div_rcp(T(1. / 3. * (MP / ME)), trat + T(1));
-    fl.te = div_rcp(uu, rho) * te_unit;
-    if ((div_rcp(Bsq, rho + T(1e-20)) > P.sigma_cut) || exp_r(fl.x[1]) > P.rt_outer || fl.te > P.te_max ||
-        fl.te < P.te_min) {
+    fl.rho = rho;
+    fl.uu = uu;
+    fl.Bsq = Bsq;
+    plasma_units(P, Ne, Bu, rhigh, rho, uu, Bsq, &fl.ne, &fl.B, &fl.te);
+    if ((div_rcp(Bsq, rho + T(1e-20)) > P.sigma_cut) || exp_r(fl.x[1]) > P.rt_outer ||
+        (UNITS && (fl.te > P.te_max || fl.te < P.te_min))) {
       fl.ne = T(0);
       return 0;
     }
@@ -1210,7 +1224,6 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
   const T rho = qv[0], uu = qv[1];
   const T gV[4] = {T(0), qv[2], qv[3], qv[4]};
   const T Bp[4] = {T(0), qv[5], qv[6], qv[7]};
-  fl.ne = rho * Ne + tiny;
   const T shift1 = div_rcp(u.u01, -u.u00);  // shift^2 = shift^3 = 0 in Kerr-Schild
   const T alpha = T(1) / sqrt(-u.u00);
   const T vv = g.d11 * gV[1] * gV[1] + T(2) * g.d13 * gV[1] * gV[3] + g.d22 * gV[2] * gV[2] + g.d33 * gV[3] * gV[3];
@@ -1227,14 +1240,12 @@ PHX_DEV int sample_fluid(const Grid<T>& G, const Loc<L>& GL, const Plasma<T>& P,
   T bd[4];
   lower(g, fl.b, bd);
   const T Bsq = fabs(fl.b[0] * bd[0] + fl.b[1] * bd[1] + fl.b[2] * bd[2] + fl.b[3] * bd[3]) + tiny;
-  fl.B = sqrt(Bsq) * Bu;
-  const T beta = div_rcp(uu * (P.gam - T(1)), T(0.5) * (Bsq + tiny));
-  const T bb = beta * P.ibc;
-  const T b2 = bb * bb;
-  const T trat = div_rcp(rhigh * b2, T(1) + b2) + div_rcp(P.rlow, T(1) + b2);
-  const T te_unit = div_rcp(T(1. / 3. * (MP / ME)), trat + T(1));
-  fl.te = div_rcp(uu, rho) * te_unit;
-  if ((div_rcp(Bsq, rho + T(1e-20)) > P.sigma_cut) || g.r > P.rt_outer || fl.te > P.te_max || fl.te < P.te_min) {
+  fl.rho = rho;
+  fl.uu = uu;
+  fl.Bsq = Bsq;
+  plasma_units(P, Ne, Bu, rhigh, rho, uu, Bsq, &fl.ne, &fl.B, &fl.te);
+  if ((div_rcp(Bsq, rho + T(1e-20)) > P.sigma_cut) || g.r > P.rt_outer ||
+      (UNITS && (fl.te > P.te_max || fl.te < P.te_min))) {
     fl.ne = T(0);
     return 0;
   }

@@ -68,3 +68,29 @@ def test_two_phase_batch_of_mixed_frequency_counts(c1):
     for rq, rb in zip(reqs, got):
         alone, _ = snap.render(rq, capi.MIXED)
         assert np.array_equal(rb, alone)
+
+
+@pytest.mark.parametrize("precision", [capi.MIXED, capi.FP32, capi.FP64])
+def test_sweep_equals_each_point_alone(c1, precision):
+    """A batch of parameter points through one camera (m_unit, R_high and frequencies differ) marches
+    its rays once and folds every point from the unit-free records; each point's image equals its own
+    render bit for bit, and the same batch rendered image by image (PHX_SWEEP=0)."""
+    run, snap = c1
+    cam = dict(img_width=40, img_height=32, inclination=60.0)
+    points = [run.request(**cam, freqs=(230e9,)),
+              run.request(**cam, freqs=(86e9, 345e9), m_unit=3e18, r_high=20.0),
+              run.request(**cam, freqs=(230e9,), m_unit=6e17, r_high=1.5),
+              run.request(**cam, freqs=tuple(np.geomspace(86e9, 690e9, 3)), m_unit=2e18, r_high=40.0)]
+    got, info = snap.render_batch(points, precision)
+    assert info["rays"] == sum(p.img_width * p.img_height * p.nfreqs for p in points)
+    os.environ["PHX_SWEEP"] = "0"
+    try:
+        apart, _ = snap.render_batch(points, precision)
+    finally:
+        os.environ.pop("PHX_SWEEP", None)
+    for k, (p, g, a) in enumerate(zip(points, got, apart)):
+        alone, _ = snap.render(p, precision)
+        print(f"\nfp{precision} point {k} (nf={p.nfreqs}): sweep == alone {np.array_equal(g, alone)}, "
+              f"batch apart == alone {np.array_equal(a, alone)}")
+        assert np.array_equal(a, alone)
+        assert np.array_equal(g, alone), f"sweep point {k} differs from its own render"
