@@ -12,7 +12,8 @@ and the poles, FP64 cell location, Kahan Stokes sums).  The same image by the pl
 into device memory (no host copies).
 
 Also measured (skip with --quick): the same image at 8 frequencies (86-690 GHz), two-phase
-against in place; the 64-image batch of uneven size in one persistent launch
+against in place; a 16-point sweep of m_unit and R_high through the same camera (rays marched once)
+against the images rendered apart; the 64-image batch of uneven size in one persistent launch
 (autotuned) against rendering the same images one after another with the plain FP32 path, and
 the serving loop on two resident 1.9M-cell snapshots under Poisson arrivals at 80% of its
 measured saturation (p50 / p99 latency against the same kind of requests served one at a time).
@@ -201,6 +202,45 @@ def bench_multifreq(run, snap, capi, prec, sync, steps: int = 3) -> dict:
     out["two_phase_vs_plain_fp32_two_phase"] = out["two_phase_rays_freq_per_s"] / out["plain_fp32_two_phase_rays_freq_per_s"]
     log(f"8 frequencies: two-phase {out['two_phase_kernel_ms']:.1f} ms, in place {out['in_place_kernel_ms']:.1f} ms, "
         f"plain FP32 in place {out['plain_fp32_in_place_kernel_ms']:.1f} ms")
+    return out
+
+
+def bench_sweep(run, snap, capi, prec, sync, npts: int = 16) -> dict:
+    """A parameter sweep of the headline image: 16 points of m_unit (4, log-spaced 5e17-5e18) x
+    R_high (4, 1-40) at 1024^2, 230 GHz, i = 10 deg, through one camera.  The sweep marches the rays
+    once and folds every point from the unit-free sample records (one render_batch call); against
+    the same 16 images as a batch of separate images (PHX_SWEEP=0, one launch) and against the plain
+    FP32 march rendering them one after another (the reference's method)."""
+    ms = np.geomspace(5e17, 5e18, 4)
+    rh = np.linspace(1.0, 40.0, 4)
+    pts = [run.request(m_unit=float(m), r_high=float(r)) for m in ms for r in rh][:npts]
+    rays = sum(p.img_width * p.img_height * p.nfreqs for p in pts)
+    out = {"points": len(pts), "rays_freq": rays}
+
+    def timed(fn):
+        fn()
+        sync()
+        t0 = time.perf_counter()
+        k = fn()
+        sync()
+        return time.perf_counter() - t0, k
+
+    w, k = timed(lambda: snap.render_batch(pts, prec, to_host=False)[1]["kernel_s"])
+    out.update(sweep_rays_freq_per_s=rays / w, sweep_kernel_s=k)
+    os.environ["PHX_SWEEP"] = "0"
+    try:
+        w, k = timed(lambda: snap.render_batch(pts, prec, to_host=False)[1]["kernel_s"])
+    finally:
+        os.environ.pop("PHX_SWEEP", None)
+    out.update(separate_batch_rays_freq_per_s=rays / w, separate_batch_kernel_s=k)
+    plain = {"queue": 0, "refill": 64, "escape": 1}
+    w, k = timed(lambda: sum(snap.render(p, capi.FP32, to_host=False, tune=plain)[1]["kernel_s"] for p in pts))
+    out.update(plain_fp32_sequential_rays_freq_per_s=rays / w, plain_fp32_sequential_kernel_s=k)
+    out["sweep_vs_plain_fp32_sequential"] = out["sweep_rays_freq_per_s"] / out["plain_fp32_sequential_rays_freq_per_s"]
+    out["sweep_vs_separate_batch"] = out["sweep_rays_freq_per_s"] / out["separate_batch_rays_freq_per_s"]
+    log(f"sweep of {len(pts)} points: {out['sweep_rays_freq_per_s']:.4g} rays*freq/s (kernels {out['sweep_kernel_s']:.3f} s), "
+        f"separate batch {out['separate_batch_rays_freq_per_s']:.4g}, plain FP32 one by one "
+        f"{out['plain_fp32_sequential_rays_freq_per_s']:.4g}")
     return out
 
 
@@ -489,6 +529,10 @@ def main() -> int:
                 extra["multifreq_8"] = bench_multifreq(run, snap, capi, prec, sync)
             except Exception as e:  # reported, not fatal: the headline is already measured
                 extra["multifreq_8"] = {"error": str(e)}
+            try:
+                extra["sweep_16"] = bench_sweep(run, snap, capi, prec, sync)
+            except Exception as e:
+                extra["sweep_16"] = {"error": str(e)}
             try:
                 extra["batch_c4"] = bench_batch(run, snap, capi, prec, sync)
             except Exception as e:  # reported, not fatal: the headline is already measured
