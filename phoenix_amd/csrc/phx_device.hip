@@ -1001,7 +1001,7 @@ static void march_jobs(phx_snapshot* s, Slot& sl, const LaunchInfo& li, const Tu
                                                                       sl.start.p);
   HIPCHECK(hipGetLastError());
   if (li.two_phase) {
-    // The record pool (grow-only, at most PHX_REC_POOL_MB MiB, 16 GiB by default): RB-record blocks,
+    // The record pool (grow-only, at most PHX_REC_POOL_MB MiB, 48 GiB by default): RB-record blocks,
     // sl.rec_per_job a job on average (8 at first: 256 records).  The jobs are marched and
     // transferred in chunks of as many jobs as it then holds; a launch in which a ray found no block
     // left is run again with twice the blocks per job (record_overflow).
@@ -1009,7 +1009,7 @@ static void march_jobs(phx_snapshot* s, Slot& sl, const LaunchInfo& li, const Tu
     const size_t blk = (size_t)RB * RECW * sizeof(T);
     if (sl.rec_per_job <= 0) sl.rec_per_job = std::max(1, env_int("PHX_REC_BLOCKS_PER_JOB", 8));
     const int64_t per_job = sl.rec_per_job;
-    const int64_t limit = std::max<int64_t>(1, env_int("PHX_REC_POOL_MB", 16384)) << 20;
+    const int64_t limit = std::max<int64_t>(1, env_int("PHX_REC_POOL_MB", 49152)) << 20;
     const int64_t want = std::max<int64_t>(64, std::min<int64_t>((int64_t)li.njobs * per_job + 64, limit / (int64_t)blk));
     ensure(sl.rec, (size_t)want * blk);
     const int64_t cap = std::min<int64_t>((int64_t)(sl.rec.cap / blk), INT32_MAX);
