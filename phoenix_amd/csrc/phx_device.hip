@@ -1022,22 +1022,28 @@ static void march_jobs(phx_snapshot* s, Slot& sl, const LaunchInfo& li, const Tu
     for (int j0 = 0; j0 < li.njobs; j0 += chunk) {
       const int j1 = std::min(li.njobs, j0 + chunk);
       if (j0 > 0) k_reset<<<1, 1, 0, stream_of(s, sl)>>>(sl.counter + 16, nullptr);  // the pool, for this chunk
-      if (li.sweep > 0) {
+      if (li.sweep > 0)
         launch_nf<M, -1>(s, sl, li, j0, j1, tn, info);
-        const int64_t lanes = (int64_t)(j1 - j0) * li.sweep * F;
-        k_transfer_sweep<M><<<(unsigned)((lanes + 255) / 256), 256, 0, stream_of(s, sl)>>>(
-            static_cast<const TaskDev<T>*>(sl.tasks.p), li.sweep, (const int2*)sl.jobs.p, j0, j1, F,
-            plasma_of<T>(s->geo, li.uni), static_cast<const T*>(sl.rec.p), static_cast<const int*>(sl.rnext.p),
-            static_cast<const int2*>(sl.rray.p), sl.tot);
-      } else {
+      else
         launch_nf<M, 0>(s, sl, li, j0, j1, tn, info);
-        const int64_t lanes = (int64_t)(j1 - j0) * F;
-        k_transfer<M><<<(unsigned)((lanes + 255) / 256), 256, 0, stream_of(s, sl)>>>(
-            static_cast<const TaskDev<T>*>(sl.tasks.p), (const int2*)sl.jobs.p, j0, j1, F,
-            static_cast<const T*>(sl.rec.p), static_cast<const int*>(sl.rnext.p), static_cast<const int2*>(sl.rray.p),
-            sl.tot);
+      // the transfer, in launches of at most 2^30 lanes (a dispatch counts its work-items in 32 bits)
+      const int64_t per_job = (int64_t)F * std::max(1, li.sweep);
+      const int step = (int)std::max<int64_t>(1, ((int64_t)1 << 30) / per_job);
+      for (int t0 = j0; t0 < j1; t0 += step) {
+        const int t1 = std::min(j1, t0 + step);
+        const unsigned blocks = (unsigned)(((int64_t)(t1 - t0) * per_job + 255) / 256);
+        if (li.sweep > 0)
+          k_transfer_sweep<M><<<blocks, 256, 0, stream_of(s, sl)>>>(
+              static_cast<const TaskDev<T>*>(sl.tasks.p), li.sweep, (const int2*)sl.jobs.p, t0, t1, F,
+              plasma_of<T>(s->geo, li.uni), static_cast<const T*>(sl.rec.p), static_cast<const int*>(sl.rnext.p),
+              static_cast<const int2*>(sl.rray.p), sl.tot);
+        else
+          k_transfer<M><<<blocks, 256, 0, stream_of(s, sl)>>>(
+              static_cast<const TaskDev<T>*>(sl.tasks.p), (const int2*)sl.jobs.p, t0, t1, F,
+              static_cast<const T*>(sl.rec.p), static_cast<const int*>(sl.rnext.p), static_cast<const int2*>(sl.rray.p),
+              sl.tot);
+        HIPCHECK(hipGetLastError());
       }
-      HIPCHECK(hipGetLastError());
       if (progress) {
         HIPCHECK(hipStreamSynchronize(stream_of(s, sl)));
         progress(j1 - j0, user);
