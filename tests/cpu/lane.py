@@ -45,20 +45,33 @@ def write_input(path: str, model, prim: np.ndarray, precision: int, pix: np.ndar
 
 
 def run(model, prim: np.ndarray, precision: int, pix, batched: int = 1, sanitized: bool = False, escape: int = 1,
-        field_free: int = 1, timeout: float = 900.0, two_phase: bool = False, env: dict | None = None):
+        field_free: int = 1, timeout: float = 900.0, two_phase: bool = False, env: dict | None = None, points=None):
     """Returns (res [npix, nf, 6], stats [steps, nplasma, bad, fp64_steps]).  two_phase: the recording
-    march and the transfer of every frequency over the records instead of the in-place march."""
+    march and the transfer of every frequency over the records instead of the in-place march.
+    points: a parameter sweep through model's camera (models with their own units and the same
+    frequency count); res is then [len(points), npix, nf, 6]."""
     pix = np.ascontiguousarray(pix, dtype=np.int32)
     with tempfile.TemporaryDirectory() as td:
         inp, out = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         write_input(inp, model, prim, precision, pix, escape, field_free)
+        extra = ["two"] if two_phase else []
+        if points is not None:
+            pts = os.path.join(td, "points.bin")
+            with open(pts, "wb") as f:
+                f.write(np.array([len(points)], dtype=np.int32).tobytes())
+                for p in points:
+                    f.write(bytes(p))
+            extra = ["sweep", pts]
         env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0",
                    UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1", **(env or {}))
-        p = subprocess.run([binary(batched, sanitized), inp, out] + (["two"] if two_phase else []), capture_output=True,
-                           text=True, env=env, timeout=timeout)
+        p = subprocess.run([binary(batched, sanitized), inp, out] + extra, capture_output=True, text=True, env=env,
+                           timeout=timeout)
         if p.returncode:
             raise RuntimeError(f"lane_host failed ({p.returncode}):\n{p.stderr[-4000:]}")
         with open(out, "rb") as f:
             st = np.frombuffer(f.read(32), dtype=np.int64).copy()
-            res = np.frombuffer(f.read(), dtype=np.float64).reshape(pix.size, model.nfreqs, 6).copy()
+            res = np.frombuffer(f.read(), dtype=np.float64).copy()
+    if points is not None:
+        return res.reshape(len(points), pix.size, model.nfreqs, 6), st
+    res = res.reshape(pix.size, model.nfreqs, 6)
     return res, st

@@ -7,7 +7,9 @@
 // kernel lane carries it.  Built by tests/cpu/Makefile, plain and under
 // -fsanitize=address,undefined, once per memory form of the FP64 march (PHX_FP64_BATCHED).
 //
-// Usage: lane_host in.bin out.bin [two]   (two: the two-phase render, recording march + transfer)
+// Usage: lane_host in.bin out.bin [two | sweep points.bin]
+//   two: the two-phase render (recording march + transfer); sweep: the rays marched once without
+//   plasma units and every point of points.bin (int32 K, K phx_model) folded from the records
 //   in.bin: int32 {magic 0x50485831, sizeof(phx_model), precision, escape, npix, field_free}
 //           phx_model bytes (pointer fields ignored)
 //           f64 lb[nleafs*3], f64 dxc[nleafs*3], int32 level_map[L][level_dims[L] product] per level,
@@ -161,8 +163,48 @@ void render_two_phase(const MarchArgs<M>& A0, SideArgs<M>& side, const Input& in
   }
 }
 
+// A parameter sweep: the recording march of every ray without plasma units (Lane<M, -1>), then each
+// point's transfer (transfer_sweep) into its own result array, as the device's sweep launches run them.
 template <int M>
-void render(const Input& in, std::vector<phx_freqout>& res, unsigned long long tot[4], bool two) {
+void render_sweep(const MarchArgs<M>& A0, SideArgs<M>& side, const Input& in, const std::vector<phx_model>& points,
+                  std::vector<TaskDev<typename ModeOf<M>::T>>& tasks, const std::vector<int2>& jobs,
+                  unsigned long long tot[4]) {
+  using T = typename ModeOf<M>::T;
+  const int njobs = (int)jobs.size(), K = (int)points.size();
+  int cap = njobs * 8 + 64;
+  for (;;) {
+    std::vector<T> rec((size_t)cap * RB * RECW);
+    std::vector<int> next(cap, -1);
+    std::vector<int2> rray(njobs);
+    int used = 0, ovf = 0;
+    side.rec = rec.data();
+    side.rec_next = next.data();
+    side.rec_used = &used;
+    side.rec_ovf = &ovf;
+    side.rray = rray.data();
+    side.rec_cap = cap;
+    *side.counter = 0;
+    for (int q = 0; q < 4; q++) tot[q] = 0;
+    march<M, -1>(A0, njobs, tot);
+    if (ovf > 0) {
+      cap = 2 * cap;
+      continue;
+    }
+    const Plasma<T> P = plasma_of<T>(in.m, &in.m);
+    for (int n = 0; n < njobs; n++)
+      for (int r = 0; r < K; r++)
+        for (int f = 0; f < points[r].nfreqs; f++) {
+          int np = 0;
+          tot[2] += transfer_sweep<M>(tasks.data(), r, jobs.data(), n, f, P, rec.data(), next.data(), rray.data(), &np);
+          tot[1] += np;
+        }
+    return;
+  }
+}
+
+template <int M>
+void render(const Input& in, std::vector<phx_freqout>& res, unsigned long long tot[4], bool two,
+            const std::vector<phx_model>* points = nullptr, std::vector<std::vector<phx_freqout>>* outs = nullptr) {
   using T = typename ModeOf<M>::T;
   using R = RecOf<M>;
   const phx_model& m = in.m;
@@ -185,7 +227,18 @@ void render(const Input& in, std::vector<phx_freqout>& res, unsigned long long t
   std::vector<TaskDev<T>> tasks;
   std::vector<T> fq;
   freq_table<T>(&m, fq);
-  const int ntasks = two ? add_task_all<T>(&m, res.data(), fq.data(), tasks) : add_tasks<T>(&m, res.data(), tasks);
+  std::vector<std::vector<T>> fqs;
+  int ntasks = 0;  // tasks whose jobs the march takes
+  if (points) {    // one task per sweep point; the march takes the first point's jobs
+    fqs.resize(points->size());
+    for (size_t r = 0; r < points->size(); r++) {
+      freq_table<T>(&(*points)[r], fqs[r]);
+      add_task_all<T>(&(*points)[r], (*outs)[r].data(), fqs[r].data(), tasks);
+    }
+    ntasks = 1;
+  } else {
+    ntasks = two ? add_task_all<T>(&m, res.data(), fq.data(), tasks) : add_tasks<T>(&m, res.data(), tasks);
+  }
   std::vector<int2> jobs;
   for (int t = 0; t < ntasks; t++)
     for (int p : in.pix) jobs.push_back(make_int2(t, p));
@@ -201,7 +254,9 @@ void render(const Input& in, std::vector<phx_freqout>& res, unsigned long long t
   const LaunchBufs bufs{tasks.data(), jobs.data(), start.data(), &counter, tot, &side};
   fill_args<M>(A, side, G, GL, m, &m, tn, bufs, 0, (int)jobs.size());
   const int nfmax = std::min(CHUNK, m.nfreqs);
-  if (two)
+  if (points)
+    render_sweep<M>(A, side, in, *points, tasks, jobs, tot);
+  else if (two)
     render_two_phase<M>(A, side, in, res, tasks, jobs, tot);
   else if (nfmax <= 1)
     march<M, 1>(A, (int)jobs.size(), tot);
@@ -219,28 +274,50 @@ void render(const Input& in, std::vector<phx_freqout>& res, unsigned long long t
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc != 3 && !(argc == 4 && std::strcmp(argv[3], "two") == 0)) {
-    std::fprintf(stderr, "usage: %s in.bin out.bin [two]\n", argv[0]);
+  const bool two = argc == 4 && std::strcmp(argv[3], "two") == 0;
+  const bool sweep = argc == 5 && std::strcmp(argv[3], "sweep") == 0;
+  if (argc != 3 && !two && !sweep) {
+    std::fprintf(stderr, "usage: %s in.bin out.bin [two | sweep points.bin]\n", argv[0]);
     return 2;
   }
-  const bool two = argc == 4;
   try {
     const Input in = read_input(argv[1]);
     const phx_model& m = in.m;
     std::vector<phx_freqout> res((size_t)m.img_width * m.img_height * m.nfreqs);
     std::memset(res.data(), 0, res.size() * sizeof(phx_freqout));
     unsigned long long tot[4] = {0, 0, 0, 0};
+    // sweep: points.bin = int32 K, then K phx_model (the points' units and frequencies; the same
+    // camera, image and frequency count as in.bin's model)
+    std::vector<phx_model> points;
+    std::vector<std::vector<phx_freqout>> outs;
+    if (sweep) {
+      FILE* pf = std::fopen(argv[4], "rb");
+      if (!pf) throw std::runtime_error("cannot open points");
+      int32_t K = 0;
+      read_into(pf, &K, 1);
+      points.resize(K);
+      read_into(pf, reinterpret_cast<char*>(points.data()), (size_t)K * sizeof(phx_model));
+      std::fclose(pf);
+      for (const phx_model& p : points)
+        if (p.nfreqs != m.nfreqs) throw std::runtime_error("sweep points must share the frequency count");
+      outs.assign(K, std::vector<phx_freqout>(res.size()));
+    }
     if (in.precision == PHX_FP32)
-      render<M32>(in, res, tot, two);
+      render<M32>(in, res, tot, two, sweep ? &points : nullptr, &outs);
     else if (in.precision == PHX_FP64)
-      render<M64>(in, res, tot, two);
+      render<M64>(in, res, tot, two, sweep ? &points : nullptr, &outs);
     else
-      render<MIX>(in, res, tot, two);
+      render<MIX>(in, res, tot, two, sweep ? &points : nullptr, &outs);
     FILE* f = std::fopen(argv[2], "wb");
     if (!f) throw std::runtime_error("cannot write output");
     const long long st[4] = {(long long)tot[0], (long long)tot[1], (long long)tot[2], (long long)tot[3]};
     std::fwrite(st, sizeof st, 1, f);
-    for (int32_t p : in.pix) std::fwrite(&res[(size_t)p * m.nfreqs], sizeof(phx_freqout), m.nfreqs, f);
+    if (sweep) {
+      for (const auto& o : outs)
+        for (int32_t p : in.pix) std::fwrite(&o[(size_t)p * m.nfreqs], sizeof(phx_freqout), m.nfreqs, f);
+    } else {
+      for (int32_t p : in.pix) std::fwrite(&res[(size_t)p * m.nfreqs], sizeof(phx_freqout), m.nfreqs, f);
+    }
     std::fclose(f);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "lane_host: %s\n", e.what());

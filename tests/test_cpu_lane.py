@@ -111,3 +111,21 @@ def test_two_phase_equals_in_place_march(precision, snapdir, tmp_path, oracle_li
     print(f"\n{case} fp{precision} nf={run.model.nfreqs}: counts {s0.tolist()}")
     assert np.array_equal(r0, r1) and np.array_equal(s0, s1)
     assert np.array_equal(r1, r2) and np.array_equal(s1, s2)
+
+
+@pytest.mark.parametrize("precision", [capi.MIXED, capi.FP32, capi.FP64])
+def test_sweep_equals_each_point_alone(precision, snapdir, tmp_path, oracle_lib):
+    """A parameter sweep (the rays marched once and recorded without plasma units, every point's
+    n_e, |B|, Theta_e, Theta_e cut and transfer from the records) gives each point's own in-place
+    render bit for bit on the CPU, under the sanitizers."""
+    case = "c2_i163_8f_40"
+    run, _, prim = _setup(case, snapdir, tmp_path, oracle_lib)
+    pix = _pixels(run, 17)
+    points = [run.request(m_unit=3e18, r_high=20.0), run.request(m_unit=6e17, r_high=1.5),
+              run.request(m_unit=1.2e18, r_high=40.0)]
+    res, st = lane.run(run.model, prim, precision, pix, sanitized=True, points=points)
+    for k, p in enumerate(points):
+        alone, sa = lane.run(p, prim, precision, pix)
+        print(f"\n{case} fp{precision} point {k}: identical {np.array_equal(res[k], alone)}")
+        assert np.array_equal(res[k], alone), f"sweep point {k} differs from its own render"
+    assert st[0] == sa[0]  # one march: the steps of one render
