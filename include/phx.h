@@ -283,7 +283,7 @@ int phx_snapshot_set_tune(phx_snapshot* snap, const phx_tune* tune);
 typedef struct phx_server phx_server;
 typedef struct phx_server_opts {
   int32_t precision;    /* PHX_FP32 (default), PHX_FP64 or PHX_MIXED */
-  int32_t quantum_jobs; /* jobs per launch; 0 = 262144 */
+  int32_t quantum_jobs; /* jobs per launch; 0 = 262144, at least 64 (launches take whole waves of 64 jobs) */
   int64_t hbm_budget;   /* bytes of admitted result buffers (the result pool); 0 = 8 GiB */
   int32_t max_active;   /* requests in flight at most; 0 = no limit but the budget */
   int32_t order;        /* jobs per launch taken 0: oldest request first, 1: least attained service first */

@@ -1016,7 +1016,9 @@ static void march_jobs(phx_snapshot* s, Slot& sl, const LaunchInfo& li, const Tu
     ensure(sl.rnext, (size_t)cap * sizeof(int));
     ensure(sl.rray, (size_t)std::max(li.njobs, 1) * sizeof(int2));
     sl.rec_cap = (int)cap;
-    const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(li.njobs, (cap - 64) / per_job));
+    // chunks of whole waves of 64 jobs, so that the rays that share a wave (and, in mixed
+    // precision, its FP64 steps) are the ones of an unchunked launch
+    const int chunk = (int)std::max<int64_t>(64, std::min<int64_t>(li.njobs, (cap - 64) / per_job / 64 * 64));
     sl.rec_chunk = chunk;
     const int F = std::max(1, li.nfmax);
     for (int j0 = 0; j0 < li.njobs; j0 += chunk) {
@@ -2036,6 +2038,17 @@ static void fail_request(Server* sv, Request* r, const std::string& why) {  // s
   r->error = why;
 }
 
+// The jobs a launch takes from a request with `left` jobs left, given `room` in the launch: whole
+// waves of 64 (its remaining jobs when they fit).  Every slice but a request's last then starts and
+// ends on a multiple of 64 in its own job list and in the launch's, and a launch ends after a last
+// slice that is not whole waves: a wave marches the same 64 consecutive rays of a request as when
+// the request is rendered alone, so the rays that share its FP64 steps (mixed precision) are the
+// same, and so are the results.
+static size_t wave_take_of(size_t left, size_t room) {
+  if (left <= room) return left;
+  return room / 64 * 64;
+}
+
 // builds and queues one launch of snapshot k on a free slot from the active requests' remaining
 // jobs; false when the snapshot has no job left to launch or no free slot
 template <class T>
@@ -2049,29 +2062,86 @@ static bool launch_tick(Server* sv, int k, const std::vector<std::shared_ptr<Req
   if (sv->opts.order == 1)  // least attained service first (ties: arrival order)
     std::stable_sort(order.begin(), order.end(),
                      [](const std::shared_ptr<Request>& x, const std::shared_ptr<Request>& y) { return x->next < y->next; });
-  // one launch marches requests of one kind: those of the first request in the order (chunked
-  // or two-phase), the others wait for a launch of their kind
-  const bool two = order.front()->two;
-  order.erase(std::remove_if(order.begin(), order.end(),
-                             [&](const std::shared_ptr<Request>& r) { return r->two != two; }),
-              order.end());
   const size_t quantum = (size_t)std::max(1, sv->opts.quantum_jobs);
   phx_snapshot* s = sv->snaps[k];
   Slot& sl = sv->slots[k][si];
   Flight fl;
   fl.snap = k;
   fl.slot = si;
+  // Parameter-sweep points: the active requests that look through the first request's camera and
+  // have reached the same job (points of a sweep submitted together) share one march of the next
+  // quantum of its pixels; each point's results are folded from the records (transfer_sweep).
+  // Every request's job list is one job per pixel here (one frequency in place, or two-phase),
+  // so their pixels line up.
+  {
+    const std::shared_ptr<Request>& lead = order.front();
+    // a request whose jobs are one per pixel: two-phase, or in place with one frequency chunk
+    auto one_per_pixel = [](const Request& r) { return r.two || r.rq.nfreqs <= CHUNK; };
+    std::vector<std::shared_ptr<Request>> group;
+    if (env_int("PHX_SWEEP", 1) != 0 && one_per_pixel(*lead))
+      for (auto& r : order)
+        if (r->next == lead->next && one_per_pixel(*r) && same_camera(&lead->rq, &r->rq)) group.push_back(r);
+    if (group.size() >= 2) {
+      const size_t take = wave_take_of(lead->jobs.size() - lead->next, quantum);
+      std::vector<const phx_model*> rqs;
+      std::vector<const phx_freqout*> outs;
+      int nfmax = 1;
+      for (auto& r : group) {
+        rqs.push_back(&r->rq);
+        outs.push_back(r->d_res);
+        nfmax = std::max(nfmax, r->rq.nfreqs);
+      }
+      std::unique_lock<std::mutex> snap_lock(s->mu);
+      HIPCHECK(hipSetDevice(s->device));
+      std::vector<TaskDev<T>> tasks;
+      std::vector<int> first;
+      build_tasks<T>(s, sl, rqs.data(), outs.data(), (int)rqs.size(), true, tasks, first);
+      const std::vector<Cam> cams(tasks.size(), cam_of(s->geo, &lead->rq));
+      std::vector<int2> jobs(take);
+      for (size_t j = 0; j < take; j++) jobs[j] = make_int2(0, lead->jobs[lead->next + j].y);
+      for (auto& r : group) {
+        fl.slices.push_back({r, r->next, r->next + take});
+        r->next += take;
+      }
+      fl.njobs = (int)take;
+      fl.two = true;
+      upload_tasks(s, sl, tasks, cams);
+      ensure(sl.jobs, jobs.size() * sizeof(int2));
+      HIPCHECK(hipMemcpyAsync(sl.jobs.p, jobs.data(), jobs.size() * sizeof(int2), hipMemcpyHostToDevice,
+                              stream_of(s, sl)));
+      LaunchInfo li;
+      li.njobs = fl.njobs;
+      li.ntasks = (int)tasks.size();
+      li.nfmax = nfmax;
+      li.uni = &lead->rq;
+      li.two_phase = true;
+      li.sweep = (int)group.size();
+      fl.li = li;
+      enqueue_precision(s, sl, sv->opts.precision, li, tune_from_env(s->tune), nullptr, nullptr, nullptr);
+      sv->slot_busy[k][si] = 1;
+      sv->flights.push_back(std::move(fl));
+      return true;
+    }
+  }
+  // one launch marches requests of one kind: those of the first request in the order (chunked
+  // or two-phase), the others wait for a launch of their kind
+  const bool two = order.front()->two;
+  order.erase(std::remove_if(order.begin(), order.end(),
+                             [&](const std::shared_ptr<Request>& r) { return r->two != two; }),
+              order.end());
   std::vector<const phx_model*> rqs;
   std::vector<const phx_freqout*> outs;
   std::vector<size_t> takes;
   size_t njob = 0;
   for (auto& r : order) {
     if (njob >= quantum) break;
-    const size_t take = std::min(r->jobs.size() - r->next, quantum - njob);
+    const size_t left = r->jobs.size() - r->next, take = wave_take_of(left, quantum - njob);
+    if (take == 0) break;
     rqs.push_back(&r->rq);
     outs.push_back(r->d_res);
     takes.push_back(take);
     njob += take;
+    if (take % 64) break;  // a request's last slice ends the launch's whole waves
   }
   std::unique_lock<std::mutex> snap_lock(s->mu);  // the snapshot's own renders wait while a launch is queued
   HIPCHECK(hipSetDevice(s->device));
@@ -2426,6 +2496,7 @@ int phx_server_start(phx_snapshot* const* snaps, int32_t nsnaps, const phx_serve
     for (phx_snapshot* s : sv.snaps) check_precision(s, sv.opts.precision);
     if (sv.opts.hbm_budget <= 0) sv.opts.hbm_budget = (int64_t)8 << 30;
     if (sv.opts.quantum_jobs <= 0) sv.opts.quantum_jobs = 1 << 18;
+    sv.opts.quantum_jobs = std::max(sv.opts.quantum_jobs, 64);  // at least one whole wave of jobs
     if (sv.opts.order < 0 || sv.opts.order > 1) throw Error(PHX_E_ARG, "order must be 0 (oldest first) or 1 (least attained service)");
     HIPCHECK(hipSetDevice(sv.snaps[0]->device));
     sv.slots.resize(nsnaps);

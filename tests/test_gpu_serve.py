@@ -1,10 +1,12 @@
 """The serving loop (phx_server_*): requests that arrive while others are on the GPU share its
 launches (continuous batching), and each result equals the same request rendered alone.
 
-A request's rays go through the same per-ray code in or out of the mix, so a result is
-bit-identical to the alone render whenever the launches that marched it used the same kernel
-instantiation (all requests of the mix have one frequency chunk of the same width); otherwise
-it is held to the parity gates.
+A request's rays go through the same per-ray code in or out of the mix, and launches take whole
+waves of 64 of a request's jobs, so that a wave marches the same rays as in the alone render (in
+mixed precision they share their FP64 steps).  A result is then bit-identical to the alone render
+whenever the launches that marched it used the same kernel instantiation (all requests of the mix
+have one frequency chunk of the same width, or render in two phases); otherwise it is held to the
+parity gates.
 """
 import threading
 import time
@@ -151,3 +153,30 @@ def test_device_flux_matches_compute_spec(env):
     srv.stop()
     assert none is None and np.array_equal(img, res)
     assert np.all(np.abs(spec - host) <= 1e-12 * np.abs(host[:, :1]))
+
+
+@pytest.mark.parametrize("precision", [capi.MIXED, capi.FP32])
+def test_served_sweep_points_share_the_march(env, precision):
+    """Sweep points submitted together (one camera; m_unit, R_high and frequencies differ) are
+    marched once per launch and folded point by point; each result equals the point rendered alone
+    bit for bit, beside a request through another camera, and with several launches per request."""
+    run, run2, snaps = env
+    cam = dict(img_width=32, img_height=24, inclination=45.0)
+    points = [run.request(**cam, freqs=(230e9,)),
+              run.request(**cam, freqs=(86e9, 345e9), m_unit=3e18, r_high=20.0),
+              run.request(**cam, freqs=(230e9,), m_unit=6e17, r_high=1.5),
+              run.request(**cam, freqs=tuple(np.geomspace(86e9, 690e9, 3)), m_unit=2e18, r_high=40.0)]
+    other = _rq(run, 24, 24, (230e9,), 120.0, 5e17, 8.0)
+    reqs = [(0, p) for p in points] + [(0, other), (1, _rq(run2, 16, 16, (86e9, 230e9), 70.0))]
+    srv = capi.Server(snaps, precision, quantum_jobs=250)
+    tickets = [srv.submit(k, r) for k, r in reqs]
+    outs = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    srv.stop()
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    # the four points need 4 launches of 250 of their 768 pixels together, not 4 each
+    print(f"\nfp{precision}: {st['launches']} launches")
+    assert st["launches"] < 4 * 4
+    for (k, r), (res, lat) in zip(reqs, outs):
+        alone, _ = snaps[k].render(r, precision)
+        assert np.array_equal(res, alone), f"request {r.img_width}x{r.img_height} nf={r.nfreqs}: served != alone"
