@@ -54,6 +54,9 @@ typedef struct phx_info {
                                           leaves out the samples in field-free leaves (all cut or dropped) */
   int64_t rays;                        /* rays x frequencies rendered */
   int64_t fp64_steps;                  /* PHX_MIXED: geodesic steps taken in FP64 (near the horizon / poles) */
+  int32_t launches;                    /* march launches: frequency chunks, PHX_CHUNK chunks, two-phase record
+                                          chunks, reruns included */
+  int32_t reruns;                      /* two-phase launches run again after a ray found the record pool full */
 } phx_info;
 
 /* device::Progress (src/device/kernel.cuh:32): called on the caller's thread after each chunk */

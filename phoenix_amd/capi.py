@@ -32,7 +32,7 @@ class Info(ctypes.Structure):
                 ("vgprs", c_int), ("sgprs", c_int), ("scratch_bytes", c_int), ("lds_bytes", c_int),
                 ("waves_per_cu", c_int), ("upload_s", c_double), ("kernel_s", c_double),
                 ("decode_s", c_double), ("steps", c_int64), ("nplasma", c_int64), ("bad", c_int64),
-                ("rays", c_int64), ("fp64_steps", c_int64)]
+                ("rays", c_int64), ("fp64_steps", c_int64), ("launches", c_int), ("reruns", c_int)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}

@@ -614,6 +614,7 @@ struct Slot {
   int rec_chunk = 0;            // jobs per recording chunk of the last two-phase launch
   int rec_per_job = 0;          // record blocks provided per job (0: PHX_REC_BLOCKS_PER_JOB, 8); doubled
                                 // after a launch whose rays needed more, and kept for the next ones
+  int rec_per_job_env = 0;      // the PHX_REC_BLOCKS_PER_JOB value it was set from (a change resets it)
   int* counter = nullptr;             // [0]: jobs taken by the launch in flight
   unsigned long long* tot = nullptr;  // steps, plasma segments, dropped segments, FP64 steps
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -958,6 +959,7 @@ static void launch_march(phx_snapshot* s, Slot& sl, const LaunchInfo& li, int j0
   HIPCHECK(hipGetLastError());
   sl.last_jobs = njobs;
   if (info) {
+    info->launches++;
     info->vgprs = fa.numRegs;
     info->scratch_bytes = (int32_t)fa.localSizeBytes;
     info->lds_bytes = (int32_t)fa.sharedSizeBytes;
@@ -1007,12 +1009,17 @@ static void march_jobs(phx_snapshot* s, Slot& sl, const LaunchInfo& li, const Tu
     // left is run again with twice the blocks per job (record_overflow).
     using T = typename ModeOf<M>::T;
     const size_t blk = (size_t)RB * RECW * sizeof(T);
-    if (sl.rec_per_job <= 0) sl.rec_per_job = std::max(1, env_int("PHX_REC_BLOCKS_PER_JOB", 8));
+    const int per_job_env = std::max(1, env_int("PHX_REC_BLOCKS_PER_JOB", 8));
+    if (sl.rec_per_job <= 0 || per_job_env != sl.rec_per_job_env) sl.rec_per_job = per_job_env;
+    sl.rec_per_job_env = per_job_env;
     const int64_t per_job = sl.rec_per_job;
-    const int64_t limit = std::max<int64_t>(1, env_int("PHX_REC_POOL_MB", 49152)) << 20;
-    const int64_t want = std::max<int64_t>(64, std::min<int64_t>((int64_t)li.njobs * per_job + 64, limit / (int64_t)blk));
+    // at most PHX_REC_POOL_MB MiB, and never less than one block per lane of a wave (0: that minimum)
+    const int64_t limit = std::max<int64_t>(64, (std::max<int64_t>(0, env_int("PHX_REC_POOL_MB", 49152)) << 20) /
+                                                    (int64_t)blk);
+    const int64_t want = std::max<int64_t>(64, std::min<int64_t>((int64_t)li.njobs * per_job + 64, limit));
     ensure(sl.rec, (size_t)want * blk);
-    const int64_t cap = std::min<int64_t>((int64_t)(sl.rec.cap / blk), INT32_MAX);
+    // the pool is the part of the (grow-only) buffer within the limit
+    const int64_t cap = std::min<int64_t>({(int64_t)(sl.rec.cap / blk), limit, INT32_MAX});
     ensure(sl.rnext, (size_t)cap * sizeof(int));
     ensure(sl.rray, (size_t)std::max(li.njobs, 1) * sizeof(int2));
     sl.rec_cap = (int)cap;
@@ -1182,16 +1189,23 @@ static bool record_overflow(Slot& sl, const LaunchInfo& li) {
   int ovf = 0;
   HIPCHECK(hipMemcpy(&ovf, sl.counter + 17, sizeof ovf, hipMemcpyDeviceToHost));
   if (ovf == 0) return false;
-  if (sl.rec_chunk <= 1) throw Error(PHX_E_RANGE, "two-phase render: one ray's records exceed the record pool");
+  // chunks are whole waves: a chunk of one wave that ran out cannot be split further
+  if (sl.rec_chunk <= 64)
+    throw Error(PHX_E_RANGE, "two-phase render: the records of one wave of 64 rays exceed the record pool "
+                             "(PHX_REC_POOL_MB)");
   sl.rec_per_job = std::min(sl.rec_per_job * 2, 1 << 16);
   return true;
 }
 static void run_precision(phx_snapshot* s, int precision, const LaunchInfo& li, const Tune& tn, phx_info* info,
                           phx_progress_fn progress, void* user) {
-  do {
+  double t = 0.0;
+  for (;;) {
     enqueue_precision(s, s->slot, precision, li, tn, info, progress, user);
-    finish_march(s, s->slot, li.njobs, info);
-  } while (record_overflow(s->slot, li));
+    t += finish_march(s, s->slot, li.njobs, info);
+    if (!record_overflow(s->slot, li)) break;
+    if (info) info->reruns++;
+  }
+  if (info) info->kernel_s = t;  // the launches that ran out of records included
 }
 
 template <class T>
@@ -1506,6 +1520,8 @@ static BatchPlan run_batch(phx_snapshot* s, const phx_model* const* rqs, int n, 
     info->nplasma += gi.nplasma;
     info->bad += gi.bad;
     info->fp64_steps += gi.fp64_steps;
+    info->launches += gi.launches;
+    info->reruns += gi.reruns;
     all.njobs += bp.njobs;
     all.ntasks += bp.ntasks;
     all.nfmax = std::max(all.nfmax, bp.nfmax);

@@ -45,14 +45,36 @@ def test_two_phase_equals_in_place(c1, precision):
     rq = run.request(img_width=48, img_height=40, inclination=60.0, freqs=tuple(np.geomspace(86e9, 690e9, 5)))
     ref, iref = _render(snap, rq, precision, PHX_TWO_PHASE_MIN_NF=0)  # in place, frequency chunks of 4
     counts = (iref["steps"], iref["nplasma"], iref["bad"])
-    variants = [dict(),  # the default: two phases from 2 frequencies
-                dict(PHX_REC_BLOCKS_PER_JOB=1, PHX_REC_POOL_MB=1)]  # chunks of a few hundred jobs, reruns
+    variants = [dict(),  # the default: two phases from 2 frequencies, one launch
+                # a 4 MiB pool (1365-2730 blocks for 1920 rays of 2-10 blocks each) and 1 block per
+                # job at first: launches that run out and run again with more blocks per job, then
+                # chunks of a few hundred jobs
+                dict(PHX_REC_BLOCKS_PER_JOB=1, PHX_REC_POOL_MB=4)]
     for env in variants:
         res, info = _render(snap, rq, precision, **env)
         print(f"\nfp{precision} {env or 'default'}: kernel {info['kernel_s'] * 1e3:.2f} ms "
-              f"(in place {iref['kernel_s'] * 1e3:.2f} ms); counts {info['steps']}, {info['nplasma']}, {info['bad']}")
+              f"(in place {iref['kernel_s'] * 1e3:.2f} ms); counts {info['steps']}, {info['nplasma']}, {info['bad']}; "
+              f"{info['launches']} launches, {info['reruns']} reruns")
         assert np.array_equal(res, ref), f"two-phase render differs from the in-place march ({env})"
         assert (info["steps"], info["nplasma"], info["bad"]) == counts
+        if env:  # the small pool really chunked the jobs and ran out
+            assert info["reruns"] >= 1 and info["launches"] >= info["reruns"] + 2
+        else:
+            assert (info["launches"], info["reruns"]) == (1, 0)
+
+
+def test_record_pool_smaller_than_a_wave(c1):
+    """A pool of 64 blocks (PHX_REC_POOL_MB=0) cannot hold the records of a wave of 64 rays through
+    the torus: the render fails with PHX_E_RANGE instead of splitting a wave, and the snapshot renders
+    normally afterwards."""
+    run, snap = c1
+    rq = run.request(img_width=48, img_height=40, inclination=60.0, freqs=(86e9, 230e9))
+    ref, _ = _render(snap, rq, capi.MIXED)
+    with pytest.raises(capi.PhxError, match="record pool") as e:
+        _render(snap, rq, capi.MIXED, PHX_REC_POOL_MB=0)
+    assert e.value.code == -4  # PHX_E_RANGE
+    again, info = _render(snap, rq, capi.MIXED)
+    assert np.array_equal(again, ref) and info["reruns"] == 0
 
 
 def test_two_phase_batch_of_mixed_frequency_counts(c1):
