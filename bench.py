@@ -24,6 +24,7 @@ counts, over its HIP-event duration) and a CPU baseline: the reference's CPU bui
 when present) or the oracle, timed on a strided sample of the same image on the host's cores.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--precision mixed|fp32|fp64] [--quick]
+                    [--only multifreq|sweep|batch|serve]
 """
 from __future__ import annotations
 
@@ -465,6 +466,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=os.cpu_count() or 1)
     ap.add_argument("--cpu-stride", type=int, default=16)
+    ap.add_argument("--only", choices=["multifreq", "sweep", "batch", "serve"],
+                    help="run only this measurement (8 frequencies, the 16-point sweep, the 64-image batch or "
+                         "the serving loop) and print its figures as one JSON line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (scales the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -503,6 +507,16 @@ def main() -> int:
         nf = run.model.nfreqs
         rays = W * H * nf
         log(f"resident snapshot {snap.bytes / 2**20:.1f} MiB")
+        if args.only:
+            legs = {"multifreq": lambda: bench_multifreq(run, snap, capi, prec, sync),
+                    "sweep": lambda: bench_sweep(run, snap, capi, prec, sync),
+                    "batch": lambda: bench_batch(run, snap, capi, prec, sync),
+                    "serve": lambda: bench_serve(capi, prec, td)}
+            res = legs[args.only]()
+            snap.close()
+            run.close()
+            print(json.dumps({"only": args.only, "precision": args.precision, args.only: res}))
+            return 0
         wall, kern, info = timed_renders(snap, prec, args.steps, args.warmup, sync, args.precision)
         value = args.steps * rays / wall
         kernel_s = float(np.mean(kern))
