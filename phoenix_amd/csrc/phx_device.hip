@@ -1104,10 +1104,29 @@ static void check_uniform(const phx_model* a, const phx_model* b) {
                            "r_low, beta_c, sigma_cut, thetae bounds and rt_outer");
 }
 
+// Job n of a request's npix pixels x ntasks frequency chunks, in whole waves: each 64-pixel tile of
+// the pixel list once per chunk (tile-major), so that a wave of whole-wave refill marches the same 64
+// rays for every chunk, as a render with one chunk does (in mixed precision the rays of a wave share
+// its FP64 steps); the last, partial tile's r pixels follow once per chunk, and the waves there hold
+// copies of those r rays only.  *t: the chunk, *i: the index in the pixel list.
+__host__ __device__ inline void tile_major(int n, int npix, int ntasks, int* t, int* i) {
+  const int full = npix / 64 * 64;
+  if (n < full * ntasks) {
+    const int tile = n / (64 * ntasks), rem = n - tile * 64 * ntasks;
+    *t = rem / 64;
+    *i = tile * 64 + (rem & 63);
+  } else {
+    const int m = n - full * ntasks, r = npix - full;
+    *t = m / r;
+    *i = full + (m - *t * r);
+  }
+}
+
 __global__ void k_expand(const int* __restrict__ pix, int npix, int task0, int ntasks, int2* __restrict__ jobs) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= npix * ntasks) return;
-  const int t = n / npix, i = n % npix;
+  int t, i;
+  tile_major(n, npix, ntasks, &t, &i);
   jobs[n] = make_int2(task0 + t, pix[i]);
 }
 
@@ -1371,8 +1390,9 @@ static BatchPlan plan_batch(phx_snapshot* s, const phx_model* const* rqs, const 
     bp.nfmax = std::max(bp.nfmax, two ? sub[q]->nfreqs : std::min(CHUNK, sub[q]->nfreqs));
     morton_pixels(sub[q], pix);
     std::vector<int2>& jl = per[q];
-    for (int t = 0; t < nt; t++)
-      for (size_t i = 0; i < pix.size(); i += 64 * (size_t)sample_stride)
+    // whole waves of 64 pixels, each once per frequency chunk (the order of tile_major)
+    for (size_t i = 0; i < pix.size(); i += 64 * (size_t)sample_stride)
+      for (int t = 0; t < nt; t++)
         for (size_t k = i; k < std::min(pix.size(), i + 64); k++) jl.push_back(make_int2(t0 + t, pix[k]));
     bp.rays += (int64_t)(jl.size() / nt) * sub[q]->nfreqs;
   }
@@ -1690,7 +1710,8 @@ __global__ __launch_bounds__(BLOCK) void k_noalloc_setup(const NoAllocSetup<T> a
   }
   const int njobs = a.npix * a.ntasks;
   for (int j = n; j < njobs; j += gridDim.x * blockDim.x) {
-    const int t = j / a.npix, k = j - t * a.npix;
+    int t, k;
+    tile_major(j, a.npix, a.ntasks, &t, &k);
     const int2 ij = morton_nth((unsigned)k, a.W, a.H, a.levels);
     jobs[j] = make_int2(t, ij.x * a.H + ij.y);
   }
@@ -2274,9 +2295,12 @@ static void serve_loop(Server* sv) {
       morton_pixels(&r->rq, pix);
       r->two = two_phase(&r->rq);
       const int nch = r->two ? 1 : (r->rq.nfreqs + CHUNK - 1) / CHUNK;
-      r->jobs.reserve(pix.size() * nch);
-      for (int c = 0; c < nch; c++)
-        for (int32_t p : pix) r->jobs.push_back(make_int2(c, p));
+      r->jobs.resize(pix.size() * nch);
+      for (size_t n = 0; n < r->jobs.size(); n++) {  // whole waves of 64 pixels, each once per chunk
+        int t, i;
+        tile_major((int)n, (int)pix.size(), nch, &t, &i);
+        r->jobs[n] = make_int2(t, pix[i]);
+      }
     }
     try {
       // keep both slots of every snapshot with work queued

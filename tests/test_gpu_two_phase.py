@@ -116,3 +116,30 @@ def test_sweep_equals_each_point_alone(c1, precision):
               f"batch apart == alone {np.array_equal(a, alone)}")
         assert np.array_equal(a, alone)
         assert np.array_equal(g, alone), f"sweep point {k} differs from its own render"
+
+
+@pytest.mark.parametrize("shape,nf", [((1, 1), 2), ((7, 5), 3), ((3, 130), 32), ((65, 1), 32)])
+def test_two_phase_edge_shapes_and_the_frequency_cap(c1, shape, nf):
+    """Images smaller than a wave or of ragged size, and the 32-frequency cap (PHX_FREQS_MAX): the
+    two-phase render equals the in-place march bit for bit."""
+    run, snap = c1
+    rq = run.request(img_width=shape[0], img_height=shape[1], inclination=70.0,
+                     freqs=tuple(np.geomspace(86e9, 690e9, nf)))
+    ref, iref = _render(snap, rq, capi.MIXED, PHX_TWO_PHASE_MIN_NF=0)
+    res, info = _render(snap, rq, capi.MIXED)
+    assert np.array_equal(res, ref)
+    assert (info["steps"], info["nplasma"], info["bad"]) == (iref["steps"], iref["nplasma"], iref["bad"])
+
+
+def test_sweep_of_ragged_images_and_frequency_counts(c1):
+    """Sweep points through a camera whose image is smaller than a wave (5 x 3) and through a ragged
+    one (33 x 9), with 1 to 32 frequencies per point: each equals its own render."""
+    run, snap = c1
+    for w, h in ((5, 3), (33, 9)):
+        cam = dict(img_width=w, img_height=h, inclination=40.0)
+        pts = [run.request(**cam, freqs=tuple(np.geomspace(86e9, 690e9, nf)), m_unit=m, r_high=rh)
+               for nf, m, rh in ((1, 1e18, 3.0), (32, 2e18, 10.0), (5, 7e17, 1.0))]
+        got, _ = snap.render_batch(pts, capi.MIXED)
+        for k, p in enumerate(pts):
+            alone, _ = snap.render(p, capi.MIXED)
+            assert np.array_equal(got[k], alone), f"{w}x{h} point {k} differs from its own render"
