@@ -398,7 +398,12 @@ __global__ __launch_bounds__(BLOCK) void k_rays(const Cam* __restrict__ cams, co
   if (n >= njobs) return;
   const int2 jb = jobs[n];
   constexpr int REC = M == MIX ? StHam::REC : StRef<typename ModeOf<M>::T>::REC;
-  ray_start<M>(cams[jb.x], jb.y, static_cast<R*>(out_) + (size_t)n * REC);
+  R* const o = static_cast<R*>(out_) + (size_t)n * REC;
+  if (jb.y < 0) {  // a padding job (no pixel): a zero record, which its lane never marches
+    for (int q = 0; q < REC; q++) o[q] = R(0);
+    return;
+  }
+  ray_start<M>(cams[jb.x], jb.y, o);
 }
 
 // zeroes a launch's job counter and (when tot is given) its run statistics: a kernel rather than a
@@ -904,6 +909,34 @@ static Tune tune_from_env(Tune t) {
 }
 
 // what one march launch needs beyond the device buffers of the snapshot
+// The jobs of a request's npix pixels x ntasks frequency chunks, in whole waves: each 64-pixel tile
+// of the pixel list once per chunk (tile-major), so that a wave of whole-wave refill marches the same
+// rays for every chunk, as a render with one chunk does (in mixed precision the rays of a wave share
+// its FP64 steps).  With more than one chunk a partial last tile is padded to 64 jobs with jobs of
+// no pixel (-1), which end at once; with one chunk the jobs are the pixels in order.
+__host__ __device__ inline int64_t wave_jobs(int npix, int ntasks) {
+  return ntasks <= 1 ? (int64_t)npix : (int64_t)((npix + 63) / 64) * 64 * ntasks;
+}
+// job n of that list: *t the chunk, return the index in the pixel list (-1: padding)
+__host__ __device__ inline int wave_job(int n, int npix, int ntasks, int* t) {
+  if (ntasks <= 1) {
+    *t = 0;
+    return n;
+  }
+  const int tile = n / (64 * ntasks), rem = n - tile * 64 * ntasks;
+  *t = rem / 64;
+  const int i = tile * 64 + (rem & 63);
+  return i < npix ? i : -1;
+}
+
+// jobs of [j0, j1) of such a list that march a pixel (padding excluded)
+static int real_jobs(int j0, int j1, int npix, int ntasks) {
+  if (ntasks <= 1 || npix <= 0) return j1 - j0;
+  int n = 0, t;
+  for (int j = j0; j < j1; j++) n += wave_job(j, npix, ntasks, &t) >= 0;
+  return n;
+}
+
 struct LaunchInfo {
   int njobs = 0, ntasks = 0, nfmax = 1;
   const phx_model* uni = nullptr;  // the request whose launch-wide parameters (stepping, cuts) apply
@@ -911,6 +944,7 @@ struct LaunchInfo {
   bool capture = false;            // being recorded into a HIP graph: no timing events
   bool two_phase = false;          // march once recording samples, then one transfer lane per frequency
   int sweep = 0;                   // two-phase: the number of parameter points sharing the rays (0: none)
+  int npix = 0;                    // a single request's pixels (the jobs are wave_job's list of npix x ntasks)
 };
 
 template <int M, int NF, bool Q, int BS>
@@ -1073,7 +1107,7 @@ static void march_jobs(phx_snapshot* s, Slot& sl, const LaunchInfo& li, const Tu
       launch_nf<M, 4>(s, sl, li, j0, j1, tn, info);
     if (progress) {
       HIPCHECK(hipStreamSynchronize(stream_of(s, sl)));
-      progress(j1 - j0, user);
+      progress(real_jobs(j0, j1, li.npix, li.ntasks), user);
     }
   }
 }
@@ -1104,30 +1138,12 @@ static void check_uniform(const phx_model* a, const phx_model* b) {
                            "r_low, beta_c, sigma_cut, thetae bounds and rt_outer");
 }
 
-// Job n of a request's npix pixels x ntasks frequency chunks, in whole waves: each 64-pixel tile of
-// the pixel list once per chunk (tile-major), so that a wave of whole-wave refill marches the same 64
-// rays for every chunk, as a render with one chunk does (in mixed precision the rays of a wave share
-// its FP64 steps); the last, partial tile's r pixels follow once per chunk, and the waves there hold
-// copies of those r rays only.  *t: the chunk, *i: the index in the pixel list.
-__host__ __device__ inline void tile_major(int n, int npix, int ntasks, int* t, int* i) {
-  const int full = npix / 64 * 64;
-  if (n < full * ntasks) {
-    const int tile = n / (64 * ntasks), rem = n - tile * 64 * ntasks;
-    *t = rem / 64;
-    *i = tile * 64 + (rem & 63);
-  } else {
-    const int m = n - full * ntasks, r = npix - full;
-    *t = m / r;
-    *i = full + (m - *t * r);
-  }
-}
-
 __global__ void k_expand(const int* __restrict__ pix, int npix, int task0, int ntasks, int2* __restrict__ jobs) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= npix * ntasks) return;
-  int t, i;
-  tile_major(n, npix, ntasks, &t, &i);
-  jobs[n] = make_int2(task0 + t, pix[i]);
+  if (n >= wave_jobs(npix, ntasks)) return;
+  int t;
+  const int i = wave_job(n, npix, ntasks, &t);
+  jobs[n] = make_int2(task0 + t, i < 0 ? -1 : pix[i]);
 }
 
 // pixel order of a full image: Morton order of (i, j) (the reference's GPU schedule,
@@ -1328,8 +1344,8 @@ static void render(phx_snapshot* s, const phx_model* rq, int precision, const in
     ntasks = (int)tk.size();
     upload_tasks(s, s->slot, tk, std::vector<Cam>(ntasks, cam));
   }
-  const int64_t ntot = (int64_t)njobs * ntasks;
-  if (ntot > INT32_MAX) throw Error(PHX_E_RANGE, "too many jobs in one render");
+  const int64_t ntot = wave_jobs(njobs, ntasks);
+  if ((int64_t)njobs * ntasks > INT32_MAX / 2 || ntot > INT32_MAX) throw Error(PHX_E_RANGE, "too many jobs in one render");
   ensure(s->slot.jobs, (size_t)std::max<int64_t>(ntot, 1) * sizeof(int2));
   if (ntot > 0)
     k_expand<<<(unsigned)((ntot + BLOCK - 1) / BLOCK), BLOCK, 0, s->stream>>>((const int*)s->pix.p, njobs, 0, ntasks,
@@ -1346,6 +1362,7 @@ static void render(phx_snapshot* s, const phx_model* rq, int precision, const in
   li.nfmax = two ? rq->nfreqs : std::min(CHUNK, rq->nfreqs) <= 1 ? 1 : std::min(CHUNK, rq->nfreqs) <= 2 ? 2 : 4;
   li.uni = rq;
   li.two_phase = two;
+  li.npix = njobs;
   run_precision(s, precision, li, tn, info, progress, user);
   info->rays = (int64_t)njobs * rq->nfreqs;
   s->last_nres = (int64_t)nres;
@@ -1390,11 +1407,18 @@ static BatchPlan plan_batch(phx_snapshot* s, const phx_model* const* rqs, const 
     bp.nfmax = std::max(bp.nfmax, two ? sub[q]->nfreqs : std::min(CHUNK, sub[q]->nfreqs));
     morton_pixels(sub[q], pix);
     std::vector<int2>& jl = per[q];
-    // whole waves of 64 pixels, each once per frequency chunk (the order of tile_major)
+    // whole waves of 64 pixels, each once per frequency chunk (the order of wave_job); a partial
+    // tile is padded to a whole wave with jobs of no pixel (-1), which end at once, so that every
+    // request's waves start on a multiple of 64 and march the rays they march when it renders alone
     for (size_t i = 0; i < pix.size(); i += 64 * (size_t)sample_stride)
-      for (int t = 0; t < nt; t++)
-        for (size_t k = i; k < std::min(pix.size(), i + 64); k++) jl.push_back(make_int2(t0 + t, pix[k]));
-    bp.rays += (int64_t)(jl.size() / nt) * sub[q]->nfreqs;
+      for (int t = 0; t < nt; t++) {
+        const size_t e = std::min(pix.size(), i + 64);
+        for (size_t k = i; k < e; k++) jl.push_back(make_int2(t0 + t, pix[k]));
+        for (size_t k = e; k < i + 64; k++) jl.push_back(make_int2(t0 + t, -1));
+      }
+    int64_t real = 0;
+    for (size_t i = 0; i < pix.size(); i += 64 * (size_t)sample_stride) real += (int64_t)std::min<size_t>(64, pix.size() - i);
+    bp.rays += real * sub[q]->nfreqs;
   }
   std::vector<int2> jobs;
   if (order == 1) {  // round-robin over requests, 64 jobs (one wave) at a time
@@ -1710,8 +1734,7 @@ __global__ __launch_bounds__(BLOCK) void k_noalloc_setup(const NoAllocSetup<T> a
   }
   const int njobs = a.npix * a.ntasks;
   for (int j = n; j < njobs; j += gridDim.x * blockDim.x) {
-    int t, k;
-    tile_major(j, a.npix, a.ntasks, &t, &k);
+    const int t = j / a.npix, k = j - t * a.npix;  // one task: the pixels in order
     const int2 ij = morton_nth((unsigned)k, a.W, a.H, a.levels);
     jobs[j] = make_int2(t, ij.x * a.H + ij.y);
   }
@@ -2295,11 +2318,11 @@ static void serve_loop(Server* sv) {
       morton_pixels(&r->rq, pix);
       r->two = two_phase(&r->rq);
       const int nch = r->two ? 1 : (r->rq.nfreqs + CHUNK - 1) / CHUNK;
-      r->jobs.resize(pix.size() * nch);
+      r->jobs.resize((size_t)wave_jobs((int)pix.size(), nch));
       for (size_t n = 0; n < r->jobs.size(); n++) {  // whole waves of 64 pixels, each once per chunk
-        int t, i;
-        tile_major((int)n, (int)pix.size(), nch, &t, &i);
-        r->jobs[n] = make_int2(t, pix[i]);
+        int t;
+        const int i = wave_job((int)n, (int)pix.size(), nch, &t);
+        r->jobs[n] = make_int2(t, i < 0 ? -1 : pix[i]);
       }
     }
     try {

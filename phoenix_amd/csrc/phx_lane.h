@@ -603,6 +603,7 @@ struct Lane {
       tau[f] = tauF[f] = T(0);
     }
     steps = turns = stop = np = bad = 0;
+    if (pix < 0) stop = 1;  // a padding job (no pixel): the ray ends at its first pass, writing nothing
     jobn = n;
     rhead = rcur = -1;
     rn = 0;
@@ -784,6 +785,7 @@ struct Lane {
   // zeros when no plasma segment was processed.  np_counted: the sums were built by this lane
   // (otherwise a ray without segments has folded nothing in and its sums are the zeros anyway).
   PHX_DEV void write_out(const MarchArgs<M>& A, bool np_counted) const {
+    if (pix < 0) return;  // a padding job
     if constexpr (REC) {  // the ray's records for the transfer kernel
       const gmut<int> o = glb_mut(reinterpret_cast<int*>(side_mem(A).rray) + 2 * (size_t)jobn);
       o[0] = rhead;
@@ -811,6 +813,7 @@ PHX_DEV int transfer_one(const TaskDev<typename ModeOf<M>::T>* tasks, const int2
   using T = typename ModeOf<M>::T;
   constexpr bool KAHAN = M == MIX;
   const int2 jb = jobs[job];
+  if (jb.y < 0) return 0;  // a padding job (no pixel)
   const TaskDev<T> tk = load_global(tasks + jb.x);
   if (f >= tk.nf) return 0;
   const int2 rr = rray[job];
@@ -864,6 +867,7 @@ PHX_DEV int transfer_sweep(const TaskDev<typename ModeOf<M>::T>* tasks, int r, c
   constexpr bool KAHAN = M == MIX;
   *np = 0;
   const int2 jb = jobs[job];
+  if (jb.y < 0) return 0;  // a padding job (no pixel)
   const TaskDev<T> tk = load_global(tasks + r);
   if (f >= tk.nf) return 0;
   const int2 rr = rray[job];

@@ -118,10 +118,14 @@ def test_sweep_equals_each_point_alone(c1, precision):
         assert np.array_equal(g, alone), f"sweep point {k} differs from its own render"
 
 
-@pytest.mark.parametrize("shape,nf", [((1, 1), 2), ((7, 5), 3), ((3, 130), 32), ((65, 1), 32)])
+@pytest.mark.parametrize("shape,nf", [((1, 1), 2), ((7, 5), 3), ((3, 130), 32), ((65, 1), 32), ((40, 1), 8),
+                                      ((5, 9), 9)])
 def test_two_phase_edge_shapes_and_the_frequency_cap(c1, shape, nf):
     """Images smaller than a wave or of ragged size, and the 32-frequency cap (PHX_FREQS_MAX): the
-    two-phase render equals the in-place march bit for bit."""
+    two-phase render equals the in-place march bit for bit.  In place, a request with more than 4
+    frequencies marches each ray once per chunk of 4; its job list takes each 64-pixel tile once per
+    chunk and pads a partial last tile to a whole wave (40 x 1 and 5 x 9 pixels: the chunks' last
+    tiles would otherwise share waves in other combinations than the one-chunk render's)."""
     run, snap = c1
     rq = run.request(img_width=shape[0], img_height=shape[1], inclination=70.0,
                      freqs=tuple(np.geomspace(86e9, 690e9, nf)))
