@@ -208,6 +208,10 @@ int phx_snapshot_flux(phx_snapshot* snap, double* spec, int32_t nfreqs);
  * and frequency chunk, so images of uneven size, inclination, frequency, m_unit and R_high share
  * the GPU.  Requests may differ only in camera, image size, frequencies, m_unit and R_high.
  * res[i] (res or res[i] may be NULL) receives request i's img_width*img_height*nfreqs results.
+ * Each request's jobs fill whole waves of its own rays, and requests with 2 or more frequencies
+ * render in two phases (a launch of each kind when both are present), so every image equals its
+ * phx_snapshot_render bit for bit under the default launch settings.  Requests that share the
+ * camera and image (parameter-sweep points) march their rays once.
  * The batched form of device::render (src/device/kernel.cu:284-360), which marches one image. */
 int phx_snapshot_render_batch(phx_snapshot* snap, const phx_model* const* requests, int32_t n, int32_t precision,
                               phx_freqout* const* res, phx_info* info);
