@@ -909,26 +909,6 @@ static Tune tune_from_env(Tune t) {
 }
 
 // what one march launch needs beyond the device buffers of the snapshot
-// The jobs of a request's npix pixels x ntasks frequency chunks, in whole waves: each 64-pixel tile
-// of the pixel list once per chunk (tile-major), so that a wave of whole-wave refill marches the same
-// rays for every chunk, as a render with one chunk does (in mixed precision the rays of a wave share
-// its FP64 steps).  With more than one chunk a partial last tile is padded to 64 jobs with jobs of
-// no pixel (-1), which end at once; with one chunk the jobs are the pixels in order.
-__host__ __device__ inline int64_t wave_jobs(int npix, int ntasks) {
-  return ntasks <= 1 ? (int64_t)npix : (int64_t)((npix + 63) / 64) * 64 * ntasks;
-}
-// job n of that list: *t the chunk, return the index in the pixel list (-1: padding)
-__host__ __device__ inline int wave_job(int n, int npix, int ntasks, int* t) {
-  if (ntasks <= 1) {
-    *t = 0;
-    return n;
-  }
-  const int tile = n / (64 * ntasks), rem = n - tile * 64 * ntasks;
-  *t = rem / 64;
-  const int i = tile * 64 + (rem & 63);
-  return i < npix ? i : -1;
-}
-
 // jobs of [j0, j1) of such a list that march a pixel (padding excluded)
 static int real_jobs(int j0, int j1, int npix, int ntasks) {
   if (ntasks <= 1 || npix <= 0) return j1 - j0;

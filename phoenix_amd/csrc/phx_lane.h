@@ -960,4 +960,24 @@ PHX_DEV void ray_start(Cam C, int idx, RecOf<M>* out_rec) {
   }
 }
 
+// The jobs of a request's npix pixels x ntasks frequency chunks, in whole waves: each 64-pixel tile
+// of the pixel list once per chunk (tile-major), so that a wave of whole-wave refill marches the same
+// rays for every chunk, as a render with one chunk does (in mixed precision the rays of a wave share
+// its FP64 steps).  With more than one chunk a partial last tile is padded to 64 jobs with jobs of
+// no pixel (-1), which end at once; with one chunk the jobs are the pixels in order.
+PHX_HD int64_t wave_jobs(int npix, int ntasks) {
+  return ntasks <= 1 ? (int64_t)npix : (int64_t)((npix + 63) / 64) * 64 * ntasks;
+}
+// job n of that list: *t the chunk, return the index in the pixel list (-1: padding)
+PHX_HD int wave_job(int n, int npix, int ntasks, int* t) {
+  if (ntasks <= 1) {
+    *t = 0;
+    return n;
+  }
+  const int tile = n / (64 * ntasks), rem = n - tile * 64 * ntasks;
+  *t = rem / 64;
+  const int i = tile * 64 + (rem & 63);
+  return i < npix ? i : -1;
+}
+
 }  // namespace phxd

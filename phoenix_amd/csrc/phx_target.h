@@ -15,6 +15,7 @@
 #include <cstdint>
 
 #define PHX_DEV inline
+#define PHX_HD inline
 #define PHX_CONSTANT static const
 #define PHX_SCHED_BARRIER() ((void)0)
 
@@ -87,6 +88,7 @@ PHX_DEV T load_constant(const T* p, int i) {
 #include <hip/hip_runtime.h>
 
 #define PHX_DEV __device__ __forceinline__
+#define PHX_HD __host__ __device__ inline  // host and device (job lists)
 #define PHX_CONSTANT __constant__ static const
 #define PHX_SCHED_BARRIER() __builtin_amdgcn_sched_barrier(0)
 
