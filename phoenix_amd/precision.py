@@ -1,4 +1,4 @@
-"""Precision report: the FP32 and mixed-precision marches against the FP64 march as the judge.
+"""Precision report: the FP32 and mixed-precision marches against the FP64 march as the accuracy reference.
 
 For each snapshot and inclination, one image of `size`^2 pixels at `nf` frequencies log-spaced
 from 86 to 690 GHz is rendered in FP64, FP32 and mixed precision on the resident snapshot, and
@@ -37,7 +37,7 @@ def fractions(F: np.ndarray):
 
 
 def pixel_figures(a: np.ndarray, b: np.ndarray, floor: float = 1e-6) -> dict:
-    """a, b: [npix, 6] of one frequency (b is the judge)"""
+    """a, b: [npix, 6] of one frequency (b is the FP64 reference)"""
     fin = np.all(np.isfinite(a[:, :4]), axis=1) & np.all(np.isfinite(b[:, :4]), axis=1)
     I = b[:, 0]
     peak = np.max(np.abs(I[fin])) if fin.any() else 0.0
@@ -62,8 +62,8 @@ def report(snapshots, incls, size, nf, out=sys.stdout):
             for incl in incls:
                 rq = run.request(img_width=size, img_height=size, inclination=float(incl), freqs=freqs)
                 t0 = time.time()
-                judge, ij = snap.render(rq, capi.FP64)
-                F64 = run.flux(judge, rq)
+                ref64, ij = snap.render(rq, capi.FP64)
+                F64 = run.flux(ref64, rq)
                 lp64, cp64, ev64 = fractions(F64)
                 for label, prec in (("fp32", capi.FP32), ("mixed", capi.MIXED)):
                     res, inf = snap.render(rq, prec)
@@ -72,7 +72,7 @@ def report(snapshots, incls, size, nf, out=sys.stdout):
                     dev = np.abs((ev - ev64 + 90.0) % 180.0 - 90.0)
                     per_f = []
                     for f in range(nf):
-                        pf = pixel_figures(res[:, f], judge[:, f])
+                        pf = pixel_figures(res[:, f], ref64[:, f])
                         per_f.append({"freq_hz": freqs[f], "flux_I": float(F64[f, 0]),
                                       "dF_I": float(abs(F[f, 0] - F64[f, 0]) / abs(F64[f, 0])),
                                       "dF_Q": float(abs(F[f, 1] - F64[f, 1]) / abs(F64[f, 0])),

@@ -1,4 +1,4 @@
-"""Image comparison against the FP64 judge with the acceptance gates of the path:
+"""Image comparison against the FP64 march with the acceptance gates of the path:
 
 * per pixel, on pixels whose |I| is above 1e-6 of the image peak:
   |I - I_ref| / |I_ref| <= 1e-3 and |X - X_ref| / |I_ref| <= 1e-3 for X in Q, U, V;

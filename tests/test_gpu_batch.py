@@ -67,13 +67,13 @@ def test_batch_mixed_frequency_counts(c1, precision):
         assert np.array_equal(rb, alone), f"{rq.img_width}x{rq.img_height} nf={rq.nfreqs}: batch != alone"
 
 
-def test_batch_vs_fp64_judge(c1):
+def test_batch_vs_fp64(c1):
     """Every image of a mixed-precision batch against the same images rendered in FP64."""
     run, snap = c1
     reqs = _requests(run, SPECS + SPECS_NF)
     mixed, _ = snap.render_batch(reqs, capi.MIXED)
-    judge, _ = snap.render_batch(reqs, capi.FP64)
-    for rq, a, b in zip(reqs, mixed, judge):
+    ref64, _ = snap.render_batch(reqs, capi.FP64)
+    for rq, a, b in zip(reqs, mixed, ref64):
         figs = parity.compare(a, b, preset="c1")
         print(f"\n{rq.img_width}x{rq.img_height} i={rq.inclination} nf={rq.nfreqs}: {parity.summary(figs)}")
         assert parity.passes(figs), parity.summary(figs)

@@ -35,10 +35,10 @@ def c2(snapdir, tmp_path_factory):
 
 def test_headline_image_within_the_gates_of_fp64(c2):
     run, snap = c2
-    judge, _ = snap.render(None, capi.FP64)
+    ref64, _ = snap.render(None, capi.FP64)
     for name, prec, tune in (("mixed", capi.MIXED, None), ("plain fp32", capi.FP32, {"queue": 0, "refill": 64, "escape": 1})):
         res, info = snap.render(None, prec, tune=tune)
-        figs = parity.compare(res, judge, preset="c2")
+        figs = parity.compare(res, ref64, preset="c2")
         print(f"\n1024^2 {name}: {parity.summary(figs)}; kernel {info['kernel_s'] * 1e3:.1f} ms")
         assert parity.passes(figs), parity.summary(figs)
 
@@ -81,8 +81,8 @@ def test_c4_batch_equals_each_image_alone(c2):
 def test_precision_form_2048_8_frequencies_within_the_gates_of_fp64(c2):
     run, snap = c2
     rq = run.request(img_width=2048, img_height=2048, inclination=60.0, freqs=FREQS8)
-    judge, _ = snap.render(rq, capi.FP64)
+    ref64, _ = snap.render(rq, capi.FP64)
     res, info = snap.render(rq, capi.MIXED)
-    figs = parity.compare(res, judge, preset="c2")
+    figs = parity.compare(res, ref64, preset="c2")
     print(f"\n2048^2 x 8 mixed: {parity.summary(figs)}; kernel {info['kernel_s'] * 1e3:.1f} ms")
     assert parity.passes(figs), parity.summary(figs)

@@ -2,7 +2,7 @@
 
 * device decode of the dump against the oracle's host decode;
 * every golden case at FP64 and FP32 against the reference's FP64 build (the accuracy
-  judge), gated per pixel and on total flux (tests/parity.py), with the run statistics
+  ref64), gated per pixel and on total flux (tests/parity.py), with the run statistics
   (geodesic steps, plasma segments, dropped segments) against the reference's;
 * a case outside the fixtures against the oracle rendered live on the host;
 * resident-snapshot reuse, partial job lists and error returns.

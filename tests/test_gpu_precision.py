@@ -1,5 +1,5 @@
 """The precision form (FP32 bulk, FP64 near the horizon and the poles, FP64 cell location, Kahan
-Stokes sums) and the plain FP32 march against the FP64 march as the judge, at 8 frequencies
+Stokes sums) and the plain FP32 march against the FP64 march as the accuracy reference, at 8 frequencies
 log-spaced over 86-690 GHz, three inclinations and both spins: total I flux within 1e-4, Q/U/V
 flux within 1e-4 of I, image LP / CP within 1e-4 and EVPA within 0.05 deg, per-pixel gates with
 the chaotic-pixel allowance of tests/parity.py.  (phoenix_amd/precision.py is the full-size report.)"""
@@ -21,12 +21,12 @@ def test_precision_form_vs_fp64(snapdir, tmp_path, preset):
     snap = capi.Snapshot(run, capi.TABLE_FP32 | capi.TABLE_FP64)
     for incl in (10.0, 60.0, 163.0):
         rq = run.request(img_width=96, img_height=96, inclination=incl, freqs=FREQS)
-        judge, _ = snap.render(rq, capi.FP64)
-        F64 = run.flux(judge, rq)
+        ref64, _ = snap.render(rq, capi.FP64)
+        F64 = run.flux(ref64, rq)
         lp0, cp0, ev0 = fractions(F64)
         for name, prec in (("fp32", capi.FP32), ("mixed", capi.MIXED)):
             res, info = snap.render(rq, prec)
-            figs = parity.compare(res, judge, preset=preset)
+            figs = parity.compare(res, ref64, preset=preset)
             F = run.flux(res, rq)
             lp, cp, ev = fractions(F)
             dev = np.abs((ev - ev0 + 90.0) % 180.0 - 90.0)
@@ -56,11 +56,11 @@ def test_outlier_rate_within_1p5x_of_the_reference(case, snapdir, tmp_path):
     run = capi.Run(configs.write_config(str(tmp_path / "o.toml"), dump, grid))
     snap = capi.Snapshot(run, capi.TABLE_FP32 | capi.TABLE_FP64)
     rq = run.request(img_width=512, img_height=512, inclination=float(meas["inclination"]), freqs=FREQS)
-    judge, _ = snap.render(rq, capi.FP64)
+    ref64, _ = snap.render(rq, capi.FP64)
     ref_bad = sum(meas["nbad"])
     for name, prec in (("fp32", capi.FP32), ("mixed", capi.MIXED)):
         res, _ = snap.render(rq, prec)
-        figs = parity.compare(res, judge, allowance=False)
+        figs = parity.compare(res, ref64, allowance=False)
         ours = sum(d["nbad"] for d in figs)
         print(f"\n{case} {name}: {ours} pixel-frequencies outside the gate, the reference's FP32 build {ref_bad} "
               f"(of {sum(d['gated'] for d in figs)} gated); flux dI max {max(d['flux_dI'] for d in figs):.1e}")
