@@ -5,7 +5,9 @@
 * the same image at the 8 frequencies of the precision form: two-phase equals in place bit for bit;
 * the 16-point sweep of that image: every point equals its own render bit for bit;
 * the 64-image batch of uneven size (13.9M rays): every image equals its own render bit for bit;
-* one 2048 x 2048 x 8-frequency image of the precision form (c2, i = 60 deg) against the FP64 march.
+* two 2048 x 2048 x 8-frequency images of the precision form (c2 at i = 60 deg, c2_a05 at i = 163 deg)
+  against the FP64 march;
+* requests of the serving workload's kind served together: each equals its render alone.
 """
 import os
 
@@ -86,3 +88,51 @@ def test_precision_form_2048_8_frequencies_within_the_gates_of_fp64(c2):
     figs = parity.compare(res, ref64, preset="c2")
     print(f"\n2048^2 x 8 mixed: {parity.summary(figs)}; kernel {info['kernel_s'] * 1e3:.1f} ms")
     assert parity.passes(figs), parity.summary(figs)
+
+
+def test_precision_form_2048_spin_half_south_within_the_gates_of_fp64(snapdir, tmp_path):
+    """The second C3 snapshot (spin 0.5) from below the equator (i = 163 deg), 2048 x 2048 x 8."""
+    dump, grid, _ = S.ensure_snapshot("c2_a05", snapdir)
+    run = capi.Run(configs.write_config(str(tmp_path / "a05.toml"), dump, grid))
+    snap = capi.Snapshot(run, capi.TABLE_FP32 | capi.TABLE_FP64)
+    try:
+        rq = run.request(img_width=2048, img_height=2048, inclination=163.0, freqs=FREQS8)
+        ref64, _ = snap.render(rq, capi.FP64)
+        res, info = snap.render(rq, capi.MIXED)
+        figs = parity.compare(res, ref64, preset="c2_a05")
+        print(f"\n2048^2 x 8 spin 0.5 i=163 mixed: {parity.summary(figs[:2])}")
+        assert parity.passes(figs), parity.summary(figs)
+    finally:
+        snap.close()
+        run.close()
+
+
+def test_served_c5_requests_equal_their_renders_alone(snapdir, tmp_path):
+    """Requests of the serving workload's kind (the two 1.9M / 1.4M-cell c5 snapshots, 256^2 to
+    1024^2 pixels, 1 to 4 frequencies, the units of each drawn at random) submitted together to one
+    server: every result equals the same request rendered alone, bit for bit."""
+    from bench import c5_request
+
+    runs, snaps = [], []
+    for name in ("c5", "c5_a05"):
+        dump, grid, _ = S.ensure_snapshot(name, snapdir)
+        r = capi.Run(configs.write_config(str(tmp_path / f"{name}.toml"), dump, grid))
+        runs.append(r)
+        snaps.append(capi.Snapshot(r, capi.TABLE_FP32))
+    rng = np.random.default_rng(11)
+    reqs = [(k % 2, c5_request(runs[k % 2], rng)) for k in range(8)]
+    srv = capi.Server(snaps, capi.MIXED)
+    try:
+        got = [srv.wait(t)[0] for t in [srv.submit(k, rq) for k, rq in reqs]]
+    finally:
+        srv.stop()
+    try:
+        for (k, rq), g in zip(reqs, got):
+            alone, _ = snaps[k].render(rq, capi.MIXED)
+            print(f"\nsnapshot {k}: {rq.img_width}^2 x {rq.nfreqs}: served == alone {np.array_equal(g, alone)}")
+            assert np.array_equal(g, alone)
+    finally:
+        for s in snaps:
+            s.close()
+        for r in runs:
+            r.close()
