@@ -108,7 +108,7 @@ def test_precision_form_2048_spin_half_south_within_the_gates_of_fp64(snapdir, t
 
 
 def test_served_c5_requests_equal_their_renders_alone(snapdir, tmp_path):
-    """Requests of the serving workload's kind (the two 1.9M / 1.4M-cell c5 snapshots, 256^2 to
+    """Requests of the serving workload's kind (the two 1.9M / 1.4M-cell c5 snapshots, 128^2 to
     1024^2 pixels, 1 to 4 frequencies, the units of each drawn at random) submitted together to one
     server: every result equals the same request rendered alone, bit for bit."""
     from bench import c5_request
