@@ -246,6 +246,32 @@ def bench_sweep(run, snap, capi, prec, sync, npts: int = 16) -> dict:
     return out
 
 
+def bench_host(run, snap, capi, prec, sync, steps: int = 3) -> dict:
+    """The headline image with its result handed to the host, as the drop-in boundary does: the
+    resident snapshot's render with the 50 MB result copied over PCIe, and the one-shot path
+    (phx_dev_render, device::render's contract: per-call device memory, table upload, the dump read
+    and decoded on the device, march, copy back).  Neither is `value`."""
+    rays = W * H * run.model.nfreqs
+    out = {}
+    snap.render(None, prec)
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        snap.render(None, prec)
+    out["resident_to_host_rays_freq_per_s"] = steps * rays / (time.perf_counter() - t0)
+    run.dev_render(prec)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, info = run.dev_render(prec)
+    w = time.perf_counter() - t0
+    out.update(one_shot_rays_freq_per_s=steps * rays / w, one_shot_ms=w / steps * 1e3,
+               one_shot_kernel_ms=info["kernel_s"] * 1e3, one_shot_decode_ms=info["decode_s"] * 1e3,
+               one_shot_upload_ms=info["upload_s"] * 1e3)
+    log(f"result to the host: resident {out['resident_to_host_rays_freq_per_s']:.4g} rays*freq/s, one-shot "
+        f"phx_dev_render {out['one_shot_ms']:.1f} ms per image ({out['one_shot_rays_freq_per_s']:.4g} rays*freq/s)")
+    return out
+
+
 def bench_batch(run, snap, capi, prec, sync) -> dict:
     reqs = c4_requests(run)
     rays = sum(r.img_width * r.img_height * r.nfreqs for r in reqs)
@@ -466,7 +492,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=os.cpu_count() or 1)
     ap.add_argument("--cpu-stride", type=int, default=16)
-    ap.add_argument("--only", choices=["multifreq", "sweep", "batch", "serve"],
+    ap.add_argument("--only", choices=["multifreq", "sweep", "host", "batch", "serve"],
                     help="run only this measurement (8 frequencies, the 16-point sweep, the 64-image batch or "
                          "the serving loop) and print its figures as one JSON line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -510,6 +536,7 @@ def main() -> int:
         if args.only:
             legs = {"multifreq": lambda: bench_multifreq(run, snap, capi, prec, sync),
                     "sweep": lambda: bench_sweep(run, snap, capi, prec, sync),
+                    "host": lambda: bench_host(run, snap, capi, prec, sync),
                     "batch": lambda: bench_batch(run, snap, capi, prec, sync),
                     "serve": lambda: bench_serve(capi, prec, td)}
             res = legs[args.only]()
@@ -574,6 +601,10 @@ def main() -> int:
                 extra["sweep_16"] = bench_sweep(run, snap, capi, prec, sync)
             except Exception as e:
                 extra["sweep_16"] = {"error": str(e)}
+            try:
+                extra["host_result"] = bench_host(run, snap, capi, prec, sync)
+            except Exception as e:
+                extra["host_result"] = {"error": str(e)}
             try:
                 extra["batch_c4"] = bench_batch(run, snap, capi, prec, sync)
             except Exception as e:  # reported, not fatal: the headline is already measured
