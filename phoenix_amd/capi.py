@@ -97,6 +97,10 @@ class ServeStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+# device::Progress (src/device/kernel.cuh:32): jobs done in the chunk, user pointer
+PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_int32, ctypes.c_void_p)
+
+
 class PhxError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"phx error {code}: {msg}")
@@ -235,8 +239,9 @@ class Run:
         check(lib().phx_schedule(ctypes.byref(m), int(morton), jobs.ctypes.data_as(P_int), n))
         return jobs
 
-    def dev_render(self, precision: int = FP32, model: Model | None = None, jobs=None):
-        """phx_dev_render: the drop-in one-shot device path.  Returns (res [W*H, nf, 6], info)."""
+    def dev_render(self, precision: int = FP32, model: Model | None = None, jobs=None, progress=None):
+        """phx_dev_render: the drop-in one-shot device path.  Returns (res [W*H, nf, 6], info).
+        progress: called with the jobs done after each chunk (device::Progress)."""
         m = model or self.model
         if jobs is None:
             jobs = self.schedule(True, m)
@@ -244,8 +249,10 @@ class Run:
         n = m.img_width * m.img_height * m.nfreqs
         res, rp = _res_array(n)
         info = Info()
+        cb = PROGRESS(lambda done, user: progress(done)) if progress else None
         check(lib().phx_dev_render(ctypes.byref(m), self.config.dump_file, precision, jobs.ctypes.data_as(P_int),
-                                   jobs.size, rp, ctypes.byref(info), None, None))
+                                   jobs.size, rp, ctypes.byref(info), ctypes.cast(cb, ctypes.c_void_p) if cb else None,
+                                   None))
         return res.reshape(m.img_width * m.img_height, m.nfreqs, 6), info.as_dict()
 
     def write_output(self, res: np.ndarray, outdir: str | None = None, image: str | None = None,

@@ -116,6 +116,22 @@ def test_c2_inclination60_vs_oracle(snapdir, tmp_path, oracle_lib):
         assert parity.passes(figs), parity.summary(figs)
 
 
+def test_one_shot_progress_counts_the_jobs_of_pixels(c1_run, monkeypatch):
+    """phx_dev_render in launches of PHX_CHUNK jobs with a progress callback (device::Progress): a
+    7 x 9 image at 9 frequencies marched in place has 3 chunks of frequencies, whose job list pads the
+    63 pixels of each chunk to a wave; the callbacks add up to 63 x 3 jobs, the padding left out,
+    and the image equals the unchunked render's (FP32: results do not depend on wave composition)."""
+    run = c1_run
+    rq = run.request(img_width=7, img_height=9, inclination=50.0, freqs=tuple(np.geomspace(86e9, 690e9, 9)))
+    monkeypatch.setenv("PHX_TWO_PHASE_MIN_NF", "0")
+    whole, _ = run.dev_render(capi.FP32, model=rq)
+    monkeypatch.setenv("PHX_CHUNK", "100")
+    calls = []
+    chunked, _ = run.dev_render(capi.FP32, model=rq, progress=calls.append)
+    assert len(calls) == 2 and sum(calls) == 63 * 3, calls
+    assert np.array_equal(chunked, whole)
+
+
 def test_resident_snapshot_reuse_and_partial_jobs(c1_run):
     snap = capi.Snapshot(c1_run, capi.TABLE_FP32 | capi.TABLE_FP64)
     rq = c1_run.request(img_width=32, img_height=32, inclination=30.0)
